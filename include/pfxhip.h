@@ -1,0 +1,292 @@
+/* pfxhip.h -- C ABI of the MI355X path tracer (libpfxhip.so).
+ *
+ * It replaces the still-render surface of the reference tracer,
+ * `pfx_trace::Trace` in crates/trace/src/gpu.rs, for a host that binds it
+ * over FFI. Plain C types only: pointers, sizes, fixed-layout structs.
+ *
+ * Ownership: every pfxh_* call copies what it needs out of caller memory
+ * before returning and never keeps a caller pointer. A pfxh_trace owns all
+ * of its device memory and one HIP stream. Calls on one trace must be
+ * serialised by the caller (the reference's `&mut self` methods); distinct
+ * traces may be used from distinct threads.
+ *
+ * Errors: every fallible call returns a pfxh_status and, when `err` is not
+ * NULL, writes a NUL-terminated message of at most `err_len` bytes. Build
+ * failures are split like `TraceError::{Refused, Setup}`
+ * (crates/trace/src/gpu.rs:268-307): PFXH_REFUSED names a device limit or
+ * memory shortfall, PFXH_SETUP an invalid scene or a failed device call.
+ */
+#ifndef PFXHIP_H
+#define PFXHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pfxh_status {
+    PFXH_OK = 0,
+    PFXH_REFUSED = 1, /* TraceError::Refused: limits or memory (gpu.rs:1254-1295) */
+    PFXH_SETUP = 2,   /* TraceError::Setup: invalid scene or device failure */
+    PFXH_INVALID = 3, /* runtime Err(String) of a setter or sampler call */
+    PFXH_DEVICE = 4   /* HIP runtime error during sampling or readback */
+} pfxh_status;
+
+/* ---- scene input (crates/trace/src/gpu.rs:249-256, detail.rs) ---------- */
+
+/* bvh::Triangle (crates/trace/src/bvh.rs:15-19): a flat world triangle. */
+typedef struct pfxh_triangle {
+    float vertices[9]; /* a.xyz, b.xyz, c.xyz */
+    uint32_t material;
+} pfxh_triangle;
+
+/* detail::TriangleDetail without uv sets: vertices plus per-vertex normals
+ * (all-zero normals mean "use the geometric normal"). */
+typedef struct pfxh_triangle_detail {
+    float vertices[9];
+    float normals[9];
+    uint32_t material;
+} pfxh_triangle_detail;
+
+/* A shared mesh: a range of pfxh_scene.mesh_triangles. Instances naming
+ * the same mesh share one bottom-level BVH, as instances sharing one
+ * `Arc<[TriangleDetail]>` do in crates/trace/src/levels.rs:266-283. */
+typedef struct pfxh_mesh {
+    uint32_t first;
+    uint32_t count;
+} pfxh_mesh;
+
+/* detail::Instance (crates/trace/src/detail.rs:53-60): a placed mesh.
+ * transform is column-major 4x4 (`Transform = [[f32; 4]; 4]`), affine. */
+typedef struct pfxh_instance {
+    uint32_t mesh;
+    uint32_t word; /* surface flags: 1 shadow-only, 2 casts no shadow, 1<<23 one-sided */
+    float transform[16];
+} pfxh_instance;
+
+/* The diffuse + GGX subset of pfx_materials::Material
+ * (crates/materials/src/material.rs:493-575) the tracer shades. */
+typedef struct pfxh_material {
+    float base[3];
+    float roughness;
+    float metalness;
+    float specular;
+    float clearcoat;
+    float clearcoat_roughness;
+    float sheen;
+    float emission[3];
+    float fresnel_power; /* <= 0 means 5 */
+} pfxh_material;
+
+/* lights::LocalLight (crates/trace/src/lights.rs:30-39), point and spot. */
+enum { PFXH_LIGHT_POINT = 0, PFXH_LIGHT_SPOT = 1 };
+typedef struct pfxh_light {
+    float position[3];
+    float colour[3];
+    float intensity;
+    float radius;
+    float range;
+    uint32_t shadow;
+    uint32_t kind;       /* PFXH_LIGHT_POINT or PFXH_LIGHT_SPOT */
+    float direction[3];  /* spot axis */
+    float inner_deg;
+    float outer_deg;
+} pfxh_light;
+
+/* gpu::Camera (crates/trace/src/gpu.rs:172-178) */
+typedef struct pfxh_camera {
+    float origin[3];
+    float forward[3];
+    float right[3];
+    float up[3];
+} pfxh_camera;
+
+/* gpu::Sun (crates/trace/src/gpu.rs:230-235) */
+typedef struct pfxh_sun {
+    float direction[3];
+    float color[3];
+    float intensity;
+} pfxh_sun;
+
+/* sky::Environment::Hdr (crates/trace/src/sky.rs:11-15): a lat-long
+ * RGBA32F texture, row-major, `width * height * 4` floats. width == 0 is a
+ * black sky (the sky slot is still sampled, as the reference always does). */
+typedef struct pfxh_environment {
+    uint32_t width;
+    uint32_t height;
+    const float* texels;
+} pfxh_environment;
+
+typedef struct pfxh_scene {
+    const pfxh_triangle* triangles;
+    uint32_t triangle_count;
+    const pfxh_triangle_detail* world_details; /* Detail::triangles */
+    uint32_t world_detail_count;
+    const pfxh_triangle_detail* mesh_triangles;
+    uint32_t mesh_triangle_count;
+    const pfxh_mesh* meshes;
+    uint32_t mesh_count;
+    const pfxh_instance* instances;
+    uint32_t instance_count;
+    const pfxh_material* materials;
+    uint32_t material_count;
+    const pfxh_light* lights;
+    uint32_t light_count;
+    pfxh_environment sky;
+    pfxh_camera camera;
+    pfxh_sun sun;
+} pfxh_scene;
+
+/* detail::Bounces (crates/trace/src/detail.rs:414-453) */
+typedef struct pfxh_bounces {
+    uint32_t total;
+    uint32_t diffuse;
+    uint32_t glossy;
+    uint32_t transmission;
+} pfxh_bounces;
+
+enum { PFXH_ACCUM_COMPENSATED = 0, PFXH_ACCUM_PLAIN = 1 };
+
+/* The parts of detail::Detail (crates/trace/src/detail.rs:374-398) the
+ * path uses. The sampler is PCG (`Sampler::Pcg`, sobol.rs:13-23). */
+typedef struct pfxh_detail {
+    pfxh_bounces bounces;
+    float lens_shift[2];
+    float aperture;
+    float focus_distance;
+    float sun_radius_deg;
+    uint32_t accumulate; /* PFXH_ACCUM_COMPENSATED (Neumaier) or PFXH_ACCUM_PLAIN */
+    uint32_t wave_paths; /* paths in flight per wave; 0 picks a default */
+} pfxh_detail;
+
+/* adaptive::Adaptive (crates/trace/src/adaptive.rs:9-26) */
+typedef struct pfxh_adaptive {
+    float threshold;
+    uint32_t min_samples;
+    uint32_t max_samples;
+    float growth;
+} pfxh_adaptive;
+
+/* adaptive::Converged + Round (crates/trace/src/adaptive.rs:66-83) */
+#define PFXH_MAX_ROUNDS 64
+typedef struct pfxh_converged {
+    uint32_t round_count;
+    uint32_t samples[PFXH_MAX_ROUNDS];
+    uint64_t active[PFXH_MAX_ROUNDS];
+    uint64_t pixels;
+} pfxh_converged;
+
+/* Probe (crates/trace/src/probe.rs:10-18) plus the hit primitive: the
+ * caller's triangle index (world triangles first, then world_details, then
+ * mesh_triangles by mesh-local index), or UINT32_MAX on a miss. */
+typedef struct pfxh_probe_hit {
+    float depth;
+    float distance;
+    uint32_t material_id; /* material + 1, 0 on a miss */
+    uint32_t primitive;
+    uint32_t instance;    /* 0 world, k for the k-th placed instance */
+    float normal[3];
+} pfxh_probe_hit;
+
+/* Counters the kernels keep, for throughput and roofline reports. */
+typedef struct pfxh_stats {
+    uint64_t closest_rays;  /* extension rays traced */
+    uint64_t shadow_rays;   /* any-hit rays traced */
+    uint64_t paths;         /* camera paths started */
+    uint64_t nodes_visited; /* filled when counting is on (pfxh_trace_set_counting) */
+    uint64_t prims_tested;
+    double trace_ms;        /* GPU time of closest-hit kernels (HIP events) */
+    double shadow_ms;       /* GPU time of any-hit kernels */
+    double shade_ms;        /* GPU time of raygen, shading and accumulation kernels */
+    uint32_t launches;      /* closest-hit launches behind trace_ms */
+} pfxh_stats;
+
+typedef struct pfxh_trace pfxh_trace;
+
+/* Trace::new_detailed (crates/trace/src/gpu.rs:1926-1934) */
+pfxh_status pfxh_trace_new(const pfxh_scene* scene, const pfxh_detail* detail, uint32_t width,
+                           uint32_t height, pfxh_trace** out, char* err, size_t err_len);
+/* Trace::sample (gpu.rs:2942-2949): `count` more samples with `seed`,
+ * accumulated onto the samples already taken. */
+pfxh_status pfxh_trace_sample(pfxh_trace* trace, uint32_t count, uint32_t seed, char* err,
+                              size_t err_len);
+/* Trace::sample_adaptive (crates/trace/src/adaptive.rs:112-324) */
+pfxh_status pfxh_trace_sample_adaptive(pfxh_trace* trace, const pfxh_adaptive* adaptive,
+                                       uint32_t seed, pfxh_converged* out, char* err,
+                                       size_t err_len);
+/* Trace::readback (gpu.rs:3236-3246): three RGBA32F images, row-major,
+ * width*height*4 floats each, caller-allocated. Any pointer may be NULL. */
+pfxh_status pfxh_trace_readback(pfxh_trace* trace, float* color, float* albedo, float* normal,
+                                char* err, size_t err_len);
+/* Trace::set_camera (gpu.rs:2817-2834); resets the sample count */
+pfxh_status pfxh_trace_set_camera(pfxh_trace* trace, const pfxh_camera* camera, char* err,
+                                  size_t err_len);
+/* Trace::set_window (gpu.rs:2836-2851); resets the sample count */
+pfxh_status pfxh_trace_set_window(pfxh_trace* trace, uint32_t full_width, uint32_t full_height,
+                                  uint32_t x, uint32_t y, char* err, size_t err_len);
+/* Trace::set_sun (gpu.rs:2912-2918); resets the sample count */
+pfxh_status pfxh_trace_set_sun(pfxh_trace* trace, const pfxh_sun* sun, char* err,
+                               size_t err_len);
+/* Trace::set_lens (gpu.rs:2858-2870); resets the sample count */
+pfxh_status pfxh_trace_set_lens(pfxh_trace* trace, float shift_x, float shift_y, float aperture,
+                                float focus_distance, char* err, size_t err_len);
+/* Trace::samples (gpu.rs:3232-3234) */
+uint32_t pfxh_trace_samples(const pfxh_trace* trace);
+/* Trace::gpu_bytes (gpu.rs:2360) */
+uint64_t pfxh_trace_gpu_bytes(const pfxh_trace* trace);
+/* Trace::probe (crates/trace/src/probe.rs:42; probe.wgsl): pixel-centre
+ * primary rays. `out` holds width*height records. */
+pfxh_status pfxh_trace_probe(pfxh_trace* trace, pfxh_probe_hit* out, char* err, size_t err_len);
+/* Kernel counters since the trace was built or the counters were reset. */
+pfxh_status pfxh_trace_stats(const pfxh_trace* trace, pfxh_stats* out);
+void pfxh_trace_reset_stats(pfxh_trace* trace);
+/* Count nodes visited and triangles tested per ray (slower; off by default). */
+void pfxh_trace_set_counting(pfxh_trace* trace, int on);
+/* Drop the trace and its device memory. NULL is allowed. */
+void pfxh_trace_free(pfxh_trace* trace);
+
+/* ---- host-only scene layout (no device needed) -------------------------- */
+
+/* bvh::Node (crates/trace/src/bvh.rs:5-13), 48 bytes */
+typedef struct pfxh_bvh_node {
+    float lo[4];
+    float hi[4];
+    uint32_t first;
+    uint32_t count;
+    uint32_t pad[2];
+} pfxh_bvh_node;
+
+/* Bvh::try_build (crates/trace/src/bvh.rs:95-133): binned SAH build.
+ * nodes needs room for 2*count nodes, order for count entries. */
+pfxh_status pfxh_layout_bvh(const pfxh_triangle* triangles, uint32_t count, pfxh_bvh_node* nodes,
+                            uint32_t* node_count, uint32_t* order, char* err, size_t err_len);
+
+/* levels::lay + SceneBuffers node buffer (crates/trace/src/levels.rs:225-383,
+ * 421-470): the reference's two-level node buffer and world order for a
+ * whole scene. Call once with nodes == NULL to learn the counts. */
+pfxh_status pfxh_layout_levels(const pfxh_scene* scene, pfxh_bvh_node* nodes,
+                               uint32_t* node_count, uint32_t* order, uint32_t* order_count,
+                               char* err, size_t err_len);
+
+/* tree::lay (crates/trace/src/tree.rs:220-243): light rows (16 floats
+ * each, `GpuLight`) for [non-treed][treed in tree order][tree nodes].
+ * rows needs room for 2*count+1 rows. */
+pfxh_status pfxh_layout_lights(const pfxh_light* lights, uint32_t count, float* rows,
+                               uint32_t* row_count, uint32_t* looped, uint32_t* tree, char* err,
+                               size_t err_len);
+
+/* lights::build_emitters (crates/trace/src/lights.rs:391-440) for the
+ * emissive triangles of a scene: entries (16 floats each, `GpuEmitter`,
+ * header first) and the CDF. Call with entries == NULL to learn the count. */
+pfxh_status pfxh_layout_emitters(const pfxh_scene* scene, float* entries, float* cdf,
+                                 uint32_t* emitter_count, char* err, size_t err_len);
+
+/* Library version string. */
+const char* pfxh_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PFXHIP_H */

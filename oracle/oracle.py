@@ -1,0 +1,162 @@
+"""ctypes wrapper of oracle/_build/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+`Oracle` follows the same calls as `pfx_amd.Trace` so parity tests read the
+same on both sides.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+_lib = None
+
+_P, _U, _F = C.c_void_p, C.c_uint32, C.c_float
+
+
+def build() -> Path:
+    """Compile the oracle with its Makefile (g++, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        h = C.CDLL(str(LIB))
+        sig = {
+            "or_scene_new": (_P, [_P, _P, _U, _U, C.c_char_p, C.c_size_t]),
+            "or_scene_free": (None, [_P]),
+            "or_set_camera": (None, [_P, _P]),
+            "or_set_window": (C.c_int, [_P, _U, _U, _U, _U]),
+            "or_sample": (None, [_P, _U, _U, _U, _U]),
+            "or_samples": (_U, [_P]),
+            "or_readback": (None, [_P, _P, _P, _P]),
+            "or_counters": (None, [_P, _P]),
+            "or_probe": (None, [_P, _P]),
+            "or_sample_adaptive": (C.c_int, [_P, _P, _U, _P]),
+            "or_tree_pick": (None, [_P, _P, _P, _F, _P]),
+            "or_light_meta": (_U, [_P, C.POINTER(_U), C.POINTER(_U)]),
+            "or_trace_counts": (None, [_P, _P, _P, _P]),
+            "or_pcg": (_U, [_U]),
+            "or_offset_axis": (_F, [_F, _F]),
+            "or_spawn_point": (None, [_P, _P, _P, _F, _P]),
+            "or_fm": (_F, [C.c_int, _F, _F]),
+            "or_fm_many": (None, [C.c_int, _P, _P, _P, C.c_size_t]),
+            "or_bvh": (_U, [_P, _U, _P, _P]),
+            "or_bvh_intersect": (_U, [_P, _U, _P, _U, _P, _P, _P]),
+            "or_triangle_hit": (_F, [_P, _P, _P, _F]),
+            "or_levels": (C.c_int, [_P, _P, C.POINTER(_U), _P, C.POINTER(_U), _P, C.c_char_p, C.c_size_t]),
+            "or_lights": (_U, [_P, _U, _P, C.POINTER(_U), C.POINTER(_U)]),
+            "or_light_pack": (None, [_P, _P]),
+            "or_light_falloff": (_F, [_P, _F]),
+            "or_light_cone": (_F, [_P, _P]),
+            "or_light_valid": (C.c_int, [_P]),
+            "or_emitters": (_U, [_P, _U, _U, _U, _P, _P]),
+            "or_sky_cdf": (None, [_P, _U, _U, _U, _U, _P, _P]),
+            "or_rounds": (_U, [_P, _P, _U]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+class Oracle:
+    """The CPU restatement with the `Trace` call surface."""
+
+    def __init__(self, scene, detail, width: int, height: int, threads: int | None = None):
+        from pfx_amd import _native as N  # struct layouts only
+
+        if threads is not None:
+            os.environ["OMP_NUM_THREADS"] = str(threads)
+        self._N = N
+        sc, keep = scene.to_c()
+        d = detail.to_c()
+        err = C.create_string_buffer(1024)
+        self._h = lib().or_scene_new(C.byref(sc), C.byref(d), width, height, err, len(err))
+        del keep
+        if not self._h:
+            raise ValueError(err.value.decode())
+        self.width, self.height = width, height
+
+    def sample(self, count: int, seed: int, rows: tuple | None = None) -> None:
+        r0, r1 = rows if rows else (0, 0)
+        lib().or_sample(self._h, count, seed, r0, r1)
+
+    def samples(self) -> int:
+        return int(lib().or_samples(self._h))
+
+    def readback(self):
+        shape = (self.height, self.width, 4)
+        c, a, n = (np.empty(shape, np.float32) for _ in range(3))
+        lib().or_readback(self._h, c.ctypes.data, a.ctypes.data, n.ctypes.data)
+        return c, a, n
+
+    def set_camera(self, camera) -> None:
+        cc = camera.to_c()
+        lib().or_set_camera(self._h, C.byref(cc))
+
+    def set_window(self, full, origin) -> None:
+        if lib().or_set_window(self._h, full[0], full[1], origin[0], origin[1]):
+            raise ValueError("the window must lie inside the full frame")
+
+    def probe(self) -> np.ndarray:
+        hits = np.zeros(self.width * self.height, dtype=self._N.PROBE_HIT)
+        lib().or_probe(self._h, hits.ctypes.data)
+        return hits.reshape(self.height, self.width)
+
+    def sample_adaptive(self, adaptive, seed: int):
+        out = self._N.ConvergedC()
+        a = adaptive.to_c()
+        if lib().or_sample_adaptive(self._h, C.byref(a), seed, C.byref(out)):
+            raise ValueError("invalid adaptive settings")
+        return [(int(out.samples[i]), int(out.active[i])) for i in range(out.round_count)], int(out.pixels)
+
+    def counters(self) -> dict:
+        out = np.zeros(3, np.uint64)
+        lib().or_counters(self._h, out.ctypes.data)
+        return {"closest_rays": int(out[0]), "shadow_rays": int(out[1]), "paths": int(out[2])}
+
+    def tree_pick(self, p, n, u: float):
+        p = np.asarray(p, np.float32)
+        n = np.asarray(n, np.float32)
+        out = np.zeros(2, np.float32)
+        lib().or_tree_pick(self._h, p.ctypes.data, n.ctypes.data, u, out.ctypes.data)
+        return float(out[0]), float(out[1])
+
+    def light_meta(self):
+        looped, tree = _U(), _U()
+        rows = lib().or_light_meta(self._h, C.byref(looped), C.byref(tree))
+        return int(rows), int(looped.value), int(tree.value)
+
+    def trace_counts(self, o, d):
+        o = np.asarray(o, np.float32)
+        d = np.asarray(d, np.float32)
+        out = np.zeros(2, np.uint64)
+        lib().or_trace_counts(self._h, o.ctypes.data, d.ctypes.data, out.ctypes.data)
+        return int(out[0]), int(out[1])
+
+    def close(self):
+        if self._h:
+            lib().or_scene_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,213 @@
+"""ctypes view of include/pfxhip.h and the loader for libpfxhip.so.
+
+The structs below mirror the C header field for field; numpy structured
+dtypes with the same layout let scene arrays pass straight through.
+There is no fallback: if the library is missing, importing the tracer fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libpfxhip.so"
+
+# ---- numpy layouts of the scene arrays ---------------------------------------
+TRIANGLE = np.dtype([("vertices", "<f4", (9,)), ("material", "<u4")])
+TRIANGLE_DETAIL = np.dtype([("vertices", "<f4", (9,)), ("normals", "<f4", (9,)), ("material", "<u4")])
+MESH = np.dtype([("first", "<u4"), ("count", "<u4")])
+INSTANCE = np.dtype([("mesh", "<u4"), ("word", "<u4"), ("transform", "<f4", (16,))])
+MATERIAL = np.dtype(
+    [
+        ("base", "<f4", (3,)),
+        ("roughness", "<f4"),
+        ("metalness", "<f4"),
+        ("specular", "<f4"),
+        ("clearcoat", "<f4"),
+        ("clearcoat_roughness", "<f4"),
+        ("sheen", "<f4"),
+        ("emission", "<f4", (3,)),
+        ("fresnel_power", "<f4"),
+    ]
+)
+LIGHT = np.dtype(
+    [
+        ("position", "<f4", (3,)),
+        ("colour", "<f4", (3,)),
+        ("intensity", "<f4"),
+        ("radius", "<f4"),
+        ("range", "<f4"),
+        ("shadow", "<u4"),
+        ("kind", "<u4"),
+        ("direction", "<f4", (3,)),
+        ("inner_deg", "<f4"),
+        ("outer_deg", "<f4"),
+    ]
+)
+PROBE_HIT = np.dtype(
+    [
+        ("depth", "<f4"),
+        ("distance", "<f4"),
+        ("material_id", "<u4"),
+        ("primitive", "<u4"),
+        ("instance", "<u4"),
+        ("normal", "<f4", (3,)),
+    ]
+)
+BVH_NODE = np.dtype(
+    [("lo", "<f4", (4,)), ("hi", "<f4", (4,)), ("first", "<u4"), ("count", "<u4"), ("pad", "<u4", (2,))]
+)
+assert TRIANGLE.itemsize == 40 and TRIANGLE_DETAIL.itemsize == 76 and INSTANCE.itemsize == 72
+assert MATERIAL.itemsize == 52 and LIGHT.itemsize == 64 and PROBE_HIT.itemsize == 32 and BVH_NODE.itemsize == 48
+
+LIGHT_POINT, LIGHT_SPOT = 0, 1
+ACCUM_COMPENSATED, ACCUM_PLAIN, ACCUM_FP64 = 0, 1, 2
+
+OK, REFUSED, SETUP, INVALID, DEVICE = 0, 1, 2, 3, 4
+MAX_ROUNDS = 64
+
+
+class Environment(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("texels", C.c_void_p)]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("origin", C.c_float * 3),
+        ("forward", C.c_float * 3),
+        ("right", C.c_float * 3),
+        ("up", C.c_float * 3),
+    ]
+
+
+class Sun(C.Structure):
+    _fields_ = [("direction", C.c_float * 3), ("color", C.c_float * 3), ("intensity", C.c_float)]
+
+
+class SceneC(C.Structure):
+    _fields_ = [
+        ("triangles", C.c_void_p),
+        ("triangle_count", C.c_uint32),
+        ("world_details", C.c_void_p),
+        ("world_detail_count", C.c_uint32),
+        ("mesh_triangles", C.c_void_p),
+        ("mesh_triangle_count", C.c_uint32),
+        ("meshes", C.c_void_p),
+        ("mesh_count", C.c_uint32),
+        ("instances", C.c_void_p),
+        ("instance_count", C.c_uint32),
+        ("materials", C.c_void_p),
+        ("material_count", C.c_uint32),
+        ("lights", C.c_void_p),
+        ("light_count", C.c_uint32),
+        ("sky", Environment),
+        ("camera", Camera),
+        ("sun", Sun),
+    ]
+
+
+class Bounces(C.Structure):
+    _fields_ = [
+        ("total", C.c_uint32),
+        ("diffuse", C.c_uint32),
+        ("glossy", C.c_uint32),
+        ("transmission", C.c_uint32),
+    ]
+
+
+class DetailC(C.Structure):
+    _fields_ = [
+        ("bounces", Bounces),
+        ("lens_shift", C.c_float * 2),
+        ("aperture", C.c_float),
+        ("focus_distance", C.c_float),
+        ("sun_radius_deg", C.c_float),
+        ("accumulate", C.c_uint32),
+        ("wave_paths", C.c_uint32),
+    ]
+
+
+class AdaptiveC(C.Structure):
+    _fields_ = [
+        ("threshold", C.c_float),
+        ("min_samples", C.c_uint32),
+        ("max_samples", C.c_uint32),
+        ("growth", C.c_float),
+    ]
+
+
+class ConvergedC(C.Structure):
+    _fields_ = [
+        ("round_count", C.c_uint32),
+        ("samples", C.c_uint32 * MAX_ROUNDS),
+        ("active", C.c_uint64 * MAX_ROUNDS),
+        ("pixels", C.c_uint64),
+    ]
+
+
+class StatsC(C.Structure):
+    _fields_ = [
+        ("closest_rays", C.c_uint64),
+        ("shadow_rays", C.c_uint64),
+        ("paths", C.c_uint64),
+        ("nodes_visited", C.c_uint64),
+        ("prims_tested", C.c_uint64),
+        ("trace_ms", C.c_double),
+        ("shadow_ms", C.c_double),
+        ("shade_ms", C.c_double),
+        ("launches", C.c_uint32),
+    ]
+
+
+# Every symbol include/pfxhip.h declares, with its ctypes signature.
+_P = C.c_void_p
+_U = C.c_uint32
+_S = C.c_char_p
+_Z = C.c_size_t
+SIGNATURES = {
+    "pfxh_trace_new": (C.c_int, [_P, _P, _U, _U, C.POINTER(_P), _S, _Z]),
+    "pfxh_trace_sample": (C.c_int, [_P, _U, _U, _S, _Z]),
+    "pfxh_trace_sample_adaptive": (C.c_int, [_P, _P, _U, _P, _S, _Z]),
+    "pfxh_trace_readback": (C.c_int, [_P, _P, _P, _P, _S, _Z]),
+    "pfxh_trace_set_camera": (C.c_int, [_P, _P, _S, _Z]),
+    "pfxh_trace_set_window": (C.c_int, [_P, _U, _U, _U, _U, _S, _Z]),
+    "pfxh_trace_set_sun": (C.c_int, [_P, _P, _S, _Z]),
+    "pfxh_trace_set_lens": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_float, _S, _Z]),
+    "pfxh_trace_samples": (_U, [_P]),
+    "pfxh_trace_gpu_bytes": (C.c_uint64, [_P]),
+    "pfxh_trace_probe": (C.c_int, [_P, _P, _S, _Z]),
+    "pfxh_trace_stats": (C.c_int, [_P, _P]),
+    "pfxh_trace_reset_stats": (None, [_P]),
+    "pfxh_trace_set_counting": (None, [_P, C.c_int]),
+    "pfxh_trace_free": (None, [_P]),
+    "pfxh_layout_bvh": (C.c_int, [_P, _U, _P, C.POINTER(_U), _P, _S, _Z]),
+    "pfxh_layout_levels": (C.c_int, [_P, _P, C.POINTER(_U), _P, C.POINTER(_U), _S, _Z]),
+    "pfxh_layout_lights": (C.c_int, [_P, _U, _P, C.POINTER(_U), C.POINTER(_U), C.POINTER(_U), _S, _Z]),
+    "pfxh_layout_emitters": (C.c_int, [_P, _P, _P, C.POINTER(_U), _S, _Z]),
+    "pfxh_version": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libpfxhip.so (built in-tree by pfx_amd.build); fail loudly if absent."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m pfx_amd.build` (there is no CPU fallback)"
+            )
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def err_buffer() -> C.Array:
+    return C.create_string_buffer(1024)

@@ -1,0 +1,102 @@
+// Kernel parameter blocks and launch entry points of the wavefront tracer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_types.h"
+
+namespace pfxh {
+
+// Scene constants and read-only device buffers (passed by value).
+struct SceneDev {
+    const WideNode* nodes;
+    const TriPos* tris;
+    const TriAttr* attrs;
+    const Group* groups;
+    const Surface* surfaces;
+    const MatDev* materials;
+    const float4* lights;    // GpuLight rows, 4 float4 each
+    const float4* emitters;  // GpuEmitter rows, 4 float4 each
+    const float* cdf;
+    const float* sky_rows;
+    const float* sky_columns;
+    const float* env;
+    const uint32_t* slot_prim;
+    const uint32_t* record_instance;
+    RootEntry root;
+    uint32_t env_w, env_h, env_black, lit;
+    uint32_t looped, tree, emitter_count, light_slots;
+    uint32_t entries;  // NEE entries a vertex may queue
+    uint32_t total, diffuse, glossy, transmission;
+    float sun_dir[3], sun_color[3], sun_intensity, sun_w;
+    float origin[3], forward[3], right[3], up[3], lens[4];
+    uint32_t window[4];
+    uint32_t width, height;
+};
+
+// One wave of paths: SoA state in HBM.
+struct WaveDev {
+    float4* o;      // origin.xyz, previous pdf
+    float4* d;      // direction.xyz, lobe counters (bits)
+    float4* thr;    // throughput.xyz, rng state (bits)
+    float4* res;    // result.xyz
+    float4* alb;    // first albedo
+    float4* nrm;    // first normal
+    float4* hit;    // t, u, v, slot (bits)
+    uint32_t* hit_rec;
+    float4* e_o;    // per entry: shadow origin.xyz, limit
+    float4* e_d;    // per entry: direction.xyz
+    float4* e_c;    // per entry: contribution.xyz
+    uint32_t* q_in;
+    uint32_t* q_next;
+    uint32_t* q_sh;
+    uint32_t* counters;  // [0] next, [1] shadow, [2..] stats
+    uint2* spill;        // traversal stack overflow
+    uint32_t capacity;   // paths per wave (entry arrays are strided by it)
+    uint32_t spill_threads;
+};
+
+// Accumulation buffers (per pixel).
+struct AccumDev {
+    float4* sum;   // color (w = count), albedo, normal: 3 float4 per pixel
+    float4* comp;  // Neumaier compensation, 3 float4 per pixel
+    double* sum64; // 12 doubles per pixel in FP64 mode
+    uint32_t mode;
+};
+
+enum Counter : uint32_t {
+    C_NEXT = 0,
+    C_SHADOW = 1,
+    C_CLOSEST_RAYS = 2,  // 64-bit pairs from here on
+    C_SHADOW_RAYS = 4,
+    C_NODES = 6,
+    C_PRIMS = 8,
+    C_RUNNING = 10,
+    C_ACTIVE = 11,
+    C_COUNT = 12
+};
+
+constexpr int TRACE_BLOCK = 128;
+constexpr int STACK_SHORT = 12;
+constexpr int STACK_DEPTH = 64;
+
+void launch_camera(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* active, uint32_t npix,
+                   uint32_t nsamples, uint32_t acc, uint32_t seed);
+void launch_extend(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* queue, uint32_t n,
+                   bool count, uint32_t grid);
+void launch_shade(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* queue, uint32_t n,
+                  uint32_t bounce);
+void launch_shadow(hipStream_t s, const SceneDev& S, const WaveDev& W, uint32_t n, bool count, uint32_t grid);
+void launch_finish(hipStream_t s, const WaveDev& W, const AccumDev& A, const uint32_t* active, uint32_t npix,
+                   uint32_t nsamples, float count_after);
+void launch_develop(hipStream_t s, const AccumDev& A, uint32_t pixels, float4* color, float4* albedo,
+                    float4* normal);
+void launch_clear(hipStream_t s, const AccumDev& A, uint32_t pixels);
+void launch_active(hipStream_t s, const AccumDev& A, uint32_t pixels, uint32_t* active, uint32_t* counter);
+void launch_probe(hipStream_t s, const SceneDev& S, const WaveDev& W, void* out);
+void launch_measure(hipStream_t s, const AccumDev& A, float4* snaps, uint32_t width, uint32_t height);
+void launch_mark(hipStream_t s, const AccumDev& A, const float4* snaps, uint32_t width, uint32_t height,
+                 uint32_t min_samples, float threshold, uint32_t* running);
+void launch_zero(hipStream_t s, uint32_t* p, uint32_t n);
+
+}  // namespace pfxh

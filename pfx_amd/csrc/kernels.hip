@@ -1,0 +1,1388 @@
+// Wavefront path tracer for gfx950 (MI355X).
+//
+// One wave of paths lives in HBM as structure-of-arrays state. Each bounce
+// runs three kernels over compacted queues:
+//   extend : closest-hit traversal of every live path (persistent blocks,
+//            LDS short stack, 64-byte two-child nodes)
+//   shade  : emission, next-event candidates, BSDF sampling, Russian
+//            roulette; appends survivors to the next queue and paths with
+//            light contributions to the shadow queue
+//   shadow : any-hit traversal of each queued path's shadow rays, adding the
+//            unoccluded contributions in the reference's slot order
+// followed, per wave, by `finish`, which folds each pixel's samples into the
+// accumulator in sample order.
+//
+// The per-vertex arithmetic restates crates/trace/src/trace.wgsl `radiance`
+// (:2153-2502) and crates/trace/src/lights.wgsl with the same operation
+// order; everything is built with -ffp-contract=off and IEEE div/sqrt.
+#include <hip/hip_runtime.h>
+
+#include "../../include/pfxhip.h"
+#include "kernels.h"
+#include "vec3.h"
+
+namespace pfxh {
+
+using pfx::basis;
+using pfx::dot;
+using pfx::fm_acos;
+using pfx::fm_clamp;
+using pfx::fm_cos;
+using pfx::fm_pow;
+using pfx::fm_sin;
+using pfx::m3;
+using pfx::mk;
+using pfx::normalize;
+using pfx::v3;
+
+#define DEV __device__ __forceinline__
+
+constexpr float PI = 3.141592653589793f;
+constexpr uint32_t ONE_SIDED = 8388608u;
+constexpr uint32_t NO_MATERIAL = 0xffffffffu;
+constexpr uint32_t LAST_BIT = 1u << 24;
+
+DEV v3 f3(float4 a) { return mk(a.x, a.y, a.z); }
+DEV float4 f4(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+DEV uint32_t fbits(float x) { return __float_as_uint(x); }
+DEV float ubits(uint32_t x) { return __uint_as_float(x); }
+
+// ---- wave-aggregated queue append ---------------------------------------------
+DEV uint32_t wave_append(uint32_t* counter, bool pred, uint32_t amount_if = 1) {
+    (void)amount_if;
+    unsigned long long mask = __ballot(pred);
+    uint32_t lane = __lane_id();
+    unsigned long long below = mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+    uint32_t rank = (uint32_t)__popcll(below);
+    uint32_t total = (uint32_t)__popcll(mask);
+    if (total == 0) return 0;
+    int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, total);
+    base = (uint32_t)__shfl((int)base, leader);
+    return base + rank;
+}
+
+// ---- traversal ------------------------------------------------------------------
+// The reference's box_hit (trace.wgsl:453-468): slab test with divisions;
+// `nr` is the entry distance, which decides a deferred pop exactly as the
+// reference's pop-time test does (it passes iff nr <= the current t).
+DEV bool slab(float lx, float ly, float lz, float hx, float hy, float hz, v3 o, v3 d, float limit, float& nr) {
+    float near_ = 0.0f, far_ = limit;
+    if (fabsf(d.x) < 1e-12f) {
+        if (o.x < lx || o.x > hx) return false;
+    } else {
+        float a = (lx - o.x) / d.x, b = (hx - o.x) / d.x;
+        near_ = fmaxf(near_, fminf(a, b));
+        far_ = fminf(far_, fmaxf(a, b));
+        if (near_ > far_) return false;
+    }
+    if (fabsf(d.y) < 1e-12f) {
+        if (o.y < ly || o.y > hy) return false;
+    } else {
+        float a = (ly - o.y) / d.y, b = (hy - o.y) / d.y;
+        near_ = fmaxf(near_, fminf(a, b));
+        far_ = fminf(far_, fmaxf(a, b));
+        if (near_ > far_) return false;
+    }
+    if (fabsf(d.z) < 1e-12f) {
+        if (o.z < lz || o.z > hz) return false;
+    } else {
+        float a = (lz - o.z) / d.z, b = (hz - o.z) / d.z;
+        near_ = fmaxf(near_, fminf(a, b));
+        far_ = fminf(far_, fmaxf(a, b));
+        if (near_ > far_) return false;
+    }
+    nr = near_;
+    return true;
+}
+
+// trace.wgsl:449-452 to_object: rows . (p, w), summed x, y, z, w
+DEV v3 to_object(const float* r, v3 p, float w) {
+    return mk(r[0] * p.x + r[1] * p.y + r[2] * p.z + r[3] * w, r[4] * p.x + r[5] * p.y + r[6] * p.z + r[7] * w,
+              r[8] * p.x + r[9] * p.y + r[10] * p.z + r[11] * w);
+}
+
+struct Stack {
+    uint2* lds;    // this lane's column in the block's LDS stack
+    uint2* spill;  // this lane's column in the global overflow area
+    uint32_t spill_stride;
+    int top;
+    DEV void push(uint32_t ref, float nr) {
+        uint2 e = make_uint2(ref, fbits(nr));
+        if (top < STACK_SHORT)
+            lds[top * TRACE_BLOCK] = e;
+        else
+            spill[(size_t)(top - STACK_SHORT) * spill_stride] = e;
+        ++top;
+    }
+    DEV uint2 pop() {
+        --top;
+        return top < STACK_SHORT ? lds[top * TRACE_BLOCK] : spill[(size_t)(top - STACK_SHORT) * spill_stride];
+    }
+};
+
+struct Closest {
+    float t, u, v;
+    uint32_t slot, rec;
+};
+
+// Closest hit (trace.wgsl:999-1075 `scene`, with tri_hit :780-804). Pops
+// follow the reference's left-first depth-first order, so equal-distance
+// ties resolve to the same triangle.
+template <bool COUNT>
+DEV Closest trace_closest(const SceneDev& S, v3 o, v3 d, float tmax, Stack& st, uint32_t& nodes_ctr,
+                          uint32_t& prims_ctr) {
+    Closest h{tmax, 0.0f, 0.0f, 0xffffffffu, 0u};
+    float nr;
+    if (S.root.ref == REF_NONE ||
+        !slab(S.root.lo[0], S.root.lo[1], S.root.lo[2], S.root.hi[0], S.root.hi[1], S.root.hi[2], o, d, h.t, nr))
+        return h;
+    v3 ro = o, rd = d;
+    uint32_t base = 0, rec = 0, word = 0;
+    float inv_det = 1.0f;
+    int inst_top = -1;
+    st.top = 0;
+    uint32_t cur = S.root.ref;
+    for (;;) {
+        if ((cur & (REF_LEAF | REF_INST)) == 0u) {
+            const float4* np = reinterpret_cast<const float4*>(S.nodes + cur);
+            float4 a = np[0], b = np[1], c = np[2], e = np[3];
+            if (COUNT) nodes_ctr++;
+            float n0, n1;
+            bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, ro, rd, h.t, n0);
+            bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, ro, rd, h.t, n1);
+            if (h0) {
+                if (h1) st.push(fbits(e.y), n1);
+                cur = fbits(e.x);
+                continue;
+            }
+            if (h1) {
+                cur = fbits(e.y);
+                continue;
+            }
+        } else if (cur & REF_LEAF) {
+            uint32_t first = base + (cur & REF_FIRST_MASK);
+            uint32_t count = ((cur >> 27) & 15u) + 1u;
+            for (uint32_t j = 0; j < count; ++j) {
+                uint32_t i = first + j;
+                if (COUNT) prims_ctr++;
+                if ((word & 1u) != 0u) continue;  // shadow-only surfaces are skipped by camera rays
+                const float4* tp = reinterpret_cast<const float4*>(S.tris + i);
+                float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                v3 A = mk(t0.x, t0.y, t0.z), e1 = mk(t0.w, t1.x, t1.y), e2 = mk(t1.z, t1.w, t2.x);
+                v3 p = pfx::cross(rd, e2);
+                float det = dot(e1, p);
+                float seen = det * inv_det;
+                if (fabsf(seen) < 1e-9f) continue;
+                if ((word & ONE_SIDED) != 0u && seen < 0.0f) continue;
+                float inv = 1.0f / det;
+                v3 s = ro - A;
+                float u = dot(s, p) * inv;
+                if (u < 0.0f || u > 1.0f) continue;
+                v3 q = pfx::cross(s, e1);
+                float v = dot(rd, q) * inv;
+                if (v < 0.0f || u + v > 1.0f) continue;
+                float t = dot(e2, q) * inv;
+                if (t > 0.0001f && t < h.t) {
+                    h.t = t;
+                    h.u = u;
+                    h.v = v;
+                    h.slot = i;
+                    h.rec = rec;
+                }
+            }
+        } else {
+            const Group& g = S.groups[cur & REF_INDEX_MASK];
+            base = g.base;
+            rec = g.record;
+            word = g.word;
+            inv_det = g.inv_det;
+            if (g.has_xform) {
+                ro = to_object(g.inv, o, 1.0f);
+                rd = to_object(g.inv, d, 0.0f);
+            }
+            inst_top = st.top;
+            if (slab(g.lo[0], g.lo[1], g.lo[2], g.hi[0], g.hi[1], g.hi[2], ro, rd, h.t, nr)) {
+                cur = g.root;
+                continue;
+            }
+        }
+        bool found = false;
+        for (;;) {
+            if (st.top == inst_top) {
+                ro = o;
+                rd = d;
+                base = 0;
+                rec = 0;
+                word = 0;
+                inv_det = 1.0f;
+                inst_top = -1;
+            }
+            if (st.top == 0) break;
+            uint2 e = st.pop();
+            if (ubits(e.y) <= h.t) {
+                cur = e.x;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return h;
+}
+
+// Any hit (trace.wgsl:1096-1168 `occluded`, tri_blocks :1076-1095).
+template <bool COUNT>
+DEV bool trace_any(const SceneDev& S, v3 o, v3 d, float limit, Stack& st, uint32_t& nodes_ctr, uint32_t& prims_ctr) {
+    float nr;
+    if (S.root.ref == REF_NONE ||
+        !slab(S.root.lo[0], S.root.lo[1], S.root.lo[2], S.root.hi[0], S.root.hi[1], S.root.hi[2], o, d, limit, nr))
+        return false;
+    v3 ro = o, rd = d;
+    uint32_t base = 0, word = 0;
+    float inv_det = 1.0f;
+    int inst_top = -1;
+    st.top = 0;
+    uint32_t cur = S.root.ref;
+    for (;;) {
+        if ((cur & (REF_LEAF | REF_INST)) == 0u) {
+            const float4* np = reinterpret_cast<const float4*>(S.nodes + cur);
+            float4 a = np[0], b = np[1], c = np[2], e = np[3];
+            if (COUNT) nodes_ctr++;
+            float n0, n1;
+            bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, ro, rd, limit, n0);
+            bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, ro, rd, limit, n1);
+            if (h0) {
+                if (h1) st.push(fbits(e.y), n1);
+                cur = fbits(e.x);
+                continue;
+            }
+            if (h1) {
+                cur = fbits(e.y);
+                continue;
+            }
+        } else if (cur & REF_LEAF) {
+            if ((word & 2u) == 0u) {
+                uint32_t first = base + (cur & REF_FIRST_MASK);
+                uint32_t count = ((cur >> 27) & 15u) + 1u;
+                for (uint32_t j = 0; j < count; ++j) {
+                    uint32_t i = first + j;
+                    if (COUNT) prims_ctr++;
+                    const float4* tp = reinterpret_cast<const float4*>(S.tris + i);
+                    float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                    v3 A = mk(t0.x, t0.y, t0.z), e1 = mk(t0.w, t1.x, t1.y), e2 = mk(t1.z, t1.w, t2.x);
+                    v3 p = pfx::cross(rd, e2);
+                    float det = dot(e1, p);
+                    if (fabsf(det * inv_det) < 1e-9f) continue;
+                    float inv = 1.0f / det;
+                    v3 s = ro - A;
+                    float u = dot(s, p) * inv;
+                    if (u < 0.0f || u > 1.0f) continue;
+                    v3 q = pfx::cross(s, e1);
+                    float v = dot(rd, q) * inv;
+                    if (v < 0.0f || u + v > 1.0f) continue;
+                    float t = dot(e2, q) * inv;
+                    if (t <= 0.0001f || t >= limit) continue;
+                    return true;
+                }
+            }
+        } else {
+            const Group& g = S.groups[cur & REF_INDEX_MASK];
+            base = g.base;
+            word = g.word;
+            inv_det = g.inv_det;
+            if (g.has_xform) {
+                ro = to_object(g.inv, o, 1.0f);
+                rd = to_object(g.inv, d, 0.0f);
+            }
+            inst_top = st.top;
+            if (slab(g.lo[0], g.lo[1], g.lo[2], g.hi[0], g.hi[1], g.hi[2], ro, rd, limit, nr)) {
+                cur = g.root;
+                continue;
+            }
+        }
+        bool found = false;
+        for (;;) {
+            if (st.top == inst_top) {
+                ro = o;
+                rd = d;
+                base = 0;
+                word = 0;
+                inv_det = 1.0f;
+                inst_top = -1;
+            }
+            if (st.top == 0) break;
+            uint2 e = st.pop();
+            (void)e.y;  // the limit never shrinks: every pushed entry is still live
+            cur = e.x;
+            found = true;
+            break;
+        }
+        if (!found) break;
+    }
+    return false;
+}
+
+// ---- surface at a hit (trace.wgsl:754-779 tri_fill, :681-694 world_hit) -----
+struct Surf {
+    v3 n, geometric;
+    uint32_t material;
+};
+DEV Surf surface_at(const SceneDev& S, uint32_t slot, uint32_t rec, float u, float v) {
+    const float4* tp = reinterpret_cast<const float4*>(S.tris + slot);
+    float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+    v3 e1 = mk(t0.w, t1.x, t1.y), e2 = mk(t1.z, t1.w, t2.x);
+    const float4* ap = reinterpret_cast<const float4*>(S.attrs + slot);
+    float4 a0 = ap[0], a1 = ap[1], a2 = ap[2];
+    Surf s;
+    v3 geometric = normalize(pfx::cross(e1, e2));
+    float w = (1.0f - u) - v;
+    v3 shaded = f3(a0) * w + f3(a1) * u + f3(a2) * v;
+    v3 n;
+    if (dot(shaded, shaded) > 1e-12f)
+        n = normalize(shaded);
+    else
+        n = rec > 0u ? mk(0.0f, 0.0f, 0.0f) : geometric;
+    const Surface& sf = S.surfaces[rec];
+    s.material = sf.material == NO_MATERIAL ? fbits(a0.w) : sf.material;
+    if (rec == 0u) {
+        s.geometric = geometric;
+        s.n = n;
+        return s;
+    }
+    float mirror = sf.inv_det < 0.0f ? -1.0f : 1.0f;
+    v3 i0 = mk(sf.inv[0], sf.inv[1], sf.inv[2]), i1 = mk(sf.inv[4], sf.inv[5], sf.inv[6]),
+       i2 = mk(sf.inv[8], sf.inv[9], sf.inv[10]);
+    s.geometric = normalize(i0 * geometric.x + i1 * geometric.y + i2 * geometric.z) * mirror;
+    v3 sh = i0 * n.x + i1 * n.y + i2 * n.z;
+    s.n = pfx::all_zero(n) ? s.geometric : normalize(sh);
+    return s;
+}
+
+// ---- BRDF (crates/materials/src/wgsl/brdf.wgsl) -------------------------------
+struct Mat {
+    v3 base;
+    float roughness, metalness, specular, clearcoat, clearcoat_roughness, sheen, fresnel_power;
+    v3 emission;
+};
+DEV Mat load_mat(const SceneDev& S, uint32_t m) {
+    const float4* p = reinterpret_cast<const float4*>(S.materials + m);
+    float4 a = p[0], b = p[1], c = p[2], e = p[3];
+    return Mat{mk(a.x, a.y, a.z), a.w, b.x, b.y, b.z, b.w, c.x, c.y, mk(e.x, e.y, e.z)};
+}
+DEV v3 mix3(v3 a, v3 b, float t) { return a * (1.0f - t) + b * t; }
+DEV v3 schlick(v3 f0, float cos_theta, float power) {
+    float k = fm_pow(1.0f - fm_clamp(cos_theta, 0.0f, 1.0f), power);
+    return mix3(f0, mk(1, 1, 1), k);
+}
+DEV float ggx_d(float nh, float a2) {
+    float n = fmaxf(nh, 0.0f);
+    float k = n * n * (a2 - 1.0f) + 1.0f;
+    return a2 / (PI * k * k);
+}
+DEV float smith_g1(float ndot, float a2) {
+    float n = fmaxf(ndot, 0.0f);
+    return 2.0f * n / (n + sqrtf(a2 + (1.0f - a2) * n * n));
+}
+DEV float alpha_rough(float r) {
+    float rough = fm_clamp(r, 0.02f, 1.0f);
+    return rough * rough;
+}
+DEV float fresnel_exponent(const Mat& s) { return s.fresnel_power <= 0.0f ? 5.0f : s.fresnel_power; }
+DEV v3 reflection(const Mat& s, float ndh_cos) {
+    float power = fresnel_exponent(s);
+    v3 f0 = mix3(mk(s.specular, s.specular, s.specular), s.base, fm_clamp(s.metalness, 0.0f, 1.0f));
+    if (f0.x <= 0.0f && f0.y <= 0.0f && f0.z <= 0.0f) return mk(0, 0, 0);
+    return schlick(f0, ndh_cos, power);
+}
+DEV v3 ggx_spec(float ndv, float ndl, float ndh, float a2, v3 fresnel) {
+    return fresnel * ggx_d(ndh, a2) * (smith_g1(ndv, a2) * smith_g1(ndl, a2)) / (4.0f * ndv * ndl);
+}
+DEV v3 eval(const Mat& s, v3 wo, v3 wi) {
+    float ndv = wo.z;
+    if (ndv <= 1e-5f) return mk(0, 0, 0);
+    float ndl = wi.z;
+    if (fabsf(ndl) <= 1e-8f) return mk(0, 0, 0);
+    float metal = fm_clamp(s.metalness, 0.0f, 1.0f);
+    float sheen = fm_clamp(s.sheen, 0.0f, 1.0f);
+    float sheen_cover = sheen * fm_pow(1.0f - ndv, 4.0f);
+    float view = fm_clamp(ndv, 0.0f, 1.0f);
+    if (ndl < 0.0f) return mk(0, 0, 0);
+    v3 h = normalize(wo + wi);
+    float ndh = fmaxf(h.z, 0.0f);
+    float vdh = fmaxf(dot(wo, h), 0.0f);
+    v3 fresnel = reflection(s, vdh);
+    float a = alpha_rough(s.roughness);
+    v3 spec = ggx_spec(ndv, ndl, ndh, a * a, fresnel);
+    v3 transmitted = mk(1, 1, 1) - fresnel;
+    float body = (1.0f - metal) * 1.0f * 1.0f;
+    float diffuse_w = body * (1.0f - sheen_cover);
+    float sheen_w = body * sheen_cover * fm_pow(1.0f - ndh, 5.0f);
+    v3 colour = spec + s.base * transmitted * (diffuse_w + sheen_w) / PI;
+    float coat = fm_clamp(s.clearcoat, 0.0f, 1.0f);
+    if (coat > 0.0f) {
+        float coat_a = alpha_rough(s.clearcoat_roughness);
+        v3 coat_f = schlick(mk(0.04f, 0.04f, 0.04f), vdh, fresnel_exponent(s));
+        v3 coat_spec = ggx_spec(ndv, ndl, ndh, coat_a * coat_a, coat_f);
+        float fc = schlick(mk(0.04f, 0.04f, 0.04f), view, fresnel_exponent(s)).x;
+        colour = coat_spec * coat + colour * (1.0f - coat * fc);
+    }
+    return colour;
+}
+DEV float coat_weight(const Mat& s, v3 wo) {
+    return fm_clamp(s.clearcoat * (0.12f + 0.88f * (0.04f + 0.96f * fm_pow(1.0f - fmaxf(wo.z, 0.0f), 5.0f))), 0.0f,
+                    0.9f);
+}
+DEV float bsdf_pdf(const Mat& s, v3 wo, v3 wi) {  // trace.wgsl:1774-1788
+    if (wi.z <= 0.0f) return 0.0f;
+    float coat = coat_weight(s, wo);
+    float spec = (1.0f - coat) * fm_clamp(s.metalness + s.specular, 0.0f, 0.9f);
+    v3 h = normalize(wo + wi);
+    float a = alpha_rough(s.roughness);
+    float ca = alpha_rough(s.clearcoat_roughness);
+    float denom = fmaxf(4.0f * fabsf(dot(wi, h)), 1e-8f);
+    float hz = fmaxf(h.z, 0.0f);
+    float base_pdf = ggx_d(hz, a * a) * hz / denom;
+    float coat_pdf = ggx_d(hz, ca * ca) * hz / denom;
+    return 1.0f * (coat * coat_pdf + spec * base_pdf + (1.0f - coat - spec) * wi.z / PI);
+}
+DEV v3 reflect(v3 e1, v3 e2) { return e1 - e2 * (2.0f * dot(e2, e1)); }
+
+// ---- lights (crates/trace/src/lights.wgsl) --------------------------------------
+struct Cand {
+    v3 wi, scale;
+    float pdf, roughness;
+    v3 direction;
+    float limit;
+};
+DEV Cand none() { return Cand{mk(0, 0, 1), mk(0, 0, 0), 0.0f, 0.0f, mk(0, 0, 1), -1.0f}; }
+DEV float mis_power(float chosen, float other) {
+    if (chosen <= 0.0f) return 0.0f;
+    float ratio = other / chosen;
+    return 1.0f / (1.0f + ratio * ratio);
+}
+DEV float4 row(const float4* rows, uint32_t r, int k) { return rows[r * 4u + (uint32_t)k]; }
+DEV void shadow_to(v3 origin, v3 aim, bool shadow, v3& dir, float& limit) {  // lights.wgsl:95-102
+    dir = mk(0, 0, 1);
+    limit = 0.0f;
+    if (!shadow) return;
+    v3 ray = aim - origin;
+    float distance = pfx::length(ray);
+    float l = distance * 0.9999f - 0.0002f;
+    if (l <= 0.0f) return;
+    dir = ray / distance;
+    limit = l;
+}
+DEV Cand emitter_candidate(const SceneDev& S, const m3& T, v3 p, v3 origin, uint32_t* state) {  // :103-144
+    uint32_t count = (uint32_t)S.emitters[0].x;
+    float uz = pfx::rnd(state);
+    uint32_t low = 0, high = count;
+    while (low < high) {
+        uint32_t mid = (low + high) / 2u;
+        if (S.cdf[mid] <= uz)
+            low = mid + 1u;
+        else
+            high = mid;
+    }
+    uint32_t index = (low < count - 1u ? low : count - 1u) + 1u;
+    float4 ea = row(S.emitters, index, 0), eb = row(S.emitters, index, 1), ec = row(S.emitters, index, 2),
+           ee = row(S.emitters, index, 3);
+    float r1 = sqrtf(pfx::rnd(state));
+    float r2 = pfx::rnd(state);
+    v3 a = f3(ea), b = f3(eb), c = f3(ec);
+    v3 x = a * (1.0f - r1) + b * (r1 * (1.0f - r2)) + c * (r1 * r2);
+    v3 to = x - p;
+    float distance2 = dot(to, to);
+    v3 direction = to / sqrtf(fmaxf(distance2, 1e-30f));
+    v3 local = pfx::tmul(T, direction);
+    if (local.z <= 0.0f) return none();
+    v3 normal = normalize(pfx::cross(b - a, c - a));
+    if (fabsf(dot(normal, p - a)) <= 1e-5f) return none();
+    float cosine = fabsf(dot(normal, direction));
+    if (cosine <= 1e-7f) return none();
+    Cand r;
+    shadow_to(origin, x, true, r.direction, r.limit);
+    r.wi = local;
+    r.scale = f3(ee);
+    r.pdf = ee.w / ec.w * distance2 / cosine;
+    r.roughness = 0.0f;
+    return r;
+}
+DEV Cand light_candidate(const SceneDev& S, uint32_t li, const m3& T, v3 p, v3 origin, v3 geometric,
+                         uint32_t* state) {  // :145-200 (point, spot)
+    float4 la = row(S.lights, li, 0), lb = row(S.lights, li, 1), lc = row(S.lights, li, 2), ld = row(S.lights, li, 3);
+    bool shadow = ld.w > 0.5f;
+    v3 to = f3(la) - p;
+    float distance = pfx::length(to);
+    float range = lb.w;
+    if (distance >= range || distance <= 1e-6f) return none();
+    v3 l = to / distance;
+    v3 local = pfx::tmul(T, l);
+    if (local.z <= 0.0f || dot(geometric, l) <= 0.0f) return none();
+    float cone = 1.0f;
+    if ((uint32_t)la.w == 1u) {
+        float along = fm_clamp(dot(-l, f3(lc)) * ld.x + ld.y, 0.0f, 1.0f);
+        cone = along * along;
+        if (cone <= 0.0f) return none();
+    }
+    float radius = lc.w;
+    v3 aim = f3(la);
+    if (shadow && radius > 0.0f) {
+        m3 disc = basis(l);
+        float r = radius * sqrtf(pfx::rnd(state));
+        float phi = 2.0f * PI * pfx::rnd(state);
+        aim = aim + (disc.c0 * (r * fm_cos(phi)) + disc.c1 * (r * fm_sin(phi)));
+    }
+    float ratio = distance / range;
+    float window = fm_clamp(1.0f - ratio * ratio * ratio * ratio, 0.0f, 1.0f);
+    float reach = fmaxf(distance, radius);
+    Cand r;
+    shadow_to(origin, aim, shadow, r.direction, r.limit);
+    r.wi = local;
+    r.scale = f3(lb) * (window * window * cone / (reach * reach));
+    r.pdf = 0.0f;
+    r.roughness = sqrtf(fm_clamp(radius / (2.0f * distance), 0.0f, 1.0f));
+    return r;
+}
+DEV float sin_from_cos(float c) { return sqrtf(fmaxf(1.0f - c * c, 0.0f)); }
+DEV float2 tree_importance(v3 n, v3 to, float cos_u, v3 axis, float theta_o, float theta_e, float far_, float near_,
+                           float energy) {  // :204-237
+    float sin_u = sin_from_cos(cos_u);
+    float cos_i = dot(to, n);
+    float sin_i = sin_from_cos(cos_i);
+    float cos_min_i = cos_i >= cos_u ? 1.0f : cos_i * cos_u + sin_i * sin_u;
+    if (cos_min_i < 0.0f) return make_float2(0.0f, 0.0f);
+    float cos_max_i = fmaxf(cos_i * cos_u - sin_i * sin_u, 0.0f);
+    float cos_t = 1.0f, sin_t = 0.0f;
+    v3 nt = -to;
+    if (axis.x != nt.x || axis.y != nt.y || axis.z != nt.z) {
+        cos_t = dot(axis, nt);
+        sin_t = sin_from_cos(cos_t);
+    }
+    float cos_t_minus_u = cos_t * cos_u + sin_t * sin_u;
+    float cos_o = fm_cos(theta_o);
+    float sin_o = fm_sin(theta_o);
+    float cos_min_out;
+    if (cos_t >= cos_u || cos_t_minus_u >= cos_o) {
+        cos_min_out = 1.0f;
+    } else if (theta_o + theta_e > PI || cos_t_minus_u > fm_cos(theta_o + theta_e)) {
+        cos_min_out = cos_t_minus_u * cos_o + sin_from_cos(cos_t_minus_u) * sin_o;
+    } else {
+        return make_float2(0.0f, 0.0f);
+    }
+    float most = fabsf(cos_min_i * energy * cos_min_out / (near_ * near_));
+    float cos_t_plus_u = cos_t * cos_u - sin_t * sin_u;
+    if (theta_e - theta_o < 0.0f || cos_t < 0.0f || cos_u < 0.0f || cos_t_plus_u < fm_cos(theta_e - theta_o))
+        return make_float2(most, 0.0f);
+    float cos_max_out = cos_t_plus_u * cos_o - sin_from_cos(cos_t_plus_u) * sin_o;
+    return make_float2(most, fabsf(cos_max_i * energy * cos_max_out / (far_ * far_)));
+}
+DEV float2 node_importance(float4 a, float4 b, float4 c, float4 d, v3 p, v3 n) {  // :238-250
+    v3 lo = f3(a), hi = f3(b);
+    v3 centroid = (lo + hi) * 0.5f;
+    v3 extent = hi - centroid;
+    if (dot(n, centroid - p) + dot(pfx::vabs(n), pfx::vabs(extent)) <= 0.0f) return make_float2(0.0f, 0.0f);
+    if (pfx::length(p - pfx::vmin(pfx::vmax(p, lo), hi)) >= b.w) return make_float2(0.0f, 0.0f);
+    v3 to = centroid - p;
+    float reach = pfx::length(to);
+    float radius2 = dot(extent, extent);
+    float cos_u = -1.0f;
+    if (reach * reach > radius2) cos_u = sqrtf(fmaxf(1.0f - radius2 / (reach * reach), 0.0f));
+    float distance = fmaxf(fmaxf(0.5f * sqrtf(radius2), reach), 1e-6f);
+    return tree_importance(n, to / fmaxf(reach, 1e-30f), cos_u, f3(c), c.w, d.x, distance, distance, a.w);
+}
+DEV float2 emitter_importance(const SceneDev& S, uint32_t li, v3 p, v3 n) {  // :255-271
+    float4 la = row(S.lights, li, 0), lb = row(S.lights, li, 1), lc = row(S.lights, li, 2), ld = row(S.lights, li, 3);
+    v3 to = f3(la) - p;
+    float near_ = pfx::length(to);
+    if (near_ >= lb.w || near_ <= 1e-6f) return make_float2(0.0f, 0.0f);
+    v3 toward = to / near_;
+    float radius = lc.w;
+    float far_ = sqrtf(radius * radius + near_ * near_);
+    float cos_u = near_ / far_;
+    float energy = (lb.x + lb.y + lb.z) / 3.0f;
+    if ((uint32_t)la.w == 1u) {
+        float theta_e = fm_acos(fm_clamp(-ld.y / ld.x, -1.0f, 1.0f));
+        float cos_min_out =
+            fm_cos(fmaxf(0.0f, fm_acos(fm_clamp(dot(f3(lc), -toward), -1.0f, 1.0f)) - fm_acos(cos_u)));
+        float along = fm_clamp(cos_min_out * ld.x + ld.y, 0.0f, 1.0f);
+        energy *= along * along;
+        return tree_importance(n, toward, cos_u, f3(lc), 0.0f, theta_e, far_, near_, energy);
+    }
+    return tree_importance(n, toward, cos_u, -toward, 0.0f, PI / 2.0f, far_, near_, energy);
+}
+DEV float2 child_importance(const SceneDev& S, uint32_t r, v3 p, v3 n) {  // :272-275
+    float4 a = row(S.lights, r, 0), b = row(S.lights, r, 1), c = row(S.lights, r, 2), d = row(S.lights, r, 3);
+    if ((uint32_t)d.y == 1u) return emitter_importance(S, (uint32_t)d.z, p, n);
+    return node_importance(a, b, c, d, p, n);
+}
+DEV float2 tree_leaf(const SceneDev& S, uint32_t first, uint32_t count, v3 p, v3 n, float start) {  // :276-349
+    float rand = start;
+    bool sample_max = rand > 0.5f;
+    if (count > 1u) rand = rand * 2.0f - (sample_max ? 1.0f : 0.0f);
+    int selected = -1;
+    float chosen = 0.0f, chosen_other = 0.0f, total = 0.0f, total_other = 0.0f;
+    uint32_t has = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        int index = (int)(first + i);
+        float2 imp = emitter_importance(S, first + i, p, n);
+        float weight = sample_max ? imp.x : imp.y;
+        float other = sample_max ? imp.y : imp.x;
+        if (weight > 0.0f) {
+            total += weight;
+            if (selected < 0) {
+                selected = index;
+                chosen = weight;
+            } else {
+                float thresh = weight / total;
+                if (rand <= thresh) {
+                    selected = index;
+                    chosen = weight;
+                    rand = rand / thresh;
+                } else {
+                    rand = (rand - thresh) / (1.0f - thresh);
+                }
+                rand = fm_clamp(rand, 0.0f, 1.0f);
+            }
+        }
+        if (selected == index) chosen_other = other;
+        total_other += other;
+        has |= (imp.x > 0.0f ? 1u : 0u) << i;
+    }
+    if (has == 0u) return make_float2(0.0f, 0.0f);
+    float chosen_max = sample_max ? chosen : chosen_other;
+    float total_max = sample_max ? total : total_other;
+    float chosen_min = sample_max ? chosen_other : chosen;
+    float total_min = sample_max ? total_other : total;
+    if (total_min == 0.0f) {
+        if (sample_max) {
+            chosen_min = 1.0f;
+            total_min = (float)__popc(has);
+        } else {
+            selected = -1;
+            float valid = 0.0f;
+            uint32_t bits = has;
+            for (uint32_t i = 0; i < count; ++i) {
+                if ((bits & 1u) != 0u) {
+                    valid += 1.0f;
+                    if (selected < 0) {
+                        selected = (int)(first + i);
+                    } else {
+                        float thresh = 1.0f / valid;
+                        if (rand <= thresh) {
+                            selected = (int)(first + i);
+                            rand = rand / thresh;
+                        } else {
+                            rand = (rand - thresh) / (1.0f - thresh);
+                        }
+                        rand = fm_clamp(rand, 0.0f, 1.0f);
+                    }
+                }
+                bits >>= 1u;
+            }
+            chosen_min = 1.0f;
+            total_min = valid;
+            chosen_max = emitter_importance(S, (uint32_t)selected, p, n).x;
+        }
+    }
+    return make_float2((float)selected, 0.5f * (chosen_max / total_max + chosen_min / total_min));
+}
+DEV float2 tree_pick(const SceneDev& S, v3 p, v3 n, float start) {  // :350-386
+    uint32_t at = (uint32_t)(S.emitters[0].y + S.emitters[0].w);
+    float rand = start;
+    float pdf = 1.0f;
+    for (;;) {
+        float4 d = row(S.lights, at, 3);
+        uint32_t count = (uint32_t)d.y;
+        if (count > 0u) {
+            float2 leaf = tree_leaf(S, (uint32_t)d.z, count, p, n, rand);
+            return make_float2(leaf.x, leaf.y * pdf);
+        }
+        uint32_t left = at + 1u;
+        uint32_t right = (uint32_t)d.z;
+        float2 l = child_importance(S, left, p, n);
+        float2 r = child_importance(S, right, p, n);
+        float total_max = l.x + r.x;
+        if (total_max == 0.0f) return make_float2(0.0f, 0.0f);
+        float total_min = l.y + r.y;
+        float probability_min = 0.5f * ((l.x > 0.0f ? 1.0f : 0.0f) + (r.x == 0.0f ? 1.0f : 0.0f));
+        if (total_min > 0.0f) probability_min = l.y / total_min;
+        float left_probability = 0.5f * (l.x / total_max + probability_min);
+        float right_weight = 1.0f - left_probability;
+        at = left;
+        if (right_weight > 0.0f) {
+            float thresh = right_weight / (left_probability + right_weight);
+            if (rand <= thresh) {
+                at = right;
+                rand = rand / thresh;
+            } else {
+                rand = (rand - thresh) / (1.0f - thresh);
+            }
+            rand = fm_clamp(rand, 0.0f, 1.0f);
+        }
+        pdf *= at == left ? left_probability : 1.0f - left_probability;
+    }
+}
+DEV Cand direct_light(const SceneDev& S, uint32_t slot, const m3& T, v3 p, v3 origin, v3 geometric,
+                      uint32_t* state) {  // :387-409
+    uint32_t looped = (uint32_t)S.emitters[0].y;
+    if (slot > looped) {
+        float2 picked = tree_pick(S, p, T.c2, pfx::rnd(state));
+        if (!(picked.y > 0.0f)) return none();
+        Cand c = light_candidate(S, (uint32_t)picked.x, T, p, origin, geometric, state);
+        c.scale = c.scale / picked.y;
+        return c;
+    }
+    if (slot > 0u) return light_candidate(S, slot - 1u, T, p, origin, geometric, state);
+    if (S.emitters[0].x < 0.5f) return none();
+    return emitter_candidate(S, T, p, origin, state);
+}
+
+// ---- sky (crates/trace/src/sky.rs SKY_SAMPLING_WGSL) ---------------------------
+DEV v3 env_radiance(const SceneDev& S, v3 direction) {
+    if (S.env_black) return mk(0, 0, 0);
+    v3 d = normalize(direction);
+    float u = pfx::fm_atan2(d.x, -d.z) / 6.28318531f + 0.5f;
+    u = u - floorf(u);
+    float v = fm_acos(fm_clamp(d.y, -1.0f, 1.0f)) / 3.14159265f;
+    float x = u * (float)S.env_w - 0.5f, y = v * (float)S.env_h - 0.5f;
+    float x0 = floorf(x), y0 = floorf(y);
+    float tx = x - x0, ty = y - y0;
+    int w = (int)S.env_w, h = (int)S.env_h;
+    int c0 = (((int)x0) % w + w) % w, c1 = (((int)(x0 + 1.0f)) % w + w) % w;
+    int r0 = (int)y0, r1 = (int)(y0 + 1.0f);
+    r0 = r0 < 0 ? 0 : (r0 > h - 1 ? h - 1 : r0);
+    r1 = r1 < 0 ? 0 : (r1 > h - 1 ? h - 1 : r1);
+    const float4* t = reinterpret_cast<const float4*>(S.env);
+    float4 A = t[r0 * w + c0], B = t[r0 * w + c1], C = t[r1 * w + c0], D = t[r1 * w + c1];
+    v3 a = f3(A), b = f3(B), c = f3(C), dd = f3(D);
+    return (a * (1.0f - tx) + b * tx) * (1.0f - ty) + (c * (1.0f - tx) + dd * tx) * ty;
+}
+DEV uint32_t cdf_find(const float* cdf, uint32_t n, float needle) {
+    uint32_t low = 0, high = n;
+    while (low < high) {
+        uint32_t mid = (low + high) / 2u;
+        if (cdf[mid] <= needle)
+            low = mid + 1u;
+        else
+            high = mid;
+    }
+    return low < n - 1u ? low : n - 1u;
+}
+constexpr uint32_t SKY_W = 256, SKY_H = 128;
+DEV void sky_sample(const SceneDev& S, float rx, float ry, v3& direction, float& pdf) {
+    float u = fm_clamp(rx, 0.0f, 0.99999994f);
+    float v = fm_clamp(ry, 0.0f, 0.99999994f);
+    uint32_t rowi = cdf_find(S.sky_rows, SKY_H, v);
+    float row_before = rowi > 0u ? S.sky_rows[rowi - 1u] : 0.0f;
+    float fy = (v - row_before) / fmaxf(S.sky_rows[rowi] - row_before, 0.000001f);
+    float needle = fmaxf(u - 0.0f, 0.0f) / fmaxf(1.0f - 0.0f, 0.000001f);
+    const float* cols = S.sky_columns + rowi * SKY_W;
+    uint32_t col = cdf_find(cols, SKY_W, needle);
+    float col_before = col > 0u ? cols[col - 1u] : 0.0f;
+    float fx = (needle - col_before) / fmaxf(cols[col] - col_before, 0.000001f);
+    float theta = 3.14159265f * ((float)rowi + fy) / (float)SKY_H;
+    float phi = 6.28318531f * ((float)col + fx) / (float)SKY_W;
+    direction = mk(fm_sin(theta) * fm_cos(phi), fm_cos(theta), fm_sin(theta) * fm_sin(phi));
+    float area = 6.28318531f / (float)SKY_W * 3.14159265f / (float)SKY_H * fmaxf(fm_sin(theta), 0.00001f);
+    float mass = (S.sky_rows[rowi] - row_before) * (cols[col] - col_before);
+    pdf = (1.0f - 0.0f) * mass / area + 0.0f;
+}
+DEV float sky_pdf(const SceneDev& S, v3 direction) {
+    v3 d = normalize(direction);
+    float theta = fm_acos(fm_clamp(d.y, -1.0f, 1.0f));
+    float a = pfx::fm_atan2(d.z, d.x) + 6.28318531f;
+    float phi = a - 6.28318531f * truncf(a / 6.28318531f);
+    float fr = theta / 3.14159265f * (float)SKY_H, fc = phi / 6.28318531f * (float)SKY_W;
+    uint32_t rowi = fr >= (float)SKY_H ? SKY_H - 1u : (uint32_t)fmaxf(fr, 0.0f);
+    uint32_t col = fc >= (float)SKY_W ? SKY_W - 1u : (uint32_t)fmaxf(fc, 0.0f);
+    if (rowi > SKY_H - 1u) rowi = SKY_H - 1u;
+    if (col > SKY_W - 1u) col = SKY_W - 1u;
+    float row_before = rowi > 0u ? S.sky_rows[rowi - 1u] : 0.0f;
+    const float* cols = S.sky_columns + rowi * SKY_W;
+    float col_before = col > 0u ? cols[col - 1u] : 0.0f;
+    float mass = (S.sky_rows[rowi] - row_before) * (cols[col] - col_before);
+    float area = 6.28318531f / (float)SKY_W * 3.14159265f / (float)SKY_H * fmaxf(fm_sin(theta), 0.00001f);
+    return (1.0f - 0.0f) * mass / area + 0.0f;
+}
+
+// ---- kernels ------------------------------------------------------------------------
+// Camera rays (trace.wgsl:2504-2561): one path per (sample, active pixel).
+__global__ __launch_bounds__(256) void k_camera(SceneDev S, WaveDev W, const uint32_t* active, uint32_t npix,
+                                                uint32_t nsamples, uint32_t acc, uint32_t seed) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= npix * nsamples) return;
+    uint32_t s = i / npix, p = i - s * npix;
+    uint32_t pix = active ? active[p] : p;
+    uint32_t y = pix / S.width, x = pix - y * S.width;
+    uint32_t px = x + S.window[2], py = y + S.window[3];
+    uint32_t stream = py * S.window[0] + px;
+    uint32_t state = pfx::pcg(stream * 1973u + pfx::pcg(seed * 9277u + (acc + s) * 7919u + 26699u));
+    float jx = pfx::rnd(&state);
+    float jy = pfx::rnd(&state);
+    float ndcx = ((float)px + jx) / (float)S.window[0] * 2.0f - 1.0f;
+    float ndcy = ((float)py + jy) / (float)S.window[1] * 2.0f - 1.0f;
+    v3 fwd = mk(S.forward[0], S.forward[1], S.forward[2]), right = mk(S.right[0], S.right[1], S.right[2]),
+       up = mk(S.up[0], S.up[1], S.up[2]);
+    v3 direction = normalize(fwd + right * (ndcx + S.lens[0]) + up * (-ndcy + S.lens[1]));
+    v3 origin = mk(S.origin[0], S.origin[1], S.origin[2]);
+    if (S.lens[2] > 0.0f) {
+        v3 focus = origin + direction * (S.lens[3] / dot(direction, normalize(fwd)));
+        float radius = sqrtf(pfx::rnd(&state)) * S.lens[2];
+        float phi = 2.0f * PI * pfx::rnd(&state);
+        origin = origin + (normalize(right) * (radius * fm_cos(phi)) + normalize(up) * (radius * fm_sin(phi)));
+        direction = normalize(focus - origin);
+    }
+    W.o[i] = f4(origin, 0.0f);
+    W.d[i] = f4(direction, ubits(0u));
+    W.thr[i] = make_float4(1.0f, 1.0f, 1.0f, ubits(state));
+    W.res[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    W.alb[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    W.nrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev S, WaveDev W, const uint32_t* queue, uint32_t n) {
+    __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
+    uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    uint32_t stride = gridDim.x * TRACE_BLOCK;
+    Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
+    uint32_t nodes = 0, prims = 0;
+    for (uint32_t k = gtid; k < n; k += stride) {
+        uint32_t i = queue ? queue[k] : k;
+        float4 o4 = W.o[i], d4 = W.d[i];
+        Closest h = trace_closest<COUNT>(S, f3(o4), f3(d4), 1e20f, st, nodes, prims);
+        W.hit[i] = make_float4(h.t, h.u, h.v, ubits(h.slot));
+        W.hit_rec[i] = h.rec;
+    }
+    if (COUNT) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_NODES), (unsigned long long)nodes);
+        atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_PRIMS), (unsigned long long)prims);
+    }
+}
+
+// One path vertex (trace.wgsl:2169-2501 loop body).
+__global__ __launch_bounds__(256) void k_shade(SceneDev S, WaveDev W, const uint32_t* queue, uint32_t n,
+                                               uint32_t bounce) {
+    uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    bool valid = k < n;
+    bool cont = false;
+    uint32_t entries = 0;
+    uint32_t i = 0;
+    if (valid) {
+        i = queue ? queue[k] : k;
+        float4 o4 = W.o[i], d4 = W.d[i], t4 = W.thr[i], r4 = W.res[i], hv = W.hit[i];
+        v3 o = f3(o4), d = f3(d4), throughput = f3(t4), result = f3(r4);
+        float previous_pdf = o4.w;
+        uint32_t lobes = fbits(d4.w);
+        uint32_t state = fbits(t4.w);
+        float t = hv.x;
+        bool done = false;
+        if (t >= 1e19f) {
+            if (!S.env_black) {
+                float pdf = sky_pdf(S, d);
+                float weight = (bounce > 0u && previous_pdf > 0.0f) ? previous_pdf / (previous_pdf + pdf) : 1.0f;
+                result = result + throughput * env_radiance(S, d) * weight;
+            }
+            done = true;
+        }
+        if (!done) {
+            uint32_t slot = fbits(hv.w), rec = W.hit_rec[i];
+            Surf sf = surface_at(S, slot, rec, hv.y, hv.z);
+            v3 p = o + d * t;
+            v3 origin_point = pfx::spawn_point(o, -d, o, 0.0f);
+            v3 n = dot(sf.n, d) > 0.0f ? -sf.n : sf.n;
+            Mat s = load_mat(S, sf.material);
+            v3 geometric = dot(sf.geometric, d) > 0.0f ? -sf.geometric : sf.geometric;
+            if (dot(n, geometric) < 0.05f) n = normalize(n + geometric * 0.1f);
+            s.roughness = fmaxf(s.roughness, 0.05f);
+            s.clearcoat_roughness = fmaxf(s.clearcoat_roughness, 0.05f);
+            if (bounce == 0u) {
+                W.alb[i] = f4(s.base, 1.0f);
+                W.nrm[i] = f4(n, 1.0f);
+            }
+            // emitter_weight (lights.wgsl:38-44)
+            if (S.lit) {
+                float weight = 1.0f;
+                bool emissive = !(s.emission.x <= 0.0f && s.emission.y <= 0.0f && s.emission.z <= 0.0f);
+                if (previous_pdf > 0.0f && emissive) {
+                    uint32_t index = S.attrs[slot].emitter;
+                    if (index != 0u) {
+                        float4 ee = row(S.emitters, index, 3), ec = row(S.emitters, index, 2);
+                        v3 to = p - origin_point;
+                        float distance2 = dot(to, to);
+                        float cosine = fabsf(dot(sf.geometric, to)) / sqrtf(fmaxf(distance2, 1e-30f));
+                        float epdf = ee.w / fmaxf(ec.w, 1e-30f) * distance2 / fmaxf(cosine, 1e-7f);
+                        weight = mis_power(previous_pdf, epdf);
+                    }
+                }
+                s.emission = s.emission * weight;
+            }
+            result = result + throughput * s.emission;
+            if (lobes & LAST_BIT) {
+                done = true;
+            } else {
+                m3 T = basis(n);
+                v3 wo = pfx::tmul(T, -d);
+                v3 origin = pfx::spawn_point(p, n, o, t);
+                uint32_t slots = 2u + S.light_slots;
+                for (uint32_t kk = 0; kk < slots; ++kk) {
+                    v3 local = mk(0, 0, 1), direction = mk(0, 0, 1), strength = mk(0, 0, 0);
+                    float limit = 1e19f, pdf = 0.0f;
+                    Mat lit = s;
+                    if (kk == 0u) {
+                        if (S.sun_intensity <= 0.0f) continue;
+                        direction = normalize(mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]));
+                        if (S.sun_w < 1.0f) {
+                            float height = 1.0f - pfx::rnd(&state) * (1.0f - S.sun_w);
+                            float radius = sqrtf(fmaxf(0.0f, 1.0f - height * height));
+                            float phi = 2.0f * PI * pfx::rnd(&state);
+                            m3 orient = basis(direction);
+                            direction =
+                                normalize(pfx::mul(orient, mk(radius * fm_cos(phi), radius * fm_sin(phi), height)));
+                        }
+                        local = pfx::tmul(T, direction);
+                        if (local.z <= 0.0f) continue;
+                        strength = mk(S.sun_color[0], S.sun_color[1], S.sun_color[2]) * (S.sun_intensity * 1.0f);
+                    } else if (kk == 1u) {
+                        float rx = pfx::rnd(&state);
+                        float ry = pfx::rnd(&state);
+                        if (S.env_black) continue;  // strength 0: the term adds nothing
+                        float spdf;
+                        sky_sample(S, rx, ry, direction, spdf);
+                        local = pfx::tmul(T, direction);
+                        if (local.z <= 0.0f) continue;
+                        strength = env_radiance(S, direction) * 1.0f;
+                        pdf = spdf;
+                        if (!(pdf > 0.0f)) continue;
+                    } else {
+                        Cand c = direct_light(S, kk - 2u, T, p, origin, geometric, &state);
+                        if (c.limit < 0.0f) continue;
+                        local = c.wi;
+                        direction = c.direction;
+                        strength = c.scale;
+                        pdf = c.pdf;
+                        limit = c.limit;
+                        lit.roughness = fmaxf(s.roughness, c.roughness);
+                    }
+                    if (pdf > 0.0f) {
+                        float bsdf = bsdf_pdf(s, wo, local);
+                        if (kk == 1u) {
+                            strength = strength / (pdf + bsdf);
+                        } else {
+                            float ratio = bsdf / pdf;
+                            strength = strength / (pdf * (1.0f + ratio * ratio));
+                        }
+                    }
+                    v3 add = throughput * eval(lit, wo, local) * 1.0f * local.z * strength;
+                    if (add.x == 0.0f && add.y == 0.0f && add.z == 0.0f) continue;
+                    if (entries < S.entries) {
+                        size_t e = (size_t)entries * W.capacity + i;
+                        W.e_o[e] = f4(origin, limit > 0.0f ? limit : 0.0f);
+                        W.e_d[e] = f4(direction, 0.0f);
+                        W.e_c[e] = f4(add, 0.0f);
+                        entries++;
+                    }
+                }
+                // sample_bsdf (trace.wgsl:1716-1773), diffuse + GGX + coat
+                (void)pfx::rnd(&state);  // transmission pick: never below zero transmission
+                float cw = coat_weight(s, wo);
+                float sw = (1.0f - cw) * fm_clamp(s.metalness + s.specular, 0.0f, 0.9f);
+                float pick = pfx::rnd(&state);
+                v3 wi;
+                uint32_t lobe;
+                if (pick < cw + sw) {
+                    bool coat = pick < cw;
+                    float alpha = alpha_rough(coat ? s.clearcoat_roughness : s.roughness);
+                    float u1 = pfx::rnd(&state);
+                    float u2 = pfx::rnd(&state);
+                    float phi = 2.0f * PI * u1;
+                    float cos_theta = sqrtf((1.0f - u2) / (1.0f + (alpha * alpha - 1.0f) * u2));
+                    float stt = sqrtf(fmaxf(0.0f, 1.0f - cos_theta * cos_theta));
+                    v3 h = mk(fm_cos(phi) * stt, fm_sin(phi) * stt, cos_theta);
+                    wi = reflect(-wo, h);
+                    lobe = 1u;
+                } else {
+                    float u1 = pfx::rnd(&state);
+                    float u2 = pfx::rnd(&state);
+                    float phi = 2.0f * PI * u1;
+                    float r = sqrtf(u2);
+                    wi = mk(fm_cos(phi) * r, fm_sin(phi) * r, sqrtf(1.0f - u2));
+                    lobe = 0u;
+                }
+                float spdf = bsdf_pdf(s, wo, wi);
+                uint32_t dif = (lobes & 0xffu) + (lobe == 0u ? 1u : 0u);
+                uint32_t glo = ((lobes >> 8) & 0xffu) + (lobe == 1u ? 1u : 0u);
+                uint32_t tra = (lobes >> 16) & 0xffu;
+                bool last = bounce >= S.total || dif > S.diffuse || glo > S.glossy || tra > S.transmission;
+                lobes = (dif & 0xffu) | ((glo & 0xffu) << 8) | ((tra & 0xffu) << 16) | (last ? LAST_BIT : 0u);
+                if (wi.z <= 0.0f || spdf <= 0.0f) {
+                    done = true;
+                } else {
+                    throughput = throughput * (eval(s, wo, wi) * 1.0f * wi.z / spdf * 1.0f);
+                    d = normalize(pfx::mul(T, wi));
+                    previous_pdf = spdf;
+                    o = pfx::spawn_point(p, d, o, t);
+                    if (bounce >= 3u) {
+                        float keep = fm_clamp(fmaxf(throughput.x, fmaxf(throughput.y, throughput.z)), 0.05f, 0.95f);
+                        if (pfx::rnd(&state) > keep)
+                            done = true;
+                        else
+                            throughput = throughput / keep;
+                    }
+                }
+            }
+        }
+        cont = !done;
+        W.o[i] = f4(o, previous_pdf);
+        W.d[i] = f4(d, ubits(lobes));
+        W.thr[i] = f4(throughput, ubits(state));
+        W.res[i] = f4(result, 0.0f);
+    }
+    uint32_t at = wave_append(W.counters + C_NEXT, cont);
+    if (cont) W.q_next[at] = i;
+    bool sh = valid && entries > 0u;
+    uint32_t as = wave_append(W.counters + C_SHADOW, sh);
+    if (sh) W.q_sh[as] = i | (entries << 28);
+}
+
+// Shadow rays of one path, in slot order (trace.wgsl:2465-2470).
+template <bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev S, WaveDev W, uint32_t n) {
+    __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
+    uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    uint32_t stride = gridDim.x * TRACE_BLOCK;
+    Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
+    uint32_t nodes = 0, prims = 0, rays = 0;
+    for (uint32_t k = gtid; k < n; k += stride) {
+        uint32_t q = W.q_sh[k];
+        uint32_t i = q & 0x0fffffffu, m = q >> 28;
+        float4 r = W.res[i];
+        v3 result = f3(r);
+        for (uint32_t e = 0; e < m; ++e) {
+            size_t at = (size_t)e * W.capacity + i;
+            float4 eo = W.e_o[at];
+            if (eo.w > 0.0f) {
+                rays++;
+                float4 ed = W.e_d[at];
+                if (trace_any<COUNT>(S, f3(eo), f3(ed), eo.w, st, nodes, prims)) continue;
+            }
+            result = result + f3(W.e_c[at]);
+        }
+        W.res[i] = f4(result, 0.0f);
+    }
+    if (rays) atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_SHADOW_RAYS), (unsigned long long)rays);
+    if (COUNT) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_NODES), (unsigned long long)nodes);
+        atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_PRIMS), (unsigned long long)prims);
+    }
+}
+
+DEV void neumaier(float& s, float& c, float x) {
+    float t = s + x;
+    if (fabsf(s) >= fabsf(x))
+        c += (s - t) + x;
+    else
+        c += (x - t) + s;
+    s = t;
+}
+
+// Fold a wave's samples into each pixel's sums, in sample order
+// (trace.wgsl:2562-2571).
+__global__ __launch_bounds__(256) void k_finish(WaveDev W, AccumDev A, const uint32_t* active, uint32_t npix,
+                                                uint32_t nsamples, float count_after) {
+    uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p >= npix) return;
+    uint32_t pix = active ? active[p] : p;
+    if (A.mode == PFXH_ACCUM_PLAIN) {
+        float4 c = A.sum[pix * 3u], a = A.sum[pix * 3u + 1u], nn = A.sum[pix * 3u + 2u];
+        for (uint32_t s = 0; s < nsamples; ++s) {
+            uint32_t i = s * npix + p;
+            float4 r = W.res[i], al = W.alb[i], no = W.nrm[i];
+            c.x = c.x + r.x;
+            c.y = c.y + r.y;
+            c.z = c.z + r.z;
+            a.x = a.x + al.x;
+            a.y = a.y + al.y;
+            a.z = a.z + al.z;
+            a.w = a.w + al.w;
+            nn.x = nn.x + no.x;
+            nn.y = nn.y + no.y;
+            nn.z = nn.z + no.z;
+            nn.w = nn.w + no.w;
+        }
+        c.w = count_after;
+        A.sum[pix * 3u] = c;
+        A.sum[pix * 3u + 1u] = a;
+        A.sum[pix * 3u + 2u] = nn;
+    } else if (A.mode == 2u) {
+        double* D = A.sum64 + (size_t)pix * 12u;
+        double acc[12];
+        for (int k = 0; k < 12; ++k) acc[k] = D[k];
+        for (uint32_t s = 0; s < nsamples; ++s) {
+            uint32_t i = s * npix + p;
+            float4 r = W.res[i], al = W.alb[i], no = W.nrm[i];
+            float v[12] = {r.x, r.y, r.z, 0.0f, al.x, al.y, al.z, al.w, no.x, no.y, no.z, no.w};
+            for (int k = 0; k < 12; ++k)
+                if (k != 3) acc[k] = acc[k] + (double)v[k];
+        }
+        acc[3] = (double)count_after;
+        for (int k = 0; k < 12; ++k) D[k] = acc[k];
+        float4 c = A.sum[pix * 3u];
+        c.w = count_after;
+        A.sum[pix * 3u] = c;
+    } else {
+        float4 S0 = A.sum[pix * 3u], S1 = A.sum[pix * 3u + 1u], S2 = A.sum[pix * 3u + 2u];
+        float4 C0 = A.comp[pix * 3u], C1 = A.comp[pix * 3u + 1u], C2 = A.comp[pix * 3u + 2u];
+        for (uint32_t s = 0; s < nsamples; ++s) {
+            uint32_t i = s * npix + p;
+            float4 r = W.res[i], al = W.alb[i], no = W.nrm[i];
+            neumaier(S0.x, C0.x, r.x);
+            neumaier(S0.y, C0.y, r.y);
+            neumaier(S0.z, C0.z, r.z);
+            neumaier(S1.x, C1.x, al.x);
+            neumaier(S1.y, C1.y, al.y);
+            neumaier(S1.z, C1.z, al.z);
+            neumaier(S1.w, C1.w, al.w);
+            neumaier(S2.x, C2.x, no.x);
+            neumaier(S2.y, C2.y, no.y);
+            neumaier(S2.z, C2.z, no.z);
+            neumaier(S2.w, C2.w, no.w);
+        }
+        S0.w = count_after;
+        A.sum[pix * 3u] = S0;
+        A.sum[pix * 3u + 1u] = S1;
+        A.sum[pix * 3u + 2u] = S2;
+        A.comp[pix * 3u] = C0;
+        A.comp[pix * 3u + 1u] = C1;
+        A.comp[pix * 3u + 2u] = C2;
+    }
+}
+
+DEV float total_of(const AccumDev& A, uint32_t pix, int k) {
+    if (A.mode == 2u) return (float)A.sum64[(size_t)pix * 12u + k];
+    const float* s = reinterpret_cast<const float*>(A.sum + pix * 3u);
+    if (A.mode == PFXH_ACCUM_PLAIN) return s[k];
+    const float* c = reinterpret_cast<const float*>(A.comp + pix * 3u);
+    return s[k] + c[k];
+}
+
+// The three RGBA32F images (trace.wgsl:2572-2578).
+__global__ __launch_bounds__(256) void k_develop(AccumDev A, uint32_t pixels, float4* color, float4* albedo,
+                                                 float4* normal) {
+    uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix >= pixels) return;
+    float w = A.sum[pix * 3u].w;
+    float count = fmaxf(fabsf(w), 1.0f);
+    if (A.mode == 2u) {
+        const double* D = A.sum64 + (size_t)pix * 12u;
+        double c = (double)count;
+        if (color) color[pix] = make_float4((float)(D[0] / c), (float)(D[1] / c), (float)(D[2] / c), 1.0f);
+        if (albedo) albedo[pix] = make_float4((float)(D[4] / c), (float)(D[5] / c), (float)(D[6] / c), (float)(D[7] / c));
+        v3 n = mk((float)(D[8] / c), (float)(D[9] / c), (float)(D[10] / c));
+        v3 nn = pfx::length(n) > 1e-6f ? normalize(n) : mk(0, 0, 0);
+        if (normal) normal[pix] = f4(nn, (float)(D[11] / c));
+        return;
+    }
+    if (color)
+        color[pix] = make_float4(total_of(A, pix, 0) / count, total_of(A, pix, 1) / count, total_of(A, pix, 2) / count,
+                                 1.0f);
+    if (albedo)
+        albedo[pix] = make_float4(total_of(A, pix, 4) / count, total_of(A, pix, 5) / count,
+                                  total_of(A, pix, 6) / count, total_of(A, pix, 7) / count);
+    v3 n = mk(total_of(A, pix, 8) / count, total_of(A, pix, 9) / count, total_of(A, pix, 10) / count);
+    v3 nn = pfx::length(n) > 1e-6f ? normalize(n) : mk(0, 0, 0);
+    if (normal) normal[pix] = f4(nn, total_of(A, pix, 11) / count);
+}
+
+__global__ __launch_bounds__(256) void k_clear(AccumDev A, uint32_t pixels) {
+    uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix >= pixels) return;
+    float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int k = 0; k < 3; ++k) {
+        A.sum[pix * 3u + k] = z;
+        if (A.comp) A.comp[pix * 3u + k] = z;
+    }
+    if (A.sum64)
+        for (int k = 0; k < 12; ++k) A.sum64[(size_t)pix * 12u + k] = 0.0;
+}
+
+// Pixels not frozen by adaptive sampling (trace.wgsl:2517).
+__global__ __launch_bounds__(256) void k_active(AccumDev A, uint32_t pixels, uint32_t* active, uint32_t* counter) {
+    uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    bool live = pix < pixels && !(A.sum[pix * 3u].w < 0.0f);
+    uint32_t at = wave_append(counter, live);
+    if (live) active[at] = pix;
+}
+
+// Pixel-centre primary hits (probe.wgsl:3-39) plus the caller's primitive.
+__global__ __launch_bounds__(TRACE_BLOCK) void k_probe(SceneDev S, WaveDev W, pfxh_probe_hit* out) {
+    __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
+    uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    uint32_t stride = gridDim.x * TRACE_BLOCK;
+    Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
+    uint32_t pixels = S.width * S.height, dn = 0, dp = 0;
+    v3 fwd = mk(S.forward[0], S.forward[1], S.forward[2]), right = mk(S.right[0], S.right[1], S.right[2]),
+       up = mk(S.up[0], S.up[1], S.up[2]);
+    for (uint32_t pix = gtid; pix < pixels; pix += stride) {
+        uint32_t y = pix / S.width, x = pix - y * S.width;
+        float ndcx = ((float)(x + S.window[2]) + 0.5f) / (float)S.window[0] * 2.0f - 1.0f;
+        float ndcy = ((float)(y + S.window[3]) + 0.5f) / (float)S.window[1] * 2.0f - 1.0f;
+        v3 d = normalize(fwd + right * (ndcx + S.lens[0]) + up * (-ndcy + S.lens[1]));
+        v3 o = mk(S.origin[0], S.origin[1], S.origin[2]);
+        Closest h = trace_closest<false>(S, o, d, 1e20f, st, dn, dp);
+        pfxh_probe_hit r;
+        r.depth = 0.0f;
+        r.distance = 0.0f;
+        r.material_id = 0u;
+        r.primitive = 0xffffffffu;
+        r.instance = 0u;
+        r.normal[0] = r.normal[1] = r.normal[2] = 0.0f;
+        if (h.t < 1e19f) {
+            Surf sf = surface_at(S, h.slot, h.rec, h.u, h.v);
+            v3 n = dot(sf.n, d) > 0.0f ? -sf.n : sf.n;
+            r.depth = h.t * dot(d, normalize(fwd));
+            r.distance = h.t;
+            r.material_id = sf.material + 1u;
+            r.primitive = S.slot_prim[h.slot];
+            r.instance = S.record_instance[h.rec];
+            r.normal[0] = n.x;
+            r.normal[1] = n.y;
+            r.normal[2] = n.z;
+        }
+        out[pix] = r;
+    }
+}
+
+// converge.wgsl:9-30
+__global__ __launch_bounds__(256) void k_measure(AccumDev A, float4* snaps, uint32_t width, uint32_t height) {
+    uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix >= width * height) return;
+    float count = A.sum[pix * 3u].w;
+    if (count <= 0.0f) {
+        snaps[pix * 2u + 1u].y = 0.0f;
+        return;
+    }
+    float now[4] = {total_of(A, pix, 0) / count, total_of(A, pix, 1) / count, total_of(A, pix, 2) / count,
+                    total_of(A, pix, 7) / count};
+    float4 mean = snaps[pix * 2u], info = snaps[pix * 2u + 1u];
+    float prior = info.x;
+    float m[4] = {mean.x, mean.y, mean.z, mean.w};
+    float error = 1e9f;
+    if (prior > 0.0f && count > prior) {
+        float scale = sqrtf(1.0f / count) / sqrtf(1.0f / prior + 1.0f / (count - prior));
+        float gap[4];
+        for (int k = 0; k < 4; ++k) {
+            float rest = (now[k] * count - m[k] * prior) / (count - prior);
+            gap[k] = fabsf(m[k] - rest) * scale;
+        }
+        error = (gap[0] + gap[1] + gap[2]) / (1e-4f + sqrtf(fmaxf(now[0] + now[1] + now[2], 0.0f))) + gap[3];
+    }
+    snaps[pix * 2u] = make_float4(now[0], now[1], now[2], now[3]);
+    snaps[pix * 2u + 1u] = make_float4(count, error, 0.0f, 0.0f);
+}
+
+// converge.wgsl:32-51
+__global__ __launch_bounds__(256) void k_mark(AccumDev A, const float4* snaps, uint32_t width, uint32_t height,
+                                              uint32_t min_samples, float threshold, uint32_t* running) {
+    uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    bool run = false;
+    if (pix < width * height) {
+        float count = A.sum[pix * 3u].w;
+        if (count > 0.0f) {
+            int x = (int)(pix % width), y = (int)(pix / width);
+            float worst = 0.0f;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int xx = min(max(x + dx, 0), (int)width - 1), yy = min(max(y + dy, 0), (int)height - 1);
+                    worst = fmaxf(worst, snaps[((uint32_t)yy * width + (uint32_t)xx) * 2u + 1u].y);
+                }
+            if (count >= (float)min_samples && worst < threshold) {
+                A.sum[pix * 3u].w = -count;
+            } else {
+                run = true;
+            }
+        }
+    }
+    wave_append(running, run);
+}
+
+__global__ void k_zero(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// ---- launchers ---------------------------------------------------------------------
+static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+
+void launch_camera(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* active, uint32_t npix,
+                   uint32_t nsamples, uint32_t acc, uint32_t seed) {
+    uint64_t n = (uint64_t)npix * nsamples;
+    if (!n) return;
+    hipLaunchKernelGGL(k_camera, dim3(blocks_for(n, 256)), dim3(256), 0, s, S, W, active, npix, nsamples, acc, seed);
+}
+void launch_extend(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* queue, uint32_t n, bool count,
+                   uint32_t grid) {
+    if (!n) return;
+    uint32_t blocks = std::min(blocks_for(n, TRACE_BLOCK), grid);
+    if (count)
+        hipLaunchKernelGGL(k_extend<true>, dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, W, queue, n);
+    else
+        hipLaunchKernelGGL(k_extend<false>, dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, W, queue, n);
+}
+void launch_shade(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* queue, uint32_t n,
+                  uint32_t bounce) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_shade, dim3(blocks_for(n, 256)), dim3(256), 0, s, S, W, queue, n, bounce);
+}
+void launch_shadow(hipStream_t s, const SceneDev& S, const WaveDev& W, uint32_t n, bool count, uint32_t grid) {
+    if (!n) return;
+    uint32_t blocks = std::min(blocks_for(n, TRACE_BLOCK), grid);
+    if (count)
+        hipLaunchKernelGGL(k_shadow<true>, dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, W, n);
+    else
+        hipLaunchKernelGGL(k_shadow<false>, dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, W, n);
+}
+void launch_finish(hipStream_t s, const WaveDev& W, const AccumDev& A, const uint32_t* active, uint32_t npix,
+                   uint32_t nsamples, float count_after) {
+    if (!npix) return;
+    hipLaunchKernelGGL(k_finish, dim3(blocks_for(npix, 256)), dim3(256), 0, s, W, A, active, npix, nsamples,
+                       count_after);
+}
+void launch_develop(hipStream_t s, const AccumDev& A, uint32_t pixels, float4* color, float4* albedo, float4* normal) {
+    if (!pixels) return;
+    hipLaunchKernelGGL(k_develop, dim3(blocks_for(pixels, 256)), dim3(256), 0, s, A, pixels, color, albedo, normal);
+}
+void launch_clear(hipStream_t s, const AccumDev& A, uint32_t pixels) {
+    if (!pixels) return;
+    hipLaunchKernelGGL(k_clear, dim3(blocks_for(pixels, 256)), dim3(256), 0, s, A, pixels);
+}
+void launch_active(hipStream_t s, const AccumDev& A, uint32_t pixels, uint32_t* active, uint32_t* counter) {
+    if (!pixels) return;
+    hipLaunchKernelGGL(k_active, dim3(blocks_for(pixels, 256)), dim3(256), 0, s, A, pixels, active, counter);
+}
+void launch_probe(hipStream_t s, const SceneDev& S, const WaveDev& W, void* out) {
+    uint32_t pixels = S.width * S.height;
+    if (!pixels) return;
+    uint32_t blocks = std::min(blocks_for(pixels, TRACE_BLOCK), W.spill_threads / TRACE_BLOCK);
+    hipLaunchKernelGGL(k_probe, dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, W, (pfxh_probe_hit*)out);
+}
+void launch_measure(hipStream_t s, const AccumDev& A, float4* snaps, uint32_t width, uint32_t height) {
+    uint32_t pixels = width * height;
+    if (!pixels) return;
+    hipLaunchKernelGGL(k_measure, dim3(blocks_for(pixels, 256)), dim3(256), 0, s, A, snaps, width, height);
+}
+void launch_mark(hipStream_t s, const AccumDev& A, const float4* snaps, uint32_t width, uint32_t height,
+                 uint32_t min_samples, float threshold, uint32_t* running) {
+    uint32_t pixels = width * height;
+    if (!pixels) return;
+    hipLaunchKernelGGL(k_mark, dim3(blocks_for(pixels, 256)), dim3(256), 0, s, A, snaps, width, height, min_samples,
+                       threshold, running);
+}
+void launch_zero(hipStream_t s, uint32_t* p, uint32_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_zero, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n);
+}
+
+}  // namespace pfxh

@@ -1,0 +1,66 @@
+// Host-side scene layout for the MI355X tracer: the reference's CPU build
+// (binned-SAH BVH, two-level instancing, light tree, emitter and sky CDFs)
+// followed by conversion into the device formats the kernels read.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pfxhip.h"
+#include "device_types.h"
+
+namespace pfxh {
+
+// Reference-format BVH (bvh::Node, crates/trace/src/bvh.rs:5-13)
+struct RefBvh {
+    std::vector<pfxh_bvh_node> nodes;
+    std::vector<uint32_t> order;
+};
+RefBvh build_bvh(const float* vertices9, size_t count);
+
+// levels::lay (crates/trace/src/levels.rs:225-383) in the reference node format
+struct RefLevels {
+    std::vector<pfxh_bvh_node> nodes;
+    std::vector<uint32_t> world_order;
+};
+
+// Everything a trace needs on the device, still in host memory.
+struct HostScene {
+    // reference-format two-level layout (kept for the layout entry points)
+    RefLevels ref;
+    // device formats
+    std::vector<WideNode> nodes;
+    std::vector<TriPos> tris;
+    std::vector<TriAttr> attrs;
+    std::vector<uint32_t> slot_prim;   // caller primitive index per triangle slot
+    std::vector<Group> groups;         // top-level leaves
+    std::vector<Surface> surfaces;     // one per instance-surface record (0 = world)
+    std::vector<uint32_t> record_instance;  // surface record -> caller instance + 1
+    std::vector<MatDev> materials;
+    std::vector<float> lights;         // GpuLight rows, 16 floats each
+    std::vector<float> emitters;       // GpuEmitter rows, 16 floats each (header first)
+    std::vector<float> emitter_cdf;
+    std::vector<float> sky_rows, sky_columns;
+    std::vector<float> env;            // lat-long RGBA32F texels
+    uint32_t env_w = 0, env_h = 0;
+    bool env_black = true;
+    uint32_t looped = 0, tree = 0, emitter_count = 0;
+    bool lit = false;
+    RootEntry root;
+    uint32_t slots = 0;
+    uint32_t max_depth = 0;
+};
+
+std::string build_scene(const pfxh_scene& scene, HostScene& out);
+std::string lay_levels(const pfxh_scene& scene, RefLevels& out, std::vector<uint32_t>* slot_prim);
+
+// tree::lay (crates/trace/src/tree.rs:220-243)
+void lay_lights(const pfxh_light* lights, size_t count, std::vector<float>& rows, uint32_t* looped,
+                uint32_t* tree);
+bool light_valid(const pfxh_light& l);
+// lights::build_emitters (crates/trace/src/lights.rs:391-440)
+void build_emitters(const std::vector<float>& tris12, const std::vector<uint32_t>& records, uint32_t looped,
+                    uint32_t tree, std::vector<float>& entries, std::vector<float>& cdf);
+
+}  // namespace pfxh

@@ -1,0 +1,614 @@
+// libpfxhip: the C ABI of include/pfxhip.h over the wavefront kernels.
+//
+// A pfxh_trace mirrors `pfx_trace::Trace` (crates/trace/src/gpu.rs:309-342):
+// it owns the scene buffers, the accumulation buffer and one HIP stream;
+// `sample` accumulates onto the samples already taken and every setter
+// resets the count (gpu.rs:2810-2940). The reference's ~10 ms pacing of
+// submissions (gpu.rs:2992-3107) has no counterpart: the compute queue is
+// not shared with a display, so a call runs its waves back to back.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/pfxhip.h"
+#include "kernels.h"
+#include "layout.h"
+#include "vec3.h"
+
+using namespace pfxh;
+
+namespace {
+
+void put_err(char* err, size_t n, const std::string& s) {
+    if (!err || n == 0) return;
+    size_t k = std::min(n - 1, s.size());
+    memcpy(err, s.data(), k);
+    err[k] = 0;
+}
+
+struct DeviceError {
+    pfxh_status status;
+    std::string what;
+};
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) throw DeviceError{PFXH_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+}  // namespace
+
+struct pfxh_trace {
+    HostScene host;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    std::vector<void*> allocations;
+    uint64_t gpu_bytes = 0;
+    SceneDev S{};
+    WaveDev W{};
+    AccumDev A{};
+    uint32_t* qa = nullptr;
+    uint32_t* qb = nullptr;
+    uint32_t* active = nullptr;
+    uint32_t* host_counters = nullptr;  // pinned
+    float4* snaps = nullptr;
+    float4* images = nullptr;           // color, albedo, normal for readback
+    pfxh_probe_hit* probe = nullptr;
+    uint32_t width = 0, height = 0, samples = 0;
+    uint32_t capacity = 0;
+    bool any_frozen = false;
+    bool counting = false;
+    pfxh_stats stats{};
+
+    void* alloc(size_t bytes, const char* what) {
+        void* p = nullptr;
+        if (bytes == 0) bytes = 16;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw DeviceError{PFXH_REFUSED, std::string("the tracer's ") + what + " need " + std::to_string(bytes) +
+                                                " bytes on the GPU: " + hipGetErrorString(e)};
+        }
+        allocations.push_back(p);
+        gpu_bytes += bytes;
+        return p;
+    }
+    template <class T>
+    T* upload(const std::vector<T>& v, const char* what) {
+        T* p = (T*)alloc(v.size() * sizeof(T), what);
+        if (!v.empty()) HIP_TRY(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return p;
+    }
+    ~pfxh_trace() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : allocations) (void)hipFree(p);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (host_counters) (void)hipHostFree(host_counters);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+uint32_t spill_threads_for() {
+    int device = 0, cus = 256;
+    if (hipGetDevice(&device) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+    }
+    // 16 traversal blocks per CU keep every SIMD fed; the overflow stack is
+    // sized for exactly that many threads.
+    return (uint32_t)cus * 16u * (uint32_t)TRACE_BLOCK;
+}
+
+void setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& detail, uint32_t width, uint32_t height) {
+    std::string e = build_scene(scene, t->host);
+    if (!e.empty()) throw DeviceError{PFXH_SETUP, e};
+    const HostScene& h = t->host;
+    HIP_TRY(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+    for (auto& ev : t->ev) HIP_TRY(hipEventCreate(&ev));
+    HIP_TRY(hipHostMalloc((void**)&t->host_counters, C_COUNT * sizeof(uint32_t), hipHostMallocDefault));
+    SceneDev& S = t->S;
+    S.nodes = t->upload(h.nodes, "bvh nodes");
+    S.tris = t->upload(h.tris, "triangles");
+    S.attrs = t->upload(h.attrs, "triangle attributes");
+    S.groups = t->upload(h.groups, "instances");
+    S.surfaces = t->upload(h.surfaces, "instance surfaces");
+    S.materials = t->upload(h.materials, "materials");
+    S.lights = (const float4*)t->upload(h.lights, "lights");
+    S.emitters = (const float4*)t->upload(h.emitters, "emitters");
+    S.cdf = t->upload(h.emitter_cdf, "emitter cdf");
+    S.sky_rows = t->upload(h.sky_rows, "sky rows");
+    S.sky_columns = t->upload(h.sky_columns, "sky columns");
+    S.env = t->upload(h.env, "sky texels");
+    S.slot_prim = t->upload(h.slot_prim, "primitive map");
+    S.record_instance = t->upload(h.record_instance, "instance map");
+    S.root = h.root;
+    S.env_w = h.env_w;
+    S.env_h = h.env_h;
+    S.env_black = h.env_black ? 1u : 0u;
+    S.lit = h.lit ? 1u : 0u;
+    S.looped = h.looped;
+    S.tree = h.tree;
+    S.emitter_count = h.emitter_count;
+    S.light_slots = h.lit ? 1u + h.looped + (h.tree > 0 ? 1u : 0u) : 0u;
+    S.entries = (scene.sun.intensity > 0.0f ? 1u : 0u) + (h.env_black ? 0u : 1u) +
+                (h.lit ? ((h.emitter_count > 0 ? 1u : 0u) + h.looped + (h.tree > 0 ? 1u : 0u)) : 0u);
+    if (S.entries > 15u) throw DeviceError{PFXH_SETUP, "too many looped lights for one path vertex (at most 13)"};
+    pfxh_bounces b = detail.bounces;
+    if (b.total > 64 || b.diffuse > 64 || b.glossy > 64 || b.transmission > 64)
+        throw DeviceError{PFXH_SETUP, "invalid trace bounces"};
+    S.total = b.total;
+    S.diffuse = b.diffuse;
+    S.glossy = b.glossy;
+    S.transmission = b.transmission;
+    for (int k = 0; k < 3; ++k) {
+        S.sun_dir[k] = scene.sun.direction[k];
+        S.sun_color[k] = scene.sun.color[k];
+        S.origin[k] = scene.camera.origin[k];
+        S.forward[k] = scene.camera.forward[k];
+        S.right[k] = scene.camera.right[k];
+        S.up[k] = scene.camera.up[k];
+    }
+    S.sun_intensity = scene.sun.intensity;
+    S.sun_w = pfx::fm_cos(detail.sun_radius_deg * (3.14159265358979f / 180.0f));
+    S.lens[0] = detail.lens_shift[0];
+    S.lens[1] = detail.lens_shift[1];
+    S.lens[2] = detail.aperture * 0.5f;
+    S.lens[3] = detail.focus_distance;
+    S.window[0] = width;
+    S.window[1] = height;
+    S.window[2] = 0;
+    S.window[3] = 0;
+    S.width = width;
+    S.height = height;
+    t->width = width;
+    t->height = height;
+    size_t pixels = (size_t)width * height;
+    // a wave holds every active pixel at least once
+    size_t want = detail.wave_paths ? detail.wave_paths : std::max<size_t>(pixels, (size_t)1 << 22);
+    want = std::max(want, pixels);
+    if (want >= ((size_t)1 << 28)) throw DeviceError{PFXH_REFUSED, "a wave of more than 2^28 paths does not fit the queues"};
+    t->capacity = (uint32_t)want;
+    size_t cap = t->capacity;
+    WaveDev& W = t->W;
+    W.capacity = t->capacity;
+    W.o = (float4*)t->alloc(cap * 16, "path origins");
+    W.d = (float4*)t->alloc(cap * 16, "path directions");
+    W.thr = (float4*)t->alloc(cap * 16, "path throughputs");
+    W.res = (float4*)t->alloc(cap * 16, "path results");
+    W.alb = (float4*)t->alloc(cap * 16, "path albedos");
+    W.nrm = (float4*)t->alloc(cap * 16, "path normals");
+    W.hit = (float4*)t->alloc(cap * 16, "path hits");
+    W.hit_rec = (uint32_t*)t->alloc(cap * 4, "path hit records");
+    size_t entries = std::max<uint32_t>(S.entries, 1u);
+    W.e_o = (float4*)t->alloc(cap * entries * 16, "shadow ray origins");
+    W.e_d = (float4*)t->alloc(cap * entries * 16, "shadow ray directions");
+    W.e_c = (float4*)t->alloc(cap * entries * 16, "shadow ray contributions");
+    t->qa = (uint32_t*)t->alloc(cap * 4, "ray queue");
+    t->qb = (uint32_t*)t->alloc(cap * 4, "ray queue");
+    W.q_sh = (uint32_t*)t->alloc(cap * 4, "shadow queue");
+    W.counters = (uint32_t*)t->alloc(C_COUNT * 4, "counters");
+    W.spill_threads = spill_threads_for();
+    W.spill = (uint2*)t->alloc((size_t)W.spill_threads * (STACK_DEPTH - STACK_SHORT) * 8, "traversal stacks");
+    t->active = (uint32_t*)t->alloc(pixels * 4, "active pixels");
+    AccumDev& A = t->A;
+    A.mode = detail.accumulate;
+    if (A.mode > 2u) throw DeviceError{PFXH_SETUP, "unknown accumulation mode"};
+    A.sum = (float4*)t->alloc(pixels * 48, "accumulation");
+    A.comp = A.mode == PFXH_ACCUM_COMPENSATED ? (float4*)t->alloc(pixels * 48, "accumulation compensation") : nullptr;
+    A.sum64 = A.mode == 2u ? (double*)t->alloc(pixels * 96, "fp64 accumulation") : nullptr;
+    t->images = (float4*)t->alloc(pixels * 48, "output images");
+    launch_zero(t->stream, W.counters, C_COUNT);
+    launch_clear(t->stream, A, (uint32_t)pixels);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(t->stream));
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0f;
+    }
+    return ms;
+}
+
+void sample(pfxh_trace* t, uint32_t count, uint32_t seed) {
+    if (count == 0) return;
+    hipStream_t s = t->stream;
+    uint32_t pixels = t->width * t->height;
+    if (pixels == 0) {
+        t->samples += count;
+        return;
+    }
+    uint32_t acc = t->samples;
+    if (acc == 0) {
+        launch_clear(s, t->A, pixels);
+        t->any_frozen = false;
+    }
+    const uint32_t* active = nullptr;
+    uint32_t npix = pixels;
+    if (t->any_frozen) {
+        launch_zero(s, t->W.counters + C_ACTIVE, 1);
+        launch_active(s, t->A, pixels, t->active, t->W.counters + C_ACTIVE);
+        HIP_TRY(hipMemcpyAsync(t->host_counters + C_ACTIVE, t->W.counters + C_ACTIVE, 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        npix = t->host_counters[C_ACTIVE];
+        active = t->active;
+    }
+    uint32_t grid = t->W.spill_threads / TRACE_BLOCK;
+    uint32_t done = 0;
+    while (done < count && npix > 0) {
+        uint32_t ns = std::min<uint32_t>(count - done, std::max<uint32_t>(t->capacity / npix, 1u));
+        launch_camera(s, t->S, t->W, active, npix, ns, acc + done, seed);
+        t->stats.paths += (uint64_t)npix * ns;
+        uint32_t n = npix * ns;
+        const uint32_t* queue = nullptr;
+        uint32_t* outq = t->qa;
+        bool shadow_pending = false;
+        for (uint32_t bounce = 0; bounce <= t->S.total + 1u && n > 0; ++bounce) {
+            WaveDev W = t->W;
+            W.q_next = outq;
+            HIP_TRY(hipEventRecord(t->ev[0], s));
+            launch_extend(s, t->S, W, queue, n, t->counting, grid);
+            HIP_TRY(hipEventRecord(t->ev[1], s));
+            launch_zero(s, W.counters, 2);
+            launch_shade(s, t->S, W, queue, n, bounce);
+            HIP_TRY(hipEventRecord(t->ev[2], s));
+            HIP_TRY(hipMemcpyAsync(t->host_counters, W.counters, 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            HIP_TRY(hipGetLastError());
+            t->stats.closest_rays += n;
+            t->stats.trace_ms += elapsed(t->ev[0], t->ev[1]);
+            t->stats.launches += 1;
+            t->stats.shade_ms += elapsed(t->ev[1], t->ev[2]);
+            if (shadow_pending) t->stats.shadow_ms += elapsed(t->ev[3], t->ev[4]);
+            shadow_pending = false;
+            uint32_t n_next = t->host_counters[C_NEXT], n_sh = t->host_counters[C_SHADOW];
+            if (n_sh) {
+                HIP_TRY(hipEventRecord(t->ev[3], s));
+                launch_shadow(s, t->S, W, n_sh, t->counting, grid);
+                HIP_TRY(hipEventRecord(t->ev[4], s));
+                shadow_pending = true;
+            }
+            queue = outq;
+            outq = (outq == t->qa) ? t->qb : t->qa;
+            n = n_next;
+        }
+        HIP_TRY(hipEventRecord(t->ev[5], s));
+        launch_finish(s, t->W, t->A, active, npix, ns, (float)(acc + done + ns));
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipGetLastError());
+        if (shadow_pending) t->stats.shadow_ms += elapsed(t->ev[3], t->ev[4]);
+        done += ns;
+    }
+    t->samples += count;
+}
+
+template <class F>
+pfxh_status guarded(char* err, size_t err_len, F&& f) {
+    try {
+        f();
+        return PFXH_OK;
+    } catch (const DeviceError& e) {
+        put_err(err, err_len, e.what);
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        put_err(err, err_len, "host memory is exhausted");
+        return PFXH_REFUSED;
+    }
+}
+
+bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+}  // namespace
+
+extern "C" {
+
+const char* pfxh_version(void) { return "pfxhip 0.1.0 (gfx950 wavefront tracer)"; }
+
+pfxh_status pfxh_trace_new(const pfxh_scene* scene, const pfxh_detail* detail, uint32_t width, uint32_t height,
+                           pfxh_trace** out, char* err, size_t err_len) {
+    if (!scene || !detail || !out) {
+        put_err(err, err_len, "null argument");
+        return PFXH_INVALID;
+    }
+    *out = nullptr;
+    if (!finite3(scene->camera.origin) || !finite3(scene->camera.forward) || !finite3(scene->camera.right) ||
+        !finite3(scene->camera.up)) {
+        put_err(err, err_len, "camera contains invalid parameters");
+        return PFXH_SETUP;
+    }
+    if ((uint64_t)width * height > 0xffffffffull) {
+        put_err(err, err_len, "the frame is too large");
+        return PFXH_REFUSED;
+    }
+    pfxh_trace* t = new pfxh_trace();
+    pfxh_status st = guarded(err, err_len, [&] { setup(t, *scene, *detail, width, height); });
+    if (st != PFXH_OK) {
+        delete t;
+        return st;
+    }
+    *out = t;
+    return PFXH_OK;
+}
+
+pfxh_status pfxh_trace_sample(pfxh_trace* t, uint32_t count, uint32_t seed, char* err, size_t err_len) {
+    if (!t) return PFXH_INVALID;
+    return guarded(err, err_len, [&] { sample(t, count, seed); });
+}
+
+pfxh_status pfxh_trace_sample_adaptive(pfxh_trace* t, const pfxh_adaptive* a, uint32_t seed, pfxh_converged* out,
+                                       char* err, size_t err_len) {
+    if (!t || !a || !out) return PFXH_INVALID;
+    if (!(std::isfinite(a->threshold) && a->threshold > 0.0f && a->min_samples >= 2 &&
+          a->max_samples >= a->min_samples && std::isfinite(a->growth) && a->growth > 1.0f)) {
+        put_err(err, err_len,
+                "adaptive sampling needs a positive threshold, a growth over 1 and at least 2 samples");
+        return PFXH_INVALID;
+    }
+    return guarded(err, err_len, [&] {
+        uint32_t pixels = t->width * t->height;
+        if (!t->snaps) t->snaps = (float4*)t->alloc((size_t)pixels * 32, "convergence snapshots");
+        launch_zero(t->stream, (uint32_t*)t->snaps, pixels * 8u);
+        t->samples = 0;
+        memset(out, 0, sizeof *out);
+        out->pixels = pixels;
+        // adaptive.rs:46-63 rounds
+        std::vector<uint32_t> rounds{std::max<uint32_t>(a->min_samples / 2, 1)};
+        for (;;) {
+            uint32_t last = rounds.back();
+            if (last >= a->max_samples) break;
+            float grown = ceilf((float)last * a->growth);
+            uint32_t next = grown >= 4294967296.0f ? 0xffffffffu : (uint32_t)grown;
+            next = std::max(next, last + 1);
+            next = std::max(next, rounds.size() == 1 ? a->min_samples : 0u);
+            next = std::min(next, a->max_samples);
+            rounds.push_back(next);
+        }
+        for (size_t step = 0; step < rounds.size(); ++step) {
+            uint32_t target = rounds[step];
+            uint32_t count = target > t->samples ? target - t->samples : 0;
+            sample(t, count, seed);
+            launch_measure(t->stream, t->A, t->snaps, t->width, t->height);
+            uint64_t running = pixels;
+            if (step > 0) {
+                launch_zero(t->stream, t->W.counters + C_RUNNING, 1);
+                launch_mark(t->stream, t->A, t->snaps, t->width, t->height, a->min_samples, a->threshold,
+                            t->W.counters + C_RUNNING);
+                HIP_TRY(hipMemcpyAsync(t->host_counters + C_RUNNING, t->W.counters + C_RUNNING, 4,
+                                       hipMemcpyDeviceToHost, t->stream));
+                HIP_TRY(hipStreamSynchronize(t->stream));
+                HIP_TRY(hipGetLastError());
+                running = t->host_counters[C_RUNNING];
+                t->any_frozen = running < pixels;
+            }
+            if (out->round_count < PFXH_MAX_ROUNDS) {
+                out->samples[out->round_count] = target;
+                out->active[out->round_count] = running;
+                out->round_count += 1;
+            }
+            if (running == 0) break;
+        }
+    });
+}
+
+pfxh_status pfxh_trace_readback(pfxh_trace* t, float* color, float* albedo, float* normal, char* err, size_t err_len) {
+    if (!t) return PFXH_INVALID;
+    return guarded(err, err_len, [&] {
+        size_t pixels = (size_t)t->width * t->height;
+        if (!pixels) return;
+        float4* c = t->images;
+        float4* a = t->images + pixels;
+        float4* n = t->images + 2 * pixels;
+        launch_develop(t->stream, t->A, (uint32_t)pixels, c, a, n);
+        if (color) HIP_TRY(hipMemcpyAsync(color, c, pixels * 16, hipMemcpyDeviceToHost, t->stream));
+        if (albedo) HIP_TRY(hipMemcpyAsync(albedo, a, pixels * 16, hipMemcpyDeviceToHost, t->stream));
+        if (normal) HIP_TRY(hipMemcpyAsync(normal, n, pixels * 16, hipMemcpyDeviceToHost, t->stream));
+        HIP_TRY(hipStreamSynchronize(t->stream));
+        HIP_TRY(hipGetLastError());
+    });
+}
+
+pfxh_status pfxh_trace_set_camera(pfxh_trace* t, const pfxh_camera* c, char* err, size_t err_len) {
+    if (!t || !c) return PFXH_INVALID;
+    if (!finite3(c->origin) || !finite3(c->forward) || !finite3(c->right) || !finite3(c->up)) {
+        put_err(err, err_len, "camera contains invalid parameters");
+        return PFXH_INVALID;
+    }
+    for (int k = 0; k < 3; ++k) {
+        t->S.origin[k] = c->origin[k];
+        t->S.forward[k] = c->forward[k];
+        t->S.right[k] = c->right[k];
+        t->S.up[k] = c->up[k];
+    }
+    t->samples = 0;
+    return PFXH_OK;
+}
+
+pfxh_status pfxh_trace_set_window(pfxh_trace* t, uint32_t fw, uint32_t fh, uint32_t x, uint32_t y, char* err,
+                                  size_t err_len) {
+    if (!t) return PFXH_INVALID;
+    if ((uint64_t)x + t->width > fw || (uint64_t)y + t->height > fh) {
+        put_err(err, err_len, "the window must lie inside the full frame");
+        return PFXH_INVALID;
+    }
+    if ((uint64_t)fw * fh > 0xffffffffull) {
+        put_err(err, err_len, "the full frame is too large");
+        return PFXH_INVALID;
+    }
+    t->S.window[0] = fw;
+    t->S.window[1] = fh;
+    t->S.window[2] = x;
+    t->S.window[3] = y;
+    t->samples = 0;
+    return PFXH_OK;
+}
+
+pfxh_status pfxh_trace_set_sun(pfxh_trace* t, const pfxh_sun* sun, char* err, size_t err_len) {
+    if (!t || !sun) return PFXH_INVALID;
+    if (!finite3(sun->direction) || !finite3(sun->color) || !std::isfinite(sun->intensity)) {
+        put_err(err, err_len, "sun contains invalid parameters");
+        return PFXH_INVALID;
+    }
+    uint32_t needed = (sun->intensity > 0.0f ? 1u : 0u) + (t->host.env_black ? 0u : 1u) +
+                      (t->host.lit ? ((t->host.emitter_count > 0 ? 1u : 0u) + t->host.looped + (t->host.tree > 0 ? 1u : 0u)) : 0u);
+    if (needed > t->S.entries) {
+        put_err(err, err_len, "the trace was built without a sun; rebuild it with one to light it");
+        return PFXH_INVALID;
+    }
+    for (int k = 0; k < 3; ++k) {
+        t->S.sun_dir[k] = sun->direction[k];
+        t->S.sun_color[k] = sun->color[k];
+    }
+    t->S.sun_intensity = sun->intensity;
+    t->samples = 0;
+    return PFXH_OK;
+}
+
+pfxh_status pfxh_trace_set_lens(pfxh_trace* t, float sx, float sy, float aperture, float focus, char* err,
+                                size_t err_len) {
+    if (!t) return PFXH_INVALID;
+    if (!std::isfinite(sx) || !std::isfinite(sy) || !std::isfinite(aperture) || !std::isfinite(focus) ||
+        aperture < 0.0f || (aperture > 0.0f && !(focus > 0.0f))) {
+        put_err(err, err_len, "invalid lens");
+        return PFXH_INVALID;
+    }
+    t->S.lens[0] = sx;
+    t->S.lens[1] = sy;
+    t->S.lens[2] = aperture * 0.5f;
+    t->S.lens[3] = focus;
+    t->samples = 0;
+    return PFXH_OK;
+}
+
+uint32_t pfxh_trace_samples(const pfxh_trace* t) { return t ? t->samples : 0; }
+uint64_t pfxh_trace_gpu_bytes(const pfxh_trace* t) { return t ? t->gpu_bytes : 0; }
+
+pfxh_status pfxh_trace_probe(pfxh_trace* t, pfxh_probe_hit* out, char* err, size_t err_len) {
+    if (!t || !out) return PFXH_INVALID;
+    return guarded(err, err_len, [&] {
+        size_t pixels = (size_t)t->width * t->height;
+        if (!pixels) return;
+        if (!t->probe) t->probe = (pfxh_probe_hit*)t->alloc(pixels * sizeof(pfxh_probe_hit), "probe");
+        launch_probe(t->stream, t->S, t->W, t->probe);
+        HIP_TRY(hipMemcpyAsync(out, t->probe, pixels * sizeof(pfxh_probe_hit), hipMemcpyDeviceToHost, t->stream));
+        HIP_TRY(hipStreamSynchronize(t->stream));
+        HIP_TRY(hipGetLastError());
+    });
+}
+
+pfxh_status pfxh_trace_stats(const pfxh_trace* t, pfxh_stats* out) {
+    if (!t || !out) return PFXH_INVALID;
+    *out = t->stats;
+    uint32_t c[C_COUNT];
+    if (hipMemcpy(c, t->W.counters, sizeof c, hipMemcpyDeviceToHost) != hipSuccess) {
+        (void)hipGetLastError();
+        return PFXH_DEVICE;
+    }
+    auto u64 = [&](int k) { return (uint64_t)c[k] | ((uint64_t)c[k + 1] << 32); };
+    out->shadow_rays = u64(C_SHADOW_RAYS);
+    out->nodes_visited = u64(C_NODES);
+    out->prims_tested = u64(C_PRIMS);
+    return PFXH_OK;
+}
+
+void pfxh_trace_reset_stats(pfxh_trace* t) {
+    if (!t) return;
+    t->stats = pfxh_stats{};
+    launch_zero(t->stream, t->W.counters, C_COUNT);
+    (void)hipStreamSynchronize(t->stream);
+}
+
+void pfxh_trace_set_counting(pfxh_trace* t, int on) {
+    if (t) t->counting = on != 0;
+}
+
+void pfxh_trace_free(pfxh_trace* t) { delete t; }
+
+// ---- host-only layout -------------------------------------------------------------
+
+pfxh_status pfxh_layout_bvh(const pfxh_triangle* tris, uint32_t count, pfxh_bvh_node* nodes, uint32_t* node_count,
+                            uint32_t* order, char* err, size_t err_len) {
+    if ((!tris && count) || !node_count) return PFXH_INVALID;
+    std::vector<float> v((size_t)count * 9);
+    for (uint32_t i = 0; i < count; ++i) {
+        for (int k = 0; k < 9; ++k)
+            if (!std::isfinite(tris[i].vertices[k])) {
+                put_err(err, err_len, "invalid trace geometry");
+                return PFXH_SETUP;
+            }
+        memcpy(&v[9 * (size_t)i], tris[i].vertices, 36);
+    }
+    RefBvh b = build_bvh(v.data(), count);
+    *node_count = (uint32_t)b.nodes.size();
+    if (nodes) memcpy(nodes, b.nodes.data(), b.nodes.size() * sizeof(pfxh_bvh_node));
+    if (order) memcpy(order, b.order.data(), b.order.size() * 4);
+    return PFXH_OK;
+}
+
+pfxh_status pfxh_layout_levels(const pfxh_scene* scene, pfxh_bvh_node* nodes, uint32_t* node_count, uint32_t* order,
+                               uint32_t* order_count, char* err, size_t err_len) {
+    if (!scene || !node_count || !order_count) return PFXH_INVALID;
+    RefLevels levels;
+    std::string e;
+    try {
+        e = lay_levels(*scene, levels, nullptr);
+    } catch (const std::bad_alloc&) {
+        e = "host memory is exhausted";
+    }
+    if (!e.empty()) {
+        put_err(err, err_len, e);
+        return PFXH_SETUP;
+    }
+    *node_count = (uint32_t)levels.nodes.size();
+    *order_count = (uint32_t)levels.world_order.size();
+    if (nodes) memcpy(nodes, levels.nodes.data(), levels.nodes.size() * sizeof(pfxh_bvh_node));
+    if (order) memcpy(order, levels.world_order.data(), levels.world_order.size() * 4);
+    return PFXH_OK;
+}
+
+pfxh_status pfxh_layout_lights(const pfxh_light* lights, uint32_t count, float* rows, uint32_t* row_count,
+                               uint32_t* looped, uint32_t* tree, char* err, size_t err_len) {
+    if ((!lights && count) || !row_count || !looped || !tree) return PFXH_INVALID;
+    for (uint32_t i = 0; i < count; ++i)
+        if (!light_valid(lights[i])) {
+            put_err(err, err_len, "invalid trace light");
+            return PFXH_SETUP;
+        }
+    std::vector<float> r;
+    lay_lights(lights, count, r, looped, tree);
+    *row_count = (uint32_t)(r.size() / 16);
+    if (rows) memcpy(rows, r.data(), r.size() * 4);
+    return PFXH_OK;
+}
+
+pfxh_status pfxh_layout_emitters(const pfxh_scene* scene, float* entries, float* cdf, uint32_t* emitter_count,
+                                 char* err, size_t err_len) {
+    if (!scene || !emitter_count) return PFXH_INVALID;
+    HostScene h;
+    std::string e;
+    try {
+        e = build_scene(*scene, h);
+    } catch (const std::bad_alloc&) {
+        e = "host memory is exhausted";
+    }
+    if (!e.empty()) {
+        put_err(err, err_len, e);
+        return PFXH_SETUP;
+    }
+    *emitter_count = h.emitter_count;
+    if (entries) memcpy(entries, h.emitters.data(), h.emitters.size() * 4);
+    if (cdf && h.emitter_count) memcpy(cdf, h.emitter_cdf.data(), (size_t)h.emitter_count * 4);
+    return PFXH_OK;
+}
+
+}  // extern "C"

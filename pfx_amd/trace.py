@@ -1,0 +1,213 @@
+"""`Trace`: the still-render surface of the reference tracer over libpfxhip.
+
+Mirrors `pfx_trace::Trace` (crates/trace/src/gpu.rs:309-342): build once
+from a scene, `sample` more passes (they accumulate; every setter resets the
+count), `readback` the colour/albedo/normal images (RGBA32F, row-major, the
+same layout as `Output`, gpu.rs:405-410), `probe` primary hits. Errors are
+`TraceError` with `refused` set for limits and memory, as
+`TraceError::{Refused, Setup}` (gpu.rs:268-307); runtime failures raise
+`RuntimeError`, the reference's `Err(String)`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+from .scene import Adaptive, Camera, Detail, Scene, Sun
+
+
+class TraceError(Exception):
+    def __init__(self, message: str, refused: bool):
+        super().__init__(message)
+        self.refused = refused
+
+
+@dataclass
+class Output:
+    """gpu::Output: three (H, W, 4) float32 images."""
+
+    color: np.ndarray
+    albedo: np.ndarray
+    normal: np.ndarray
+
+
+@dataclass
+class Round:
+    samples: int
+    active: int
+
+
+@dataclass
+class Converged:
+    """adaptive::Converged (crates/trace/src/adaptive.rs:66-100)."""
+
+    rounds: list
+    pixels: int
+
+    def max_samples(self) -> int:
+        return self.rounds[-1].samples if self.rounds else 0
+
+    def mean_samples(self) -> float:
+        if self.pixels == 0:
+            return 0.0
+        total, before, running = 0.0, 0, self.pixels
+        for r in self.rounds:
+            total += float(r.samples - before) * running
+            before, running = r.samples, r.active
+        return total / self.pixels
+
+    def converged(self) -> bool:
+        return bool(self.rounds) and self.rounds[-1].active == 0
+
+
+@dataclass
+class Probe:
+    """probe::Probe (crates/trace/src/probe.rs:10-18) plus primitive ids."""
+
+    width: int
+    height: int
+    depth: np.ndarray
+    distance: np.ndarray
+    id: np.ndarray
+    normal: np.ndarray
+    primitive: np.ndarray
+    instance: np.ndarray
+
+
+class Trace:
+    def __init__(self, handle, width: int, height: int):
+        self._h = handle
+        self.width = width
+        self.height = height
+
+    # Trace::new_detailed (crates/trace/src/gpu.rs:1926-1934)
+    @classmethod
+    def new_detailed(cls, scene: Scene, detail: Detail, width: int, height: int) -> "Trace":
+        lib = N.lib()
+        sc, keep = scene.to_c()
+        d = detail.to_c()
+        out = C.c_void_p()
+        err = N.err_buffer()
+        st = lib.pfxh_trace_new(C.byref(sc), C.byref(d), width, height, C.byref(out), err, len(err))
+        del keep
+        if st != N.OK:
+            raise TraceError(err.value.decode(errors="replace"), refused=st == N.REFUSED)
+        return cls(out, width, height)
+
+    @classmethod
+    def new(cls, scene: Scene, width: int, height: int) -> "Trace":
+        return cls.new_detailed(scene, Detail(), width, height)
+
+    def _check(self, st: int, err) -> None:
+        if st != N.OK:
+            raise RuntimeError(err.value.decode(errors="replace") or f"pfxhip status {st}")
+
+    def _handle(self):
+        if self._h is None:
+            raise RuntimeError("the trace is closed")
+        return self._h
+
+    def sample(self, count: int, seed: int) -> None:
+        err = N.err_buffer()
+        self._check(N.lib().pfxh_trace_sample(self._handle(), count, seed, err, len(err)), err)
+
+    def sample_adaptive(self, adaptive: Adaptive, seed: int) -> Converged:
+        err = N.err_buffer()
+        a = adaptive.to_c()
+        out = N.ConvergedC()
+        self._check(
+            N.lib().pfxh_trace_sample_adaptive(self._handle(), C.byref(a), seed, C.byref(out), err, len(err)), err
+        )
+        rounds = [Round(int(out.samples[i]), int(out.active[i])) for i in range(out.round_count)]
+        return Converged(rounds, int(out.pixels))
+
+    def readback(self) -> Output:
+        shape = (self.height, self.width, 4)
+        color, albedo, normal = (np.empty(shape, np.float32) for _ in range(3))
+        err = N.err_buffer()
+        st = N.lib().pfxh_trace_readback(
+            self._handle(), color.ctypes.data, albedo.ctypes.data, normal.ctypes.data, err, len(err)
+        )
+        self._check(st, err)
+        return Output(color, albedo, normal)
+
+    def readback_color(self) -> np.ndarray:
+        color = np.empty((self.height, self.width, 4), np.float32)
+        err = N.err_buffer()
+        self._check(N.lib().pfxh_trace_readback(self._handle(), color.ctypes.data, None, None, err, len(err)), err)
+        return color
+
+    def set_camera(self, camera: Camera) -> None:
+        err = N.err_buffer()
+        c = camera.to_c()
+        self._check(N.lib().pfxh_trace_set_camera(self._handle(), C.byref(c), err, len(err)), err)
+
+    def set_window(self, full: tuple, origin: tuple) -> None:
+        err = N.err_buffer()
+        st = N.lib().pfxh_trace_set_window(self._handle(), full[0], full[1], origin[0], origin[1], err, len(err))
+        self._check(st, err)
+
+    def set_sun(self, sun: Sun) -> None:
+        err = N.err_buffer()
+        s = sun.to_c()
+        self._check(N.lib().pfxh_trace_set_sun(self._handle(), C.byref(s), err, len(err)), err)
+
+    def set_lens(self, shift=(0.0, 0.0), aperture: float = 0.0, focus_distance: float = 0.0) -> None:
+        err = N.err_buffer()
+        st = N.lib().pfxh_trace_set_lens(self._handle(), shift[0], shift[1], aperture, focus_distance, err, len(err))
+        self._check(st, err)
+
+    def samples(self) -> int:
+        return int(N.lib().pfxh_trace_samples(self._handle()))
+
+    def gpu_bytes(self) -> int:
+        return int(N.lib().pfxh_trace_gpu_bytes(self._handle()))
+
+    def probe(self) -> Probe:
+        hits = np.zeros(self.width * self.height, dtype=N.PROBE_HIT)
+        err = N.err_buffer()
+        self._check(N.lib().pfxh_trace_probe(self._handle(), hits.ctypes.data, err, len(err)), err)
+        shape = (self.height, self.width)
+        return Probe(
+            self.width,
+            self.height,
+            hits["depth"].reshape(shape),
+            hits["distance"].reshape(shape),
+            hits["material_id"].reshape(shape),
+            hits["normal"].reshape(shape + (3,)),
+            hits["primitive"].reshape(shape),
+            hits["instance"].reshape(shape),
+        )
+
+    def stats(self) -> dict:
+        s = N.StatsC()
+        st = N.lib().pfxh_trace_stats(self._handle(), C.byref(s))
+        if st != N.OK:
+            raise RuntimeError(f"pfxhip stats failed ({st})")
+        return {name: getattr(s, name) for name, _ in N.StatsC._fields_}
+
+    def reset_stats(self) -> None:
+        N.lib().pfxh_trace_reset_stats(self._handle())
+
+    def set_counting(self, on: bool) -> None:
+        N.lib().pfxh_trace_set_counting(self._handle(), 1 if on else 0)
+
+    def close(self) -> None:
+        if self._h is not None:
+            N.lib().pfxh_trace_free(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
