@@ -272,7 +272,7 @@ pfxh_status pfxh_layout_levels(const pfxh_scene* scene, pfxh_bvh_node* nodes,
 
 /* tree::lay (crates/trace/src/tree.rs:220-243): light rows (16 floats
  * each, `GpuLight`) for [non-treed][treed in tree order][tree nodes].
- * rows needs room for 2*count+1 rows. */
+ * rows needs room for 3*count rows (lights plus fewer than 2*count nodes). */
 pfxh_status pfxh_layout_lights(const pfxh_light* lights, uint32_t count, float* rows,
                                uint32_t* row_count, uint32_t* looped, uint32_t* tree, char* err,
                                size_t err_len);

@@ -26,8 +26,8 @@ Bounds bounds_empty() {
 // bvh.rs:54-59 (f32::min / f32::max are IEEE minNum / maxNum, as fminf)
 void bounds_point(Bounds& b, const float p[3]) {
     for (int k = 0; k < 3; ++k) {
-        b.lo[k] = fminf(b.lo[k], p[k]);
-        b.hi[k] = fmaxf(b.hi[k], p[k]);
+        b.lo[k] = pfx::minf(b.lo[k], p[k]);
+        b.hi[k] = pfx::maxf(b.hi[k], p[k]);
     }
 }
 // bvh.rs:60-63
@@ -179,8 +179,8 @@ static bool box_hit_cpu(const Node& node, v3 o, v3 d, float limit) {
         } else {
             float a = (node.lo[axis] - org) / dir;
             float b = (node.hi[axis] - org) / dir;
-            near_ = fmaxf(near_, fminf(a, b));
-            far_ = fminf(far_, fmaxf(a, b));
+            near_ = pfx::maxf(near_, pfx::minf(a, b));
+            far_ = pfx::minf(far_, pfx::maxf(a, b));
             if (near_ > far_) return false;
         }
     }

@@ -62,7 +62,7 @@ bool light_valid(const pfxh_light& l) {
 static void spot_terms(float inner_deg, float outer_deg, float* scale, float* offset) {
     float inner = fm_cos(to_radians(inner_deg));
     float outer = fm_cos(to_radians(outer_deg));
-    *scale = 1.0f / fmaxf(inner - outer, 0.001f);
+    *scale = 1.0f / pfx::maxf(inner - outer, 0.001f);
     *offset = -outer * *scale;
 }
 // lights.rs:154-163
@@ -71,7 +71,7 @@ float light_falloff(const pfxh_light& l, float distance) {
     float ratio = distance / l.range;
     float r2 = ratio * ratio;
     float window = pfx::fm_clamp(1.0f - r2 * r2, 0.0f, 1.0f);
-    float reach = fmaxf(distance, l.radius);
+    float reach = pfx::maxf(distance, l.radius);
     return l.intensity * window * window / (reach * reach);
 }
 // lights.rs:165-181
@@ -120,7 +120,7 @@ static bool cone_empty(const Cone& c) { return c.axis.x == 0.0f && c.axis.y == 0
 // tree.rs:28-41
 static float cone_measure(const Cone& c) {
     if (cone_empty(c)) return 0.0f;
-    float theta_w = fminf(c.theta_o + c.theta_e, PI);
+    float theta_w = pfx::minf(c.theta_o + c.theta_e, PI);
     float cos_o = fm_cos(c.theta_o), sin_o = fm_sin(c.theta_o);
     return TAU_ * (1.0f - cos_o) +
            FRAC_PI_2 * (2.0f * theta_w * sin_o - fm_cos(c.theta_o - 2.0f * theta_w) - 2.0f * c.theta_o * sin_o + cos_o);
@@ -141,8 +141,8 @@ static Cone cone_merge(Cone self, Cone other) {
     }
     float cos_a_b = pfx::dot(a.axis, b.axis);
     float theta_d = fm_acos(pfx::fm_clamp(cos_a_b, -1.0f, 1.0f));
-    float theta_e = fmaxf(a.theta_e, b.theta_e);
-    if (a.theta_o + 5e-4f >= fminf(PI, theta_d + b.theta_o)) return Cone{a.axis, a.theta_o, theta_e};
+    float theta_e = pfx::maxf(a.theta_e, b.theta_e);
+    if (a.theta_o + 5e-4f >= pfx::minf(PI, theta_d + b.theta_o)) return Cone{a.axis, a.theta_o, theta_e};
     float theta_o = (theta_d + a.theta_o + b.theta_o) * 0.5f;
     if (theta_o >= PI) return Cone{a.axis, PI, theta_e};
     v3 axis;
@@ -175,12 +175,12 @@ static Measure measure_empty() {
 static void measure_add(Measure& self, const Measure& other) {
     if (other.energy == 0.0f) return;
     for (int k = 0; k < 3; ++k) {
-        self.min[k] = fminf(self.min[k], other.min[k]);
-        self.max[k] = fmaxf(self.max[k], other.max[k]);
+        self.min[k] = pfx::minf(self.min[k], other.min[k]);
+        self.max[k] = pfx::maxf(self.max[k], other.max[k]);
     }
     self.cone = cone_merge(self.cone, other.cone);
     self.energy += other.energy;
-    self.range = fmaxf(self.range, other.range);
+    self.range = pfx::maxf(self.range, other.range);
 }
 // tree.rs:124-132
 static float measure_calculate(const Measure& m) {
@@ -201,7 +201,7 @@ static Emitter emitter_of(size_t index, const pfxh_light& l) {
     e.index = index;
     Cone cone;
     if (l.kind == PFXH_LIGHT_SPOT) {
-        cone = Cone{rnormalize(load3(l.direction)), 0.0f, fminf(to_radians(l.outer_deg), FRAC_PI_2)};
+        cone = Cone{rnormalize(load3(l.direction)), 0.0f, pfx::minf(to_radians(l.outer_deg), FRAC_PI_2)};
     } else {
         cone = Cone{pfx::mk(0.0f, 0.0f, -1.0f), PI, FRAC_PI_2};
     }
@@ -254,11 +254,11 @@ static Measure should_split(const Emitter* emitters, size_t count, bool* split, 
     float low[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, high[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     for (size_t i = 0; i < count; ++i)
         for (int k = 0; k < 3; ++k) {
-            low[k] = fminf(low[k], emitters[i].centroid[k]);
-            high[k] = fmaxf(high[k], emitters[i].centroid[k]);
+            low[k] = pfx::minf(low[k], emitters[i].centroid[k]);
+            high[k] = pfx::maxf(high[k], emitters[i].centroid[k]);
         }
     float extent[3] = {high[0] - low[0], high[1] - low[1], high[2] - low[2]};
-    float max_extent = fmaxf(fmaxf(fmaxf(extent[0], extent[1]), extent[2]), 0.0f);
+    float max_extent = pfx::maxf(pfx::maxf(pfx::maxf(extent[0], extent[1]), extent[2]), 0.0f);
     Measure measure = measure_empty();
     float total_cost = 0.0f, min_cost = FLT_MAX;
     int split_dim = -1;
@@ -468,7 +468,7 @@ SkyCdf sky_build(const Env& env, uint32_t width, uint32_t height) {
             v3 d = pfx::mk(fm_sin(theta) * fm_cos(phi), fm_cos(theta), fm_sin(theta) * fm_sin(phi));
             v3 color = env_radiance(env, d);
             float c3[3] = {color.x, color.y, color.z};
-            row_total += (double)fmaxf(luminance(c3), 0.0f) * (double)sine;
+            row_total += (double)pfx::maxf(luminance(c3), 0.0f) * (double)sine;
             cdf.columns[(size_t)row * width + col] = (float)row_total;
         }
         if (row_total > 0.0) {

@@ -62,6 +62,7 @@ def lib() -> C.CDLL:
             "or_emitters": (_U, [_P, _U, _U, _U, _P, _P]),
             "or_sky_cdf": (None, [_P, _U, _U, _U, _U, _P, _P]),
             "or_rounds": (_U, [_P, _P, _U]),
+            "or_adaptive_error": (_F, [_P, _F, _P, _F]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(h, name)

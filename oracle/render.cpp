@@ -1017,6 +1017,18 @@ static void sample_rows(Scene& sc, uint32_t count, uint32_t seed, uint32_t row0,
     sc.samples += count;
 }
 
+// adaptive.rs:375-385 (the same formula as converge.wgsl:19-27)
+float adaptive_error(const float prior[4], float prior_count, const float now[4], float count) {
+    if (prior_count <= 0.0f || count <= prior_count) return 1e9f;
+    float scale = sqrtf(1.0f / count) / sqrtf(1.0f / prior_count + 1.0f / (count - prior_count));
+    float gap[4];
+    for (int k = 0; k < 4; ++k) {
+        float rest = (now[k] * count - prior[k] * prior_count) / (count - prior_count);
+        gap[k] = fabsf(prior[k] - rest) * scale;
+    }
+    return (gap[0] + gap[1] + gap[2]) / (1e-4f + sqrtf(fmaxf(now[0] + now[1] + now[2], 0.0f))) + gap[3];
+}
+
 // converge.wgsl:9-30 measure, :32-51 mark
 static uint64_t converge(Scene& sc, const pfxh_adaptive& a, bool mark) {
     size_t pixels = (size_t)sc.width * sc.height;
@@ -1030,16 +1042,7 @@ static uint64_t converge(Scene& sc, const pfxh_adaptive& a, bool mark) {
         float now[4] = {total_of(sc, i, 0) / count, total_of(sc, i, 1) / count, total_of(sc, i, 2) / count,
                         total_of(sc, i, 7) / count};
         float prior = snap[4];
-        float error = 1e9f;
-        if (prior > 0.0f && count > prior) {
-            float scale = sqrtf(1.0f / count) / sqrtf(1.0f / prior + 1.0f / (count - prior));
-            float gap[4];
-            for (int k = 0; k < 4; ++k) {
-                float rest = (now[k] * count - snap[k] * prior) / (count - prior);
-                gap[k] = fabsf(snap[k] - rest) * scale;
-            }
-            error = (gap[0] + gap[1] + gap[2]) / (1e-4f + sqrtf(fmaxf(now[0] + now[1] + now[2], 0.0f))) + gap[3];
-        }
+        float error = adaptive_error(snap, prior, now, count);
         for (int k = 0; k < 4; ++k) snap[k] = now[k];
         snap[4] = count;
         snap[5] = error;
@@ -1247,6 +1250,9 @@ int or_sample_adaptive(void* h, const pfxh_adaptive* a, uint32_t seed, pfxh_conv
         if (running == 0) break;
     }
     return 0;
+}
+float or_adaptive_error(const float* prior, float prior_count, const float* now, float count) {
+    return adaptive_error(prior, prior_count, now, count);
 }
 void or_tree_pick(void* h, const float* p, const float* n, float u, float* out) {
     F2 r = tree_pick(*(Scene*)h, t3(p), t3(n), u);

@@ -179,6 +179,19 @@ PFX_HD float fm_pow(float x, float y) {
     return fm_exp2(y * fm_log2(x));
 }
 
-PFX_HD float fm_clamp(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+// min / max with IEEE minNum/maxNum NaN handling and a fixed answer for
+// equal operands (the first one), so +0 and -0 come out the same on every
+// compiler. Rust's f32::min/max leave that case open.
+PFX_HD float minf(float a, float b) {
+    if (b != b) return a;
+    if (a != a) return b;
+    return b < a ? b : a;
+}
+PFX_HD float maxf(float a, float b) {
+    if (b != b) return a;
+    if (a != a) return b;
+    return b > a ? b : a;
+}
+PFX_HD float fm_clamp(float x, float lo, float hi) { return minf(maxf(x, lo), hi); }
 
 }  // namespace pfx

@@ -42,8 +42,8 @@ struct Box {
     }
     void point(const float* p) {
         for (int k = 0; k < 3; ++k) {
-            lo[k] = fminf(lo[k], p[k]);
-            hi[k] = fmaxf(hi[k], p[k]);
+            lo[k] = pfx::minf(lo[k], p[k]);
+            hi[k] = pfx::maxf(hi[k], p[k]);
         }
     }
     void grow(const Box& b) {
@@ -549,7 +549,7 @@ v3 ld3(const float* p) { return pfx::mk(p[0], p[1], p[2]); }
 void spot_terms(float inner_deg, float outer_deg, float* scale, float* offset) {
     float inner = fm_cos(to_radians(inner_deg));
     float outer = fm_cos(to_radians(outer_deg));
-    *scale = 1.0f / fmaxf(inner - outer, 0.001f);
+    *scale = 1.0f / pfx::maxf(inner - outer, 0.001f);
     *offset = -outer * *scale;
 }
 // lights.rs:183-231, 16 floats: a, b, c, d
@@ -585,7 +585,7 @@ struct Cone {
     bool empty() const { return axis.x == 0.0f && axis.y == 0.0f && axis.z == 0.0f; }
     float measure() const {  // tree.rs:28-41
         if (empty()) return 0.0f;
-        float theta_w = fminf(theta_o + theta_e, PI_F);
+        float theta_w = pfx::minf(theta_o + theta_e, PI_F);
         float cos_o = fm_cos(theta_o), sin_o = fm_sin(theta_o);
         return TAU_F * (1.0f - cos_o) +
                HALF_PI * (2.0f * theta_w * sin_o - fm_cos(theta_o - 2.0f * theta_w) - 2.0f * theta_o * sin_o + cos_o);
@@ -602,8 +602,8 @@ Cone merge(Cone self, Cone other) {  // tree.rs:43-81
     }
     float cos_a_b = pfx::dot(a.axis, b.axis);
     float theta_d = fm_acos(pfx::fm_clamp(cos_a_b, -1.0f, 1.0f));
-    float theta_e = fmaxf(a.theta_e, b.theta_e);
-    if (a.theta_o + 5e-4f >= fminf(PI_F, theta_d + b.theta_o)) return Cone{a.axis, a.theta_o, theta_e};
+    float theta_e = pfx::maxf(a.theta_e, b.theta_e);
+    if (a.theta_o + 5e-4f >= pfx::minf(PI_F, theta_d + b.theta_o)) return Cone{a.axis, a.theta_o, theta_e};
     float theta_o = (theta_d + a.theta_o + b.theta_o) * 0.5f;
     if (theta_o >= PI_F) return Cone{a.axis, PI_F, theta_e};
     v3 axis;
@@ -635,12 +635,12 @@ struct Measure {
     void add(const Measure& o) {  // tree.rs:111-122
         if (o.energy == 0.0f) return;
         for (int k = 0; k < 3; ++k) {
-            min[k] = fminf(min[k], o.min[k]);
-            max[k] = fmaxf(max[k], o.max[k]);
+            min[k] = pfx::minf(min[k], o.min[k]);
+            max[k] = pfx::maxf(max[k], o.max[k]);
         }
         cone = merge(cone, o.cone);
         energy += o.energy;
-        range = fmaxf(range, o.range);
+        range = pfx::maxf(range, o.range);
     }
     float calculate() const {  // tree.rs:124-132
         if (energy == 0.0f) return 0.0f;
@@ -697,11 +697,11 @@ Measure should_split(const LightEntry* e, size_t count, bool* split, size_t* mid
     float low[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, high[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     for (size_t i = 0; i < count; ++i)
         for (int k = 0; k < 3; ++k) {
-            low[k] = fminf(low[k], e[i].centroid[k]);
-            high[k] = fmaxf(high[k], e[i].centroid[k]);
+            low[k] = pfx::minf(low[k], e[i].centroid[k]);
+            high[k] = pfx::maxf(high[k], e[i].centroid[k]);
         }
     float extent[3] = {high[0] - low[0], high[1] - low[1], high[2] - low[2]};
-    float max_extent = fmaxf(fmaxf(fmaxf(extent[0], extent[1]), extent[2]), 0.0f);
+    float max_extent = pfx::maxf(pfx::maxf(pfx::maxf(extent[0], extent[1]), extent[2]), 0.0f);
     Measure measure = Measure::empty();
     float total_cost = 0.0f, min_cost = FLT_MAX;
     int split_dim = -1;
@@ -893,7 +893,7 @@ void lay_lights(const pfxh_light* lights, size_t n, std::vector<float>& rows, ui
         LightEntry e;
         e.index = i;
         Cone cone = l.kind == PFXH_LIGHT_SPOT
-                        ? Cone{rnormalize(ld3(l.direction)), 0.0f, fminf(to_radians(l.outer_deg), HALF_PI)}
+                        ? Cone{rnormalize(ld3(l.direction)), 0.0f, pfx::minf(to_radians(l.outer_deg), HALF_PI)}
                         : Cone{pfx::mk(0.0f, 0.0f, -1.0f), PI_F, HALF_PI};
         float s[3];
         for (int k = 0; k < 3; ++k) {
@@ -1060,7 +1060,7 @@ std::string build_scene(const pfxh_scene& sc, HostScene& out) {
             v3 d = pfx::mk(fm_sin(theta) * fm_cos(phi), fm_cos(theta), fm_sin(theta) * fm_sin(phi));
             v3 c = env_lookup(out.env, out.env_w, out.env_h, d);
             float c3[3] = {c.x, c.y, c.z};
-            row_total += (double)fmaxf(lum(c3), 0.0f) * (double)sine;
+            row_total += (double)pfx::maxf(lum(c3), 0.0f) * (double)sine;
             out.sky_columns[(size_t)row * CW + col] = (float)row_total;
         }
         if (row_total > 0.0) {

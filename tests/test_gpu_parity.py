@@ -1,0 +1,133 @@
+"""GPU parity: the HIP wavefront tracer against the CPU oracle on the same
+scenes and seeds, through the C ABI.
+
+Bars (for integer/index outputs and, with identical float operation order on
+both sides, for the images): primary-hit primitive ids and instance ids are
+bit-exact; images at equal spp match within relMSE <= 1e-4 (tests also report
+the exact-match fraction). Full-size properties are checked in
+test_gpu_properties.py.
+"""
+import numpy as np
+import pytest
+
+from oracle import Oracle
+from pfx_amd import Bounces, Detail, Trace, scenes
+from pfx_amd.scene import ACCUM_COMPENSATED, ACCUM_PLAIN
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_mse(x, r):
+    return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
+
+
+def small_terrain(width=96, height=54):
+    return scenes.terrain(seed=3, grid=(64, 32), size=(32.0, 16.0), amplitude=4.0, meshes=3,
+                          mesh_tris=(200, 600), instances=12, lights=24, width=width, height=height,
+                          light_range=12.0)
+
+
+CASES = {
+    "box_spheres": (lambda w, h: scenes.box_spheres(width=w, height=h), Bounces.deep(4), 64, 64),
+    "terrain": (lambda w, h: small_terrain(w, h), Bounces.deep(6), 96, 54),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_probe_primitive_ids_bit_exact(case):
+    make, bounces, w, h = CASES[case]
+    scene = make(w, h)
+    detail = Detail(bounces=bounces)
+    print(f"probe parity {case} {w}x{h}", flush=True)
+    ref = Oracle(scene, detail, w, h).probe()
+    with Trace.new_detailed(scene, detail, w, h) as t:
+        got = t.probe()
+    assert np.array_equal(got.primitive, ref["primitive"])
+    assert np.array_equal(got.instance, ref["instance"])
+    assert np.array_equal(got.id, ref["material_id"])
+    np.testing.assert_array_equal(got.distance, ref["distance"])
+    np.testing.assert_allclose(got.normal, ref["normal"], rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("accumulate", [ACCUM_PLAIN, ACCUM_COMPENSATED])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_images_match_oracle_at_equal_spp(case, accumulate):
+    make, bounces, w, h = CASES[case]
+    scene = make(w, h)
+    detail = Detail(bounces=bounces, accumulate=accumulate)
+    spp, seed = 8, 7
+    print(f"image parity {case} {w}x{h} spp={spp} accumulate={accumulate}", flush=True)
+    o = Oracle(scene, detail, w, h)
+    o.sample(spp, seed)
+    rc, ra, rn = o.readback()
+    with Trace.new_detailed(scene, detail, w, h) as t:
+        t.sample(spp, seed)
+        out = t.readback()
+        assert t.samples() == spp
+    exact = float(np.mean(out.color == rc))
+    print(f"  exact fraction {exact:.6f} relMSE {rel_mse(out.color[..., :3], rc[..., :3]):.3e}")
+    assert rel_mse(out.color[..., :3], rc[..., :3]) <= 1e-4
+    assert rel_mse(out.albedo, ra) <= 1e-6
+    np.testing.assert_allclose(out.normal, rn, rtol=0, atol=1e-4)
+    assert exact > 0.99
+
+
+def test_accumulation_across_calls_and_reset():
+    """sample(4)+sample(4) equals sample(8); a setter resets the count
+    (crates/trace/src/gpu.rs:2817-2940)."""
+    scene = scenes.box_spheres(width=48, height=48)
+    detail = Detail(bounces=Bounces.deep(4), accumulate=ACCUM_PLAIN)
+    with Trace.new_detailed(scene, detail, 48, 48) as t:
+        t.sample(4, 11)
+        t.sample(4, 11)
+        split = t.readback().color
+        assert t.samples() == 8
+        t.set_camera(scene.camera)
+        assert t.samples() == 0
+        t.sample(8, 11)
+        whole = t.readback().color
+    np.testing.assert_array_equal(split, whole)
+
+
+def test_window_equals_full_frame():
+    """A window renders the same bytes as that region of the full frame
+    (docs/trace.md:539-541)."""
+    W, H = 64, 48
+    scene = scenes.box_spheres(width=W, height=H)
+    detail = Detail(bounces=Bounces.deep(4))
+    with Trace.new_detailed(scene, detail, W, H) as t:
+        t.sample(4, 5)
+        full = t.readback().color
+    with Trace.new_detailed(scene, detail, 32, 16) as t:
+        t.set_window((W, H), (16, 20))
+        t.sample(4, 5)
+        part = t.readback().color
+    np.testing.assert_array_equal(part, full[20:36, 16:48])
+
+
+def test_runs_are_bit_identical():
+    scene = small_terrain()
+    detail = Detail(bounces=Bounces.deep(6))
+    imgs = []
+    for _ in range(2):
+        with Trace.new_detailed(scene, detail, 96, 54) as t:
+            t.sample(4, 3)
+            imgs.append(t.readback().color)
+    np.testing.assert_array_equal(imgs[0], imgs[1])
+
+
+def test_adaptive_rounds_match_oracle():
+    from pfx_amd import Adaptive
+
+    scene = scenes.box_spheres(width=48, height=48)
+    detail = Detail(bounces=Bounces.deep(4), accumulate=ACCUM_PLAIN)
+    ad = Adaptive(threshold=0.05, min_samples=8, max_samples=32, growth=1.5)
+    o = Oracle(scene, detail, 48, 48)
+    ref_rounds, pixels = o.sample_adaptive(ad, 9)
+    rc, _, _ = o.readback()
+    with Trace.new_detailed(scene, detail, 48, 48) as t:
+        conv = t.sample_adaptive(ad, 9)
+        got = t.readback().color
+    assert [(r.samples, r.active) for r in conv.rounds] == ref_rounds
+    assert conv.pixels == pixels
+    assert rel_mse(got[..., :3], rc[..., :3]) <= 1e-4
