@@ -195,8 +195,10 @@ typedef struct pfxh_stats {
     uint64_t closest_rays;  /* extension rays traced */
     uint64_t shadow_rays;   /* any-hit rays traced */
     uint64_t paths;         /* camera paths started */
-    uint64_t nodes_visited; /* filled when counting is on (pfxh_trace_set_counting) */
-    uint64_t prims_tested;
+    uint64_t nodes_visited; /* closest-hit 64-byte nodes fetched, when counting is on */
+    uint64_t prims_tested;  /* closest-hit triangles tested, when counting is on */
+    uint64_t shadow_nodes;  /* the same two counts for any-hit rays */
+    uint64_t shadow_prims;
     double trace_ms;        /* GPU time of closest-hit kernels (HIP events) */
     double shadow_ms;       /* GPU time of any-hit kernels */
     double shade_ms;        /* GPU time of raygen, shading and accumulation kernels */

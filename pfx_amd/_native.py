@@ -154,6 +154,8 @@ class StatsC(C.Structure):
         ("paths", C.c_uint64),
         ("nodes_visited", C.c_uint64),
         ("prims_tested", C.c_uint64),
+        ("shadow_nodes", C.c_uint64),
+        ("shadow_prims", C.c_uint64),
         ("trace_ms", C.c_double),
         ("shadow_ms", C.c_double),
         ("shade_ms", C.c_double),

@@ -71,9 +71,11 @@ enum Counter : uint32_t {
     C_SHADOW_RAYS = 4,
     C_NODES = 6,
     C_PRIMS = 8,
-    C_RUNNING = 10,
-    C_ACTIVE = 11,
-    C_COUNT = 12
+    C_SNODES = 10,
+    C_SPRIMS = 12,
+    C_RUNNING = 14,
+    C_ACTIVE = 15,
+    C_COUNT = 16
 };
 
 constexpr int TRACE_BLOCK = 128;

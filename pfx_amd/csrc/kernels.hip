@@ -1074,8 +1074,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev S, WaveDev W, u
     }
     if (rays) atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_SHADOW_RAYS), (unsigned long long)rays);
     if (COUNT) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_NODES), (unsigned long long)nodes);
-        atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_PRIMS), (unsigned long long)prims);
+        atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_SNODES), (unsigned long long)nodes);
+        atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_SPRIMS), (unsigned long long)prims);
     }
 }
 

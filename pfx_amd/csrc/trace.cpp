@@ -518,6 +518,8 @@ pfxh_status pfxh_trace_stats(const pfxh_trace* t, pfxh_stats* out) {
     out->shadow_rays = u64(C_SHADOW_RAYS);
     out->nodes_visited = u64(C_NODES);
     out->prims_tested = u64(C_PRIMS);
+    out->shadow_nodes = u64(C_SNODES);
+    out->shadow_prims = u64(C_SPRIMS);
     return PFXH_OK;
 }
 
