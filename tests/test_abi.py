@@ -1,0 +1,57 @@
+"""The C ABI: libpfxhip.so loads without a GPU, exports every function
+include/pfxhip.h declares, and its struct layouts match the Python view.
+No device compute runs here."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+from pfx_amd import _native as N
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "pfxhip.h"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(pfxh_[a-z_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported(native_lib):
+    names = declared_functions()
+    assert "pfxh_trace_new" in names and "pfxh_layout_bvh" in names
+    for name in names:
+        assert hasattr(native_lib, name), name
+    assert set(names) == set(N.SIGNATURES), set(names) ^ set(N.SIGNATURES)
+
+
+def test_version_string(native_lib):
+    assert native_lib.pfxh_version().startswith(b"pfxhip")
+
+
+def test_struct_sizes_match_the_header():
+    # sizes of the C structs as laid out by the x86-64 SysV ABI
+    assert C.sizeof(N.Camera) == 48
+    assert C.sizeof(N.Sun) == 28
+    assert C.sizeof(N.Environment) == 16
+    assert C.sizeof(N.DetailC) == 44
+    assert C.sizeof(N.AdaptiveC) == 16
+    assert C.sizeof(N.ConvergedC) == 4 + 4 * 64 + 4 + 8 * 64 + 8
+    assert C.sizeof(N.SceneC) == 7 * 16 + 16 + 48 + 28 + 4
+    assert N.PROBE_HIT.itemsize == 32 and N.BVH_NODE.itemsize == 48
+
+
+def test_invalid_inputs_are_refused_without_a_device(native_lib):
+    err = C.create_string_buffer(256)
+    tris = np.zeros(1, N.TRIANGLE)
+    tris["vertices"] = [0, 0, 0, 1, 0, 0, np.nan, 1, 0]
+    count = C.c_uint32()
+    nodes = np.zeros(2, N.BVH_NODE)
+    order = np.zeros(1, np.uint32)
+    st = native_lib.pfxh_layout_bvh(tris.ctypes.data, 1, nodes.ctypes.data, C.byref(count), order.ctypes.data,
+                                    err, 256)
+    assert st == N.SETUP and b"invalid" in err.value
+    out = C.c_void_p()
+    assert native_lib.pfxh_trace_new(None, None, 4, 4, C.byref(out), err, 256) == N.INVALID
+    assert native_lib.pfxh_trace_samples(None) == 0
+    native_lib.pfxh_trace_free(None)
