@@ -17,15 +17,21 @@ constexpr uint32_t REF_FIRST_MASK = 0x07ffffffu;
 constexpr uint32_t REF_INDEX_MASK = 0x3fffffffu;
 constexpr uint32_t REF_NONE = 0xffffffffu;
 
-// One BVH2 interior node holding the boxes of both children: 64 bytes, one
-// aligned HBM burst per visit. Both child slabs are tested from one fetch.
-struct alignas(16) WideNode {
-    float box0[6];  // lo.xyz, hi.xyz of child 0 (the reference's left child)
-    float box1[6];  // child 1 (the reference's `first`, right child)
-    uint32_t ref0, ref1;
-    uint32_t pad0, pad1;
+// Traversal stack entries per ray (LDS head + global spill); scene builds
+// whose worst-case stack would not fit are refused.
+constexpr int STACK_LIMIT = 96;
+
+// A 4-wide node: the reference's binary BVH collapsed two levels at a time,
+// children kept in the reference's left-first depth-first order. 128 bytes
+// (one L2 line): the six child-box planes as 4-wide vectors, then the four
+// child references. Empty slots have REF_NONE and a +inf box.
+struct alignas(16) Node4 {
+    float lo_x[4], lo_y[4], lo_z[4];
+    float hi_x[4], hi_y[4], hi_z[4];
+    uint32_t ref[4];
+    uint32_t pad[4];
 };
-static_assert(sizeof(WideNode) == 64, "WideNode is 64 bytes");
+static_assert(sizeof(Node4) == 128, "Node4 is 128 bytes");
 
 // Positions in edge form: a, e1 = b - a, e2 = c - a (the same subtractions
 // the Moller-Trumbore test makes, done once on the host).

@@ -9,7 +9,7 @@ namespace pfxh {
 
 // Scene constants and read-only device buffers (passed by value).
 struct SceneDev {
-    const WideNode* nodes;
+    const Node4* nodes;
     const TriPos* tris;
     const TriAttr* attrs;
     const Group* groups;
@@ -75,12 +75,14 @@ enum Counter : uint32_t {
     C_SPRIMS = 12,
     C_RUNNING = 14,
     C_ACTIVE = 15,
-    C_COUNT = 16
+    C_FETCH = 16,   // work counter of k_extend
+    C_SFETCH = 17,  // work counter of k_shadow
+    C_COUNT = 18
 };
 
 constexpr int TRACE_BLOCK = 128;
 constexpr int STACK_SHORT = 12;
-constexpr int STACK_DEPTH = 64;
+constexpr int STACK_DEPTH = STACK_LIMIT;
 
 void launch_camera(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* active, uint32_t npix,
                    uint32_t nsamples, uint32_t acc, uint32_t seed);

@@ -64,37 +64,65 @@ DEV uint32_t wave_append(uint32_t* counter, bool pred, uint32_t amount_if = 1) {
 }
 
 // ---- traversal ------------------------------------------------------------------
-// The reference's box_hit (trace.wgsl:453-468): slab test with divisions;
-// `nr` is the entry distance, which decides a deferred pop exactly as the
-// reference's pop-time test does (it passes iff nr <= the current t).
-DEV bool slab(float lx, float ly, float lz, float hx, float hy, float hz, v3 o, v3 d, float limit, float& nr) {
+// A ray in the space being traversed, with per-axis reciprocals.
+struct Ray {
+    v3 o, d, y;
+    bool dx, dy, dz;  // |d| < 1e-12 on that axis: the reference tests the slab bounds only
+};
+DEV Ray make_ray(v3 o, v3 d) {
+    Ray r;
+    r.o = o;
+    r.d = d;
+    r.dx = fabsf(d.x) < 1e-12f;
+    r.dy = fabsf(d.y) < 1e-12f;
+    r.dz = fabsf(d.z) < 1e-12f;
+    r.y = mk(r.dx ? 0.0f : 1.0f / d.x, r.dy ? 0.0f : 1.0f / d.y, r.dz ? 0.0f : 1.0f / d.z);
+    return r;
+}
+
+// RN(a / b) from y = RN(1 / b): one product and two FMA corrections give
+// the correctly rounded quotient (Markstein's theorem) whenever the
+// remainder cannot underflow; numerators below 2^-96 divide directly.
+DEV float qdiv(float a, float b, float y) {
+    if (fabsf(a) < 0x1p-96f) return a / b;
+    float q = a * y;
+    float r = __builtin_fmaf(-q, b, a);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-q, b, a);
+    return __builtin_fmaf(r, y, q);
+}
+
+// The reference's box_hit (trace.wgsl:453-468) with the same quotients. The
+// early exit after each axis is folded into one final test (the entry
+// distance only grows and the exit distance only shrinks, so both forms
+// agree). `nr` is the entry distance: a deferred child passes its pop-time
+// test exactly when nr <= the current t.
+DEV bool slab(float lx, float ly, float lz, float hx, float hy, float hz, const Ray& R, float limit, float& nr) {
     float near_ = 0.0f, far_ = limit;
-    if (fabsf(d.x) < 1e-12f) {
-        if (o.x < lx || o.x > hx) return false;
+    bool inside = true;
+    if (R.dx) {
+        inside = !(R.o.x < lx || R.o.x > hx);
     } else {
-        float a = (lx - o.x) / d.x, b = (hx - o.x) / d.x;
+        float a = qdiv(lx - R.o.x, R.d.x, R.y.x), b = qdiv(hx - R.o.x, R.d.x, R.y.x);
         near_ = fmaxf(near_, fminf(a, b));
         far_ = fminf(far_, fmaxf(a, b));
-        if (near_ > far_) return false;
     }
-    if (fabsf(d.y) < 1e-12f) {
-        if (o.y < ly || o.y > hy) return false;
+    if (R.dy) {
+        inside = inside && !(R.o.y < ly || R.o.y > hy);
     } else {
-        float a = (ly - o.y) / d.y, b = (hy - o.y) / d.y;
+        float a = qdiv(ly - R.o.y, R.d.y, R.y.y), b = qdiv(hy - R.o.y, R.d.y, R.y.y);
         near_ = fmaxf(near_, fminf(a, b));
         far_ = fminf(far_, fmaxf(a, b));
-        if (near_ > far_) return false;
     }
-    if (fabsf(d.z) < 1e-12f) {
-        if (o.z < lz || o.z > hz) return false;
+    if (R.dz) {
+        inside = inside && !(R.o.z < lz || R.o.z > hz);
     } else {
-        float a = (lz - o.z) / d.z, b = (hz - o.z) / d.z;
+        float a = qdiv(lz - R.o.z, R.d.z, R.y.z), b = qdiv(hz - R.o.z, R.d.z, R.y.z);
         near_ = fmaxf(near_, fminf(a, b));
         far_ = fminf(far_, fmaxf(a, b));
-        if (near_ > far_) return false;
     }
     nr = near_;
-    return true;
+    return inside && !(near_ > far_);
 }
 
 // trace.wgsl:449-452 to_object: rows . (p, w), summed x, y, z, w
@@ -127,18 +155,61 @@ struct Closest {
     uint32_t slot, rec;
 };
 
-// Closest hit (trace.wgsl:999-1075 `scene`, with tri_hit :780-804). Pops
-// follow the reference's left-first depth-first order, so equal-distance
-// ties resolve to the same triangle.
+// Test the four children of a Node4, then continue with the first hit
+// child and push the other hit children right to left, so children pop in
+// the reference's left-first order. Returns false when no child was hit.
 template <bool COUNT>
-DEV Closest trace_closest(const SceneDev& S, v3 o, v3 d, float tmax, Stack& st, uint32_t& nodes_ctr,
-                          uint32_t& prims_ctr) {
-    Closest h{tmax, 0.0f, 0.0f, 0xffffffffu, 0u};
+DEV bool visit4(const SceneDev& S, uint32_t index, const Ray& R, float t, Stack& st, uint32_t& cur,
+                uint32_t& nodes_ctr) {
+    const float4* np = reinterpret_cast<const float4*>(S.nodes + index);
+    float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
+    uint4 rf = reinterpret_cast<const uint4*>(np)[6];
+    if (COUNT) nodes_ctr++;
+    float n0, n1, n2, n3;
+    bool h0 = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, R, t, n0);
+    bool h1 = rf.y != REF_NONE && slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, R, t, n1);
+    bool h2 = rf.z != REF_NONE && slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, R, t, n2);
+    bool h3 = rf.w != REF_NONE && slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, R, t, n3);
+    if (h3 && (h0 || h1 || h2)) st.push(rf.w, n3);
+    if (h2 && (h0 || h1)) st.push(rf.z, n2);
+    if (h1 && h0) st.push(rf.y, n1);
+    if (h0) {
+        cur = rf.x;
+        return true;
+    }
+    if (h1) {
+        cur = rf.y;
+        return true;
+    }
+    if (h2) {
+        cur = rf.z;
+        return true;
+    }
+    if (h3) {
+        cur = rf.w;
+        return true;
+    }
+    return false;
+}
+
+// Closest hit (trace.wgsl:999-1075 `scene`, with tri_hit :780-804) or, with
+// ANY, any hit (`occluded`, :1096-1168, tri_blocks :1076-1095). The leaves are
+// tested in the reference's order, so equal-distance ties resolve to the
+// same triangle.
+template <bool ANY, bool COUNT>
+DEV bool traverse(const SceneDev& S, v3 o, v3 d, float limit, Stack& st, Closest& h, uint32_t& nodes_ctr,
+                  uint32_t& prims_ctr) {
+    h.t = limit;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    h.slot = 0xffffffffu;
+    h.rec = 0u;
+    const Ray world = make_ray(o, d);
     float nr;
     if (S.root.ref == REF_NONE ||
-        !slab(S.root.lo[0], S.root.lo[1], S.root.lo[2], S.root.hi[0], S.root.hi[1], S.root.hi[2], o, d, h.t, nr))
-        return h;
-    v3 ro = o, rd = d;
+        !slab(S.root.lo[0], S.root.lo[1], S.root.lo[2], S.root.hi[0], S.root.hi[1], S.root.hi[2], world, limit, nr))
+        return false;
+    Ray R = world;
     uint32_t base = 0, rec = 0, word = 0;
     float inv_det = 1.0f;
     int inst_top = -1;
@@ -146,50 +217,42 @@ DEV Closest trace_closest(const SceneDev& S, v3 o, v3 d, float tmax, Stack& st, 
     uint32_t cur = S.root.ref;
     for (;;) {
         if ((cur & (REF_LEAF | REF_INST)) == 0u) {
-            const float4* np = reinterpret_cast<const float4*>(S.nodes + cur);
-            float4 a = np[0], b = np[1], c = np[2], e = np[3];
-            if (COUNT) nodes_ctr++;
-            float n0, n1;
-            bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, ro, rd, h.t, n0);
-            bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, ro, rd, h.t, n1);
-            if (h0) {
-                if (h1) st.push(fbits(e.y), n1);
-                cur = fbits(e.x);
-                continue;
-            }
-            if (h1) {
-                cur = fbits(e.y);
-                continue;
-            }
+            if (visit4<COUNT>(S, cur, R, h.t, st, cur, nodes_ctr)) continue;
         } else if (cur & REF_LEAF) {
-            uint32_t first = base + (cur & REF_FIRST_MASK);
-            uint32_t count = ((cur >> 27) & 15u) + 1u;
-            for (uint32_t j = 0; j < count; ++j) {
-                uint32_t i = first + j;
-                if (COUNT) prims_ctr++;
-                if ((word & 1u) != 0u) continue;  // shadow-only surfaces are skipped by camera rays
-                const float4* tp = reinterpret_cast<const float4*>(S.tris + i);
-                float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-                v3 A = mk(t0.x, t0.y, t0.z), e1 = mk(t0.w, t1.x, t1.y), e2 = mk(t1.z, t1.w, t2.x);
-                v3 p = pfx::cross(rd, e2);
-                float det = dot(e1, p);
-                float seen = det * inv_det;
-                if (fabsf(seen) < 1e-9f) continue;
-                if ((word & ONE_SIDED) != 0u && seen < 0.0f) continue;
-                float inv = 1.0f / det;
-                v3 s = ro - A;
-                float u = dot(s, p) * inv;
-                if (u < 0.0f || u > 1.0f) continue;
-                v3 q = pfx::cross(s, e1);
-                float v = dot(rd, q) * inv;
-                if (v < 0.0f || u + v > 1.0f) continue;
-                float t = dot(e2, q) * inv;
-                if (t > 0.0001f && t < h.t) {
-                    h.t = t;
-                    h.u = u;
-                    h.v = v;
-                    h.slot = i;
-                    h.rec = rec;
+            bool skip = ANY ? (word & 2u) != 0u : (word & 1u) != 0u;  // casts-no-shadow / shadow-only
+            if (!skip) {
+                uint32_t first = base + (cur & REF_FIRST_MASK);
+                uint32_t count = ((cur >> 27) & 15u) + 1u;
+                for (uint32_t j = 0; j < count; ++j) {
+                    uint32_t i = first + j;
+                    if (COUNT) prims_ctr++;
+                    const float4* tp = reinterpret_cast<const float4*>(S.tris + i);
+                    float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                    v3 A = mk(t0.x, t0.y, t0.z), e1 = mk(t0.w, t1.x, t1.y), e2 = mk(t1.z, t1.w, t2.x);
+                    v3 p = pfx::cross(R.d, e2);
+                    float det = dot(e1, p);
+                    float seen = det * inv_det;
+                    if (fabsf(seen) < 1e-9f) continue;
+                    if (!ANY && (word & ONE_SIDED) != 0u && seen < 0.0f) continue;
+                    float inv = 1.0f / det;
+                    v3 s = R.o - A;
+                    float u = dot(s, p) * inv;
+                    if (u < 0.0f || u > 1.0f) continue;
+                    v3 q = pfx::cross(s, e1);
+                    float v = dot(R.d, q) * inv;
+                    if (v < 0.0f || u + v > 1.0f) continue;
+                    float t = dot(e2, q) * inv;
+                    if (ANY) {
+                        if (t <= 0.0001f || t >= limit) continue;
+                        return true;
+                    }
+                    if (t > 0.0001f && t < h.t) {
+                        h.t = t;
+                        h.u = u;
+                        h.v = v;
+                        h.slot = i;
+                        h.rec = rec;
+                    }
                 }
             }
         } else {
@@ -198,12 +261,9 @@ DEV Closest trace_closest(const SceneDev& S, v3 o, v3 d, float tmax, Stack& st, 
             rec = g.record;
             word = g.word;
             inv_det = g.inv_det;
-            if (g.has_xform) {
-                ro = to_object(g.inv, o, 1.0f);
-                rd = to_object(g.inv, d, 0.0f);
-            }
+            if (g.has_xform) R = make_ray(to_object(g.inv, o, 1.0f), to_object(g.inv, d, 0.0f));
             inst_top = st.top;
-            if (slab(g.lo[0], g.lo[1], g.lo[2], g.hi[0], g.hi[1], g.hi[2], ro, rd, h.t, nr)) {
+            if (slab(g.lo[0], g.lo[1], g.lo[2], g.hi[0], g.hi[1], g.hi[2], R, h.t, nr)) {
                 cur = g.root;
                 continue;
             }
@@ -211,8 +271,7 @@ DEV Closest trace_closest(const SceneDev& S, v3 o, v3 d, float tmax, Stack& st, 
         bool found = false;
         for (;;) {
             if (st.top == inst_top) {
-                ro = o;
-                rd = d;
+                R = world;
                 base = 0;
                 rec = 0;
                 word = 0;
@@ -229,99 +288,20 @@ DEV Closest trace_closest(const SceneDev& S, v3 o, v3 d, float tmax, Stack& st, 
         }
         if (!found) break;
     }
-    return h;
+    return false;
 }
 
-// Any hit (trace.wgsl:1096-1168 `occluded`, tri_blocks :1076-1095).
+template <bool COUNT>
+DEV Closest trace_closest(const SceneDev& S, v3 o, v3 d, float tmax, Stack& st, uint32_t& nodes_ctr,
+                          uint32_t& prims_ctr) {
+    Closest h;
+    traverse<false, COUNT>(S, o, d, tmax, st, h, nodes_ctr, prims_ctr);
+    return h;
+}
 template <bool COUNT>
 DEV bool trace_any(const SceneDev& S, v3 o, v3 d, float limit, Stack& st, uint32_t& nodes_ctr, uint32_t& prims_ctr) {
-    float nr;
-    if (S.root.ref == REF_NONE ||
-        !slab(S.root.lo[0], S.root.lo[1], S.root.lo[2], S.root.hi[0], S.root.hi[1], S.root.hi[2], o, d, limit, nr))
-        return false;
-    v3 ro = o, rd = d;
-    uint32_t base = 0, word = 0;
-    float inv_det = 1.0f;
-    int inst_top = -1;
-    st.top = 0;
-    uint32_t cur = S.root.ref;
-    for (;;) {
-        if ((cur & (REF_LEAF | REF_INST)) == 0u) {
-            const float4* np = reinterpret_cast<const float4*>(S.nodes + cur);
-            float4 a = np[0], b = np[1], c = np[2], e = np[3];
-            if (COUNT) nodes_ctr++;
-            float n0, n1;
-            bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, ro, rd, limit, n0);
-            bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, ro, rd, limit, n1);
-            if (h0) {
-                if (h1) st.push(fbits(e.y), n1);
-                cur = fbits(e.x);
-                continue;
-            }
-            if (h1) {
-                cur = fbits(e.y);
-                continue;
-            }
-        } else if (cur & REF_LEAF) {
-            if ((word & 2u) == 0u) {
-                uint32_t first = base + (cur & REF_FIRST_MASK);
-                uint32_t count = ((cur >> 27) & 15u) + 1u;
-                for (uint32_t j = 0; j < count; ++j) {
-                    uint32_t i = first + j;
-                    if (COUNT) prims_ctr++;
-                    const float4* tp = reinterpret_cast<const float4*>(S.tris + i);
-                    float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-                    v3 A = mk(t0.x, t0.y, t0.z), e1 = mk(t0.w, t1.x, t1.y), e2 = mk(t1.z, t1.w, t2.x);
-                    v3 p = pfx::cross(rd, e2);
-                    float det = dot(e1, p);
-                    if (fabsf(det * inv_det) < 1e-9f) continue;
-                    float inv = 1.0f / det;
-                    v3 s = ro - A;
-                    float u = dot(s, p) * inv;
-                    if (u < 0.0f || u > 1.0f) continue;
-                    v3 q = pfx::cross(s, e1);
-                    float v = dot(rd, q) * inv;
-                    if (v < 0.0f || u + v > 1.0f) continue;
-                    float t = dot(e2, q) * inv;
-                    if (t <= 0.0001f || t >= limit) continue;
-                    return true;
-                }
-            }
-        } else {
-            const Group& g = S.groups[cur & REF_INDEX_MASK];
-            base = g.base;
-            word = g.word;
-            inv_det = g.inv_det;
-            if (g.has_xform) {
-                ro = to_object(g.inv, o, 1.0f);
-                rd = to_object(g.inv, d, 0.0f);
-            }
-            inst_top = st.top;
-            if (slab(g.lo[0], g.lo[1], g.lo[2], g.hi[0], g.hi[1], g.hi[2], ro, rd, limit, nr)) {
-                cur = g.root;
-                continue;
-            }
-        }
-        bool found = false;
-        for (;;) {
-            if (st.top == inst_top) {
-                ro = o;
-                rd = d;
-                base = 0;
-                word = 0;
-                inv_det = 1.0f;
-                inst_top = -1;
-            }
-            if (st.top == 0) break;
-            uint2 e = st.pop();
-            (void)e.y;  // the limit never shrinks: every pushed entry is still live
-            cur = e.x;
-            found = true;
-            break;
-        }
-        if (!found) break;
-    }
-    return false;
+    Closest h;
+    return traverse<true, COUNT>(S, o, d, limit, st, h, nodes_ctr, prims_ctr);
 }
 
 // ---- surface at a hit (trace.wgsl:754-779 tri_fill, :681-694 world_hit) -----
@@ -846,10 +826,17 @@ template <bool COUNT>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev S, WaveDev W, const uint32_t* queue, uint32_t n) {
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
-    uint32_t stride = gridDim.x * TRACE_BLOCK;
     Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
     uint32_t nodes = 0, prims = 0;
-    for (uint32_t k = gtid; k < n; k += stride) {
+    // persistent waves take 64 rays at a time from a shared counter, so a
+    // wave that drew short rays comes back for more
+    for (;;) {
+        uint32_t base = 0;
+        if (__lane_id() == 0) base = atomicAdd(W.counters + C_FETCH, 64u);
+        base = (uint32_t)__shfl((int)base, 0);
+        if (base >= n) break;
+        uint32_t k = base + __lane_id();
+        if (k >= n) continue;
         uint32_t i = queue ? queue[k] : k;
         float4 o4 = W.o[i], d4 = W.d[i];
         Closest h = trace_closest<COUNT>(S, f3(o4), f3(d4), 1e20f, st, nodes, prims);
@@ -1052,10 +1039,15 @@ template <bool COUNT>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev S, WaveDev W, uint32_t n) {
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
-    uint32_t stride = gridDim.x * TRACE_BLOCK;
     Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
     uint32_t nodes = 0, prims = 0, rays = 0;
-    for (uint32_t k = gtid; k < n; k += stride) {
+    for (;;) {
+        uint32_t base = 0;
+        if (__lane_id() == 0) base = atomicAdd(W.counters + C_SFETCH, 64u);
+        base = (uint32_t)__shfl((int)base, 0);
+        if (base >= n) break;
+        uint32_t k = base + __lane_id();
+        if (k >= n) continue;
         uint32_t q = W.q_sh[k];
         uint32_t i = q & 0x0fffffffu, m = q >> 28;
         float4 r = W.res[i];

@@ -482,59 +482,98 @@ std::string lay(const pfxh_scene& sc, Laid& out) {
     return "";
 }
 
-// ---- conversion of a reference-format tree into WideNodes --------------------
-struct Converter {
-    std::vector<WideNode>& out;
-    uint32_t depth = 0;
-    explicit Converter(std::vector<WideNode>& o) : out(o) {}
+// ---- conversion of a reference-format binary tree into Node4s ---------------
+// Each Node4 stands for one binary interior node with both children expanded
+// one more level, children kept in left-first depth-first order. A deferred
+// slab test of each child at pop time then visits exactly the reference's
+// leaves in the reference's order: slab entry distances are monotone in the
+// box bounds, so a skipped intermediate box passes whenever a child inside
+// it passes.
+struct Collapse {
+    std::vector<Node4>& out;
+    const std::vector<pfxh_bvh_node>* nodes = nullptr;
+    uint32_t (*leaf_ref)(const pfxh_bvh_node&, const void*) = nullptr;
+    const void* ctx = nullptr;
+    std::string* err = nullptr;
+    uint32_t depth = 0;  // binary depth of the deepest leaf
+    explicit Collapse(std::vector<Node4>& o) : out(o) {}
 
-    static void box_of(const pfxh_bvh_node& n, float* b) {
-        for (int k = 0; k < 3; ++k) {
-            b[k] = n.lo[k];
-            b[3 + k] = n.hi[k];
-        }
-    }
-    // Convert the triangle tree `nodes` (indices local to it); leaves keep
-    // `first` relative to the tree's own triangle slots.
-    uint32_t tri_tree(const std::vector<pfxh_bvh_node>& nodes, std::string& err) {
-        if (nodes.empty()) return REF_NONE;
-        std::vector<uint32_t> wide(nodes.size(), REF_NONE);
-        uint32_t base = (uint32_t)out.size();
-        uint32_t count = 0;
-        for (size_t i = 0; i < nodes.size(); ++i)
-            if (nodes[i].count == 0) wide[i] = base + count++;
-        out.resize(base + count);
-        std::vector<uint32_t> level(nodes.size(), 0);
-        for (size_t i = 0; i < nodes.size(); ++i) {
-            const pfxh_bvh_node& n = nodes[i];
-            if (n.count != 0) continue;
-            uint32_t kids[2] = {(uint32_t)i + 1, n.first};
-            WideNode& w = out[wide[i]];
-            memset(&w, 0, sizeof w);
-            for (int c = 0; c < 2; ++c) {
-                const pfxh_bvh_node& k = nodes[kids[c]];
-                level[kids[c]] = level[i] + 1;
-                depth = std::max(depth, level[kids[c]]);
-                box_of(k, c == 0 ? w.box0 : w.box1);
-                uint32_t ref;
-                if (k.count == 0) {
-                    ref = wide[kids[c]];
-                } else {
-                    if (k.count > 16 || k.first > REF_FIRST_MASK) err = "bvh leaf does not fit its reference";
-                    ref = REF_LEAF | ((k.count - 1u) << 27) | k.first;
-                }
-                if (c == 0)
-                    w.ref0 = ref;
-                else
-                    w.ref1 = ref;
+    bool inner(size_t n) const { return (*nodes)[n].count == 0; }
+    // returns the Node4 index; *stack gets the worst-case entries this
+    // subtree can hold on the traversal stack
+    uint32_t build(size_t n, uint32_t level, uint32_t* stack) {
+        uint32_t index = (uint32_t)out.size();
+        out.emplace_back();
+        size_t kids[4];
+        uint32_t kid_level[4];
+        int count = 0;
+        const size_t two[2] = {n + 1, (*nodes)[n].first};
+        for (size_t c : two) {
+            if (inner(c)) {
+                kids[count] = c + 1;
+                kid_level[count++] = level + 2;
+                kids[count] = (*nodes)[c].first;
+                kid_level[count++] = level + 2;
+            } else {
+                kids[count] = c;
+                kid_level[count++] = level + 1;
             }
         }
-        const pfxh_bvh_node& r = nodes[0];
-        if (r.count == 0) return wide[0];
-        if (r.count > 16 || r.first > REF_FIRST_MASK) err = "bvh leaf does not fit its reference";
-        return REF_LEAF | ((r.count - 1u) << 27) | r.first;
+        uint32_t refs[4] = {REF_NONE, REF_NONE, REF_NONE, REF_NONE};
+        uint32_t deepest = 0;
+        for (int k = 0; k < count; ++k) {
+            if (inner(kids[k])) {
+                uint32_t sub = 0;
+                refs[k] = build(kids[k], kid_level[k], &sub);
+                deepest = std::max(deepest, sub);
+            } else {
+                refs[k] = leaf_ref((*nodes)[kids[k]], ctx);
+                if (refs[k] == REF_NONE && err) *err = "bvh leaf does not fit its reference";
+                depth = std::max(depth, kid_level[k]);
+            }
+        }
+        // a node pushes every hit child but the first; a descendant's entries
+        // stack on top of its later siblings'
+        *stack = (uint32_t)(count - 1) + deepest;
+        Node4& w = out[index];
+        const float inf = std::numeric_limits<float>::infinity();
+        for (int k = 0; k < 4; ++k) {
+            bool live = k < count;
+            const pfxh_bvh_node* c = live ? &(*nodes)[kids[k]] : nullptr;
+            w.lo_x[k] = live ? c->lo[0] : inf;
+            w.lo_y[k] = live ? c->lo[1] : inf;
+            w.lo_z[k] = live ? c->lo[2] : inf;
+            w.hi_x[k] = live ? c->hi[0] : inf;
+            w.hi_y[k] = live ? c->hi[1] : inf;
+            w.hi_z[k] = live ? c->hi[2] : inf;
+            w.ref[k] = refs[k];
+            w.pad[k] = 0;
+        }
+        return index;
+    }
+    uint32_t tree(const std::vector<pfxh_bvh_node>& ns, uint32_t (*leaf)(const pfxh_bvh_node&, const void*),
+                  const void* c, std::string& e, uint32_t* stack) {
+        *stack = 0;
+        if (ns.empty()) return REF_NONE;
+        nodes = &ns;
+        leaf_ref = leaf;
+        ctx = c;
+        err = &e;
+        if (!inner(0)) {
+            depth = std::max(depth, 0u);
+            return leaf(ns[0], c);
+        }
+        return build(0, 0, stack);
     }
 };
+
+uint32_t triangle_leaf(const pfxh_bvh_node& n, const void*) {
+    if (n.count == 0 || n.count > 16 || n.first > REF_FIRST_MASK) return REF_NONE;
+    return REF_LEAF | ((n.count - 1u) << 27) | n.first;
+}
+uint32_t group_leaf(const pfxh_bvh_node& n, const void* order) {
+    return REF_INST | static_cast<const uint32_t*>(order)[n.first];
+}
 
 float to_radians(float deg) { return deg * (PI_F / 180.0f); }
 
@@ -1078,11 +1117,17 @@ std::string build_scene(const pfxh_scene& sc, HostScene& out) {
     }
     // device trees: world BLAS, mesh BLASes, then the top level
     out.nodes.clear();
-    Converter conv(out.nodes);
+    Collapse conv(out.nodes);
     std::string err;
-    uint32_t world_root = conv.tri_tree(laid.world_bvh.nodes, err);
+    uint32_t bottom_stack = 0, sub = 0;
+    uint32_t world_root = conv.tree(laid.world_bvh.nodes, triangle_leaf, nullptr, err, &sub);
+    bottom_stack = std::max(bottom_stack, sub);
     std::vector<uint32_t> mesh_roots;
-    for (auto& m : laid.meshes) mesh_roots.push_back(conv.tri_tree(m.bvh.nodes, err));
+    for (auto& m : laid.meshes) {
+        mesh_roots.push_back(conv.tree(m.bvh.nodes, triangle_leaf, nullptr, err, &sub));
+        bottom_stack = std::max(bottom_stack, sub);
+    }
+    uint32_t bottom_depth = conv.depth;
     if (!err.empty()) return err;
     out.groups.clear();
     Surface world_surface;
@@ -1156,45 +1201,27 @@ std::string build_scene(const pfxh_scene& sc, HostScene& out) {
         }
         if (!out.groups.empty()) {
             RefBvh top = build_over(boxes, 1, 0);
-            // top leaves point at groups: rewrite leaf `first` through the order
-            std::vector<pfxh_bvh_node> tn = top.nodes;
-            uint32_t base = (uint32_t)out.nodes.size();
-            std::vector<uint32_t> wide(tn.size(), REF_NONE);
-            uint32_t count = 0;
-            for (size_t i = 0; i < tn.size(); ++i)
-                if (tn[i].count == 0) wide[i] = base + count++;
-            out.nodes.resize(base + count);
-            auto ref_of = [&](size_t i) {
-                return tn[i].count == 0 ? wide[i] : (REF_INST | top.order[tn[i].first]);
-            };
-            std::vector<uint32_t> level(tn.size(), 0);
-            for (size_t i = 0; i < tn.size(); ++i) {
-                if (tn[i].count != 0) continue;
-                level[i + 1] = level[tn[i].first] = level[i] + 1;
-                top_depth = std::max(top_depth, level[i] + 1);
-                WideNode& w = out.nodes[wide[i]];
-                memset(&w, 0, sizeof w);
-                size_t kids[2] = {i + 1, tn[i].first};
-                for (int c = 0; c < 2; ++c) {
-                    float* b = c == 0 ? w.box0 : w.box1;
-                    for (int k = 0; k < 3; ++k) {
-                        b[k] = tn[kids[c]].lo[k];
-                        b[3 + k] = tn[kids[c]].hi[k];
-                    }
-                }
-                w.ref0 = ref_of(kids[0]);
-                w.ref1 = ref_of(kids[1]);
-            }
+            // top leaves point at groups through the instance order
+            Collapse top_conv(out.nodes);
+            uint32_t top_stack = 0;
+            uint32_t root = top_conv.tree(top.nodes, group_leaf, top.order.data(), err, &top_stack);
+            if (!err.empty()) return err;
+            top_depth = top_conv.depth;
+            out.max_stack = top_stack + bottom_stack;
             for (int k = 0; k < 3; ++k) {
-                out.root.lo[k] = tn[0].lo[k];
-                out.root.hi[k] = tn[0].hi[k];
+                out.root.lo[k] = top.nodes[0].lo[k];
+                out.root.hi[k] = top.nodes[0].hi[k];
             }
-            out.root.ref = ref_of(0);
+            out.root.ref = root;
             out.root.top = 1;
         }
     }
-    out.max_depth = conv.depth + top_depth;
-    if (out.max_depth >= 60) return "the bvh is too deep for the traversal stack";
+    if (sc.instance_count == 0) out.max_stack = bottom_stack;
+    out.max_depth = bottom_depth + top_depth;
+    // the reference's 64-entry stack never overflows on these trees (it
+    // would drop subtrees), and ours must hold the worst case
+    if (out.max_depth >= 60) return "the bvh is too deep for the reference's traversal stack";
+    if (out.max_stack >= (uint32_t)STACK_LIMIT) return "the bvh is too deep for the traversal stack";
     out.surfaces = surfaces;
     if (out.nodes.empty()) out.nodes.resize(1);
     if (out.tris.empty()) {

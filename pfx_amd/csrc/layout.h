@@ -30,7 +30,7 @@ struct HostScene {
     // reference-format two-level layout (kept for the layout entry points)
     RefLevels ref;
     // device formats
-    std::vector<WideNode> nodes;
+    std::vector<Node4> nodes;
     std::vector<TriPos> tris;
     std::vector<TriAttr> attrs;
     std::vector<uint32_t> slot_prim;   // caller primitive index per triangle slot
@@ -49,7 +49,8 @@ struct HostScene {
     bool lit = false;
     RootEntry root;
     uint32_t slots = 0;
-    uint32_t max_depth = 0;
+    uint32_t max_depth = 0;   // binary-tree depth (top + bottom)
+    uint32_t max_stack = 0;   // worst-case traversal stack entries
 };
 
 std::string build_scene(const pfxh_scene& scene, HostScene& out);

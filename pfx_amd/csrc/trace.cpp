@@ -173,7 +173,10 @@ void setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& detail, ui
     t->height = height;
     size_t pixels = (size_t)width * height;
     // a wave holds every active pixel at least once
-    size_t want = detail.wave_paths ? detail.wave_paths : std::max<size_t>(pixels, (size_t)1 << 22);
+    // several samples of every pixel per wave keep late bounces wide enough
+    // to fill the GPU; HBM is plentiful (about 180 bytes per path)
+    size_t want = detail.wave_paths ? detail.wave_paths
+                                    : std::max(pixels, std::min<size_t>((size_t)1 << 25, pixels * 64));
     want = std::max(want, pixels);
     if (want >= ((size_t)1 << 28)) throw DeviceError{PFXH_REFUSED, "a wave of more than 2^28 paths does not fit the queues"};
     t->capacity = (uint32_t)want;
@@ -257,6 +260,7 @@ void sample(pfxh_trace* t, uint32_t count, uint32_t seed) {
         for (uint32_t bounce = 0; bounce <= t->S.total + 1u && n > 0; ++bounce) {
             WaveDev W = t->W;
             W.q_next = outq;
+            launch_zero(s, W.counters + C_FETCH, 2);
             HIP_TRY(hipEventRecord(t->ev[0], s));
             launch_extend(s, t->S, W, queue, n, t->counting, grid);
             HIP_TRY(hipEventRecord(t->ev[1], s));
@@ -274,6 +278,7 @@ void sample(pfxh_trace* t, uint32_t count, uint32_t seed) {
             shadow_pending = false;
             uint32_t n_next = t->host_counters[C_NEXT], n_sh = t->host_counters[C_SHADOW];
             if (n_sh) {
+                launch_zero(s, W.counters + C_SFETCH, 1);
                 HIP_TRY(hipEventRecord(t->ev[3], s));
                 launch_shadow(s, t->S, W, n_sh, t->counting, grid);
                 HIP_TRY(hipEventRecord(t->ev[4], s));
