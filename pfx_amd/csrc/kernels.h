@@ -27,6 +27,7 @@ struct SceneDev {
     uint32_t env_w, env_h, env_black, lit;
     uint32_t looped, tree, emitter_count, light_slots;
     uint32_t entries;  // NEE entries a vertex may queue
+    uint32_t bounds_tame;  // every box bound is 0 or has magnitude >= 2^-72
     uint32_t total, diffuse, glossy, transmission;
     float sun_dir[3], sun_color[3], sun_intensity, sun_w;
     float origin[3], forward[3], right[3], up[3], lens[4];
@@ -81,7 +82,7 @@ enum Counter : uint32_t {
 };
 
 constexpr int TRACE_BLOCK = 128;
-constexpr int STACK_SHORT = 12;
+constexpr int STACK_SHORT = 8;
 constexpr int STACK_DEPTH = STACK_LIMIT;
 
 void launch_camera(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* active, uint32_t npix,

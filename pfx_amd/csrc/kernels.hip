@@ -67,29 +67,37 @@ DEV uint32_t wave_append(uint32_t* counter, bool pred, uint32_t amount_if = 1) {
 // A ray in the space being traversed, with per-axis reciprocals.
 struct Ray {
     v3 o, d, y;
-    bool dx, dy, dz;  // |d| < 1e-12 on that axis: the reference tests the slab bounds only
+    bool fast;  // no near-zero direction component, origin components 0 or >= 2^-72
 };
-DEV Ray make_ray(v3 o, v3 d) {
+DEV bool tame(float v) { return v == 0.0f || (fabsf(v) >= 0x1p-72f && fabsf(v) <= 0x1p60f); }
+DEV Ray make_ray(v3 o, v3 d, bool bounds_tame) {
     Ray r;
     r.o = o;
     r.d = d;
-    r.dx = fabsf(d.x) < 1e-12f;
-    r.dy = fabsf(d.y) < 1e-12f;
-    r.dz = fabsf(d.z) < 1e-12f;
-    r.y = mk(r.dx ? 0.0f : 1.0f / d.x, r.dy ? 0.0f : 1.0f / d.y, r.dz ? 0.0f : 1.0f / d.z);
+    bool dx = fabsf(d.x) < 1e-12f, dy = fabsf(d.y) < 1e-12f, dz = fabsf(d.z) < 1e-12f;
+    r.y = mk(dx ? 0.0f : 1.0f / d.x, dy ? 0.0f : 1.0f / d.y, dz ? 0.0f : 1.0f / d.z);
+    // the quotient check covered |d| in [2^-39, 2^20]
+    bool big = fabsf(d.x) > 0x1p20f || fabsf(d.y) > 0x1p20f || fabsf(d.z) > 0x1p20f;
+    r.fast = bounds_tame && !(dx || dy || dz || big) && tame(o.x) && tame(o.y) && tame(o.z);
     return r;
 }
 
 // RN(a / b) from y = RN(1 / b): one product and two FMA corrections give
 // the correctly rounded quotient (Markstein's theorem) whenever the
-// remainder cannot underflow; numerators below 2^-96 divide directly.
-DEV float qdiv(float a, float b, float y) {
-    if (fabsf(a) < 0x1p-96f) return a / b;
+// remainder cannot underflow, i.e. for |a| >= 2^-96 (checked against IEEE
+// division on 8e8 random pairs) or a == 0. When box bounds and origin
+// components are all 0 or at least 2^-72, every nonzero lo - o is at least
+// 2^-95, so the fast form needs no guard.
+DEV float qdiv_fast(float a, float b, float y) {
     float q = a * y;
     float r = __builtin_fmaf(-q, b, a);
     q = __builtin_fmaf(r, y, q);
     r = __builtin_fmaf(-q, b, a);
     return __builtin_fmaf(r, y, q);
+}
+DEV float qdiv(float a, float b, float y) {
+    if (fabsf(a) < 0x1p-96f) return a / b;
+    return qdiv_fast(a, b, y);
 }
 
 // The reference's box_hit (trace.wgsl:453-468) with the same quotients. The
@@ -100,21 +108,21 @@ DEV float qdiv(float a, float b, float y) {
 DEV bool slab(float lx, float ly, float lz, float hx, float hy, float hz, const Ray& R, float limit, float& nr) {
     float near_ = 0.0f, far_ = limit;
     bool inside = true;
-    if (R.dx) {
+    if (fabsf(R.d.x) < 1e-12f) {
         inside = !(R.o.x < lx || R.o.x > hx);
     } else {
         float a = qdiv(lx - R.o.x, R.d.x, R.y.x), b = qdiv(hx - R.o.x, R.d.x, R.y.x);
         near_ = fmaxf(near_, fminf(a, b));
         far_ = fminf(far_, fmaxf(a, b));
     }
-    if (R.dy) {
+    if (fabsf(R.d.y) < 1e-12f) {
         inside = inside && !(R.o.y < ly || R.o.y > hy);
     } else {
         float a = qdiv(ly - R.o.y, R.d.y, R.y.y), b = qdiv(hy - R.o.y, R.d.y, R.y.y);
         near_ = fmaxf(near_, fminf(a, b));
         far_ = fminf(far_, fmaxf(a, b));
     }
-    if (R.dz) {
+    if (fabsf(R.d.z) < 1e-12f) {
         inside = inside && !(R.o.z < lz || R.o.z > hz);
     } else {
         float a = qdiv(lz - R.o.z, R.d.z, R.y.z), b = qdiv(hz - R.o.z, R.d.z, R.y.z);
@@ -123,6 +131,16 @@ DEV bool slab(float lx, float ly, float lz, float hx, float hy, float hz, const 
     }
     nr = near_;
     return inside && !(near_ > far_);
+}
+// The same test for a fast ray: straight-line code, no branches.
+DEV bool slab_fast(float lx, float ly, float lz, float hx, float hy, float hz, const Ray& R, float limit, float& nr) {
+    float ax = qdiv_fast(lx - R.o.x, R.d.x, R.y.x), bx = qdiv_fast(hx - R.o.x, R.d.x, R.y.x);
+    float ay = qdiv_fast(ly - R.o.y, R.d.y, R.y.y), by = qdiv_fast(hy - R.o.y, R.d.y, R.y.y);
+    float az = qdiv_fast(lz - R.o.z, R.d.z, R.y.z), bz = qdiv_fast(hz - R.o.z, R.d.z, R.y.z);
+    float near_ = fmaxf(fmaxf(fmaxf(0.0f, fminf(ax, bx)), fminf(ay, by)), fminf(az, bz));
+    float far_ = fminf(fminf(fminf(limit, fmaxf(ax, bx)), fmaxf(ay, by)), fmaxf(az, bz));
+    nr = near_;
+    return !(near_ > far_);
 }
 
 // trace.wgsl:449-452 to_object: rows . (p, w), summed x, y, z, w
@@ -166,30 +184,23 @@ DEV bool visit4(const SceneDev& S, uint32_t index, const Ray& R, float t, Stack&
     uint4 rf = reinterpret_cast<const uint4*>(np)[6];
     if (COUNT) nodes_ctr++;
     float n0, n1, n2, n3;
-    bool h0 = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, R, t, n0);
-    bool h1 = rf.y != REF_NONE && slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, R, t, n1);
-    bool h2 = rf.z != REF_NONE && slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, R, t, n2);
-    bool h3 = rf.w != REF_NONE && slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, R, t, n3);
+    bool h0, h1, h2, h3;
+    if (R.fast) {
+        h0 = slab_fast(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, R, t, n0);
+        h1 = slab_fast(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, R, t, n1) && rf.y != REF_NONE;
+        h2 = slab_fast(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, R, t, n2) && rf.z != REF_NONE;
+        h3 = slab_fast(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, R, t, n3) && rf.w != REF_NONE;
+    } else {
+        h0 = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, R, t, n0);
+        h1 = rf.y != REF_NONE && slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, R, t, n1);
+        h2 = rf.z != REF_NONE && slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, R, t, n2);
+        h3 = rf.w != REF_NONE && slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, R, t, n3);
+    }
     if (h3 && (h0 || h1 || h2)) st.push(rf.w, n3);
     if (h2 && (h0 || h1)) st.push(rf.z, n2);
     if (h1 && h0) st.push(rf.y, n1);
-    if (h0) {
-        cur = rf.x;
-        return true;
-    }
-    if (h1) {
-        cur = rf.y;
-        return true;
-    }
-    if (h2) {
-        cur = rf.z;
-        return true;
-    }
-    if (h3) {
-        cur = rf.w;
-        return true;
-    }
-    return false;
+    cur = h0 ? rf.x : (h1 ? rf.y : (h2 ? rf.z : rf.w));
+    return h0 || h1 || h2 || h3;
 }
 
 // Closest hit (trace.wgsl:999-1075 `scene`, with tri_hit :780-804) or, with
@@ -204,7 +215,8 @@ DEV bool traverse(const SceneDev& S, v3 o, v3 d, float limit, Stack& st, Closest
     h.v = 0.0f;
     h.slot = 0xffffffffu;
     h.rec = 0u;
-    const Ray world = make_ray(o, d);
+    const bool tame_bounds = S.bounds_tame != 0u;
+    const Ray world = make_ray(o, d, tame_bounds);
     float nr;
     if (S.root.ref == REF_NONE ||
         !slab(S.root.lo[0], S.root.lo[1], S.root.lo[2], S.root.hi[0], S.root.hi[1], S.root.hi[2], world, limit, nr))
@@ -261,7 +273,7 @@ DEV bool traverse(const SceneDev& S, v3 o, v3 d, float limit, Stack& st, Closest
             rec = g.record;
             word = g.word;
             inv_det = g.inv_det;
-            if (g.has_xform) R = make_ray(to_object(g.inv, o, 1.0f), to_object(g.inv, d, 0.0f));
+            if (g.has_xform) R = make_ray(to_object(g.inv, o, 1.0f), to_object(g.inv, d, 0.0f), tame_bounds);
             inst_top = st.top;
             if (slab(g.lo[0], g.lo[1], g.lo[2], g.hi[0], g.hi[1], g.hi[2], R, h.t, nr)) {
                 cur = g.root;

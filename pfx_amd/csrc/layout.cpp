@@ -1217,6 +1217,15 @@ std::string build_scene(const pfxh_scene& sc, HostScene& out) {
         }
     }
     if (sc.instance_count == 0) out.max_stack = bottom_stack;
+    out.bounds_tame = true;
+    auto tame = [](float v) { return v == 0.0f || std::isinf(v) || (std::fabs(v) >= 0x1p-72f && std::fabs(v) <= 0x1p60f); };
+    for (const Node4& n : out.nodes)
+        for (int k = 0; k < 4; ++k)
+            out.bounds_tame = out.bounds_tame && tame(n.lo_x[k]) && tame(n.lo_y[k]) && tame(n.lo_z[k]) &&
+                              tame(n.hi_x[k]) && tame(n.hi_y[k]) && tame(n.hi_z[k]);
+    for (const Group& g : out.groups)
+        for (int k = 0; k < 3; ++k) out.bounds_tame = out.bounds_tame && tame(g.lo[k]) && tame(g.hi[k]);
+    for (int k = 0; k < 3; ++k) out.bounds_tame = out.bounds_tame && tame(out.root.lo[k]) && tame(out.root.hi[k]);
     out.max_depth = bottom_depth + top_depth;
     // the reference's 64-entry stack never overflows on these trees (it
     // would drop subtrees), and ours must hold the worst case

@@ -51,6 +51,7 @@ struct HostScene {
     uint32_t slots = 0;
     uint32_t max_depth = 0;   // binary-tree depth (top + bottom)
     uint32_t max_stack = 0;   // worst-case traversal stack entries
+    bool bounds_tame = true;  // every node box bound is 0 or >= 2^-72 in magnitude
 };
 
 std::string build_scene(const pfxh_scene& scene, HostScene& out);

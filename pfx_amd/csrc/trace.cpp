@@ -131,6 +131,7 @@ void setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& detail, ui
     S.slot_prim = t->upload(h.slot_prim, "primitive map");
     S.record_instance = t->upload(h.record_instance, "instance map");
     S.root = h.root;
+    S.bounds_tame = h.bounds_tame ? 1u : 0u;
     S.env_w = h.env_w;
     S.env_h = h.env_h;
     S.env_black = h.env_black ? 1u : 0u;
