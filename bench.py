@@ -36,7 +36,14 @@ WIDTH, HEIGHT = 3840, 2160
 ADAPTIVE = Adaptive(threshold=0.01, min_samples=16, max_samples=64, growth=1.5)
 BOUNCES = Bounces.deep(6)
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, PRIM_BYTES, HIT_BYTES = 64, 48, 48 + 64  # wide node, edge-form triangle, attributes + material
+# algorithmic bytes of the closest-hit kernel: one 128-byte 4-wide node per
+# node visit, one 48-byte edge-form triangle per triangle test, and per ray
+# its queue entry (4), origin and direction (32), hit record (16) and hit
+# instance record (4)
+NODE_BYTES, PRIM_BYTES, RAY_IO_BYTES = 128, 48, 4 + 32 + 16 + 4
+# HBM bytes per closest-hit ray measured with rocprofv3 counters (FETCH_SIZE
+# and WRITE_SIZE of k_extend, calibrated on a copy kernel); absent -> null
+TRAFFIC_FILE = ROOT / "profiles" / "k_extend_traffic.json"
 
 
 def log(msg: str) -> None:
@@ -101,8 +108,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by frame tiles, "
-              "see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); across GPUs the still would be "
+              "split into frame tiles, one tracer per GPU", file=sys.stderr)
         return 2
     if int(os.environ.get("RANK", "0")) != 0:
         return 0
@@ -135,11 +142,15 @@ def main() -> int:
     rays = s["closest_rays"] + s["shadow_rays"]
     mrays = rays / total / 1e6
     # roofline of the closest-hit kernel
-    bytes_per_ray = per_ray["nodes_per_ray"] * NODE_BYTES + per_ray["prims_per_ray"] * PRIM_BYTES + HIT_BYTES
+    bytes_per_ray = per_ray["nodes_per_ray"] * NODE_BYTES + per_ray["prims_per_ray"] * PRIM_BYTES + RAY_IO_BYTES
     launches = max(s["launches"], 1)
     rays_per_launch = s["closest_rays"] / launches
     avg_launch_ms = s["trace_ms"] / launches
     achieved = bytes_per_ray * rays_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    traffic = None
+    if TRAFFIC_FILE.exists():
+        measured = json.loads(TRAFFIC_FILE.read_text())
+        traffic = round(measured["hbm_bytes_per_ray"] * rays_per_launch)
     result = {
         "metric": "Mrays/s; 4K ms to relMSE≤0.01: 4spp+learned denoise vs plain 64spp",
         "value": round(mrays, 2),
@@ -171,7 +182,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_ray": round(bytes_per_ray, 1),
             "rays_per_launch": round(rays_per_launch, 1),
             "avg_launch_ms": round(avg_launch_ms, 4),

@@ -50,11 +50,13 @@ struct WaveDev {
     float4* e_c;    // per entry: contribution.xyz
     uint32_t* q_in;
     uint32_t* q_next;
-    uint32_t* q_sh;
-    uint32_t* counters;  // [0] next, [1] shadow, [2..] stats
+    uint32_t* q_sh;      // paths with NEE entries: index | entries << 28
+    uint32_t* q_sray;    // shadow rays to test: path index | entry << 28
+    uint32_t* counters;  // [0] next, [1] shadow paths, [2] shadow rays, [3..] stats
     uint2* spill;        // traversal stack overflow
     uint32_t capacity;   // paths per wave (entry arrays are strided by it)
     uint32_t spill_threads;
+    uint32_t refill;     // idle lanes that make a traversal wave fetch new rays (1..64)
 };
 
 // Accumulation buffers (per pixel).
@@ -65,20 +67,24 @@ struct AccumDev {
     uint32_t mode;
 };
 
+// Device counters, each hot one on its own 1 KiB slot: atomics from all
+// XCDs on one address (or one line) serialize, so counters that are bumped
+// at the same time must not share a line.
+constexpr uint32_t SLOT = 256;   // uint32 per slot
+constexpr uint32_t PARTS = 8;    // work parts of a traversal queue, one per XCD
 enum Counter : uint32_t {
-    C_NEXT = 0,
-    C_SHADOW = 1,
-    C_CLOSEST_RAYS = 2,  // 64-bit pairs from here on
-    C_SHADOW_RAYS = 4,
-    C_NODES = 6,
-    C_PRIMS = 8,
-    C_SNODES = 10,
-    C_SPRIMS = 12,
-    C_RUNNING = 14,
-    C_ACTIVE = 15,
-    C_FETCH = 16,   // work counter of k_extend
-    C_SFETCH = 17,  // work counter of k_shadow
-    C_COUNT = 18
+    C_NEXT = 0,                        // next-bounce queue length
+    C_SHADOW = SLOT,                   // shadow path queue length
+    C_SRAYS = 2 * SLOT,                // shadow ray queue length
+    C_RUNNING = 3 * SLOT,
+    C_ACTIVE = 4 * SLOT,
+    C_NODES = 5 * SLOT,                // 64-bit statistics
+    C_PRIMS = 5 * SLOT + 2,
+    C_SNODES = 5 * SLOT + 4,
+    C_SPRIMS = 5 * SLOT + 6,
+    C_FETCH = 8 * SLOT,                // PARTS work counters of k_extend
+    C_SFETCH = C_FETCH + PARTS * SLOT, // PARTS work counters of k_shadow
+    C_COUNT = C_SFETCH + PARTS * SLOT
 };
 
 constexpr int TRACE_BLOCK = 128;
@@ -92,6 +98,7 @@ void launch_extend(hipStream_t s, const SceneDev& S, const WaveDev& W, const uin
 void launch_shade(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* queue, uint32_t n,
                   uint32_t bounce);
 void launch_shadow(hipStream_t s, const SceneDev& S, const WaveDev& W, uint32_t n, bool count, uint32_t grid);
+void launch_fold(hipStream_t s, const WaveDev& W, uint32_t n);
 void launch_finish(hipStream_t s, const WaveDev& W, const AccumDev& A, const uint32_t* active, uint32_t npix,
                    uint32_t nsamples, float count_after);
 void launch_develop(hipStream_t s, const AccumDev& A, uint32_t pixels, float4* color, float4* albedo,

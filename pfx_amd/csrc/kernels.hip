@@ -1,14 +1,18 @@
 // Wavefront path tracer for gfx950 (MI355X).
 //
 // One wave of paths lives in HBM as structure-of-arrays state. Each bounce
-// runs three kernels over compacted queues:
-//   extend : closest-hit traversal of every live path (persistent blocks,
-//            LDS short stack, 64-byte two-child nodes)
+// runs these kernels over compacted queues:
+//   extend : closest-hit traversal of every live path over 128-byte 4-wide
+//            nodes; persistent waves with an LDS short stack refill finished
+//            lanes and alternate inner-node and leaf phases by lane majority
 //   shade  : emission, next-event candidates, BSDF sampling, Russian
-//            roulette; appends survivors to the next queue and paths with
-//            light contributions to the shadow queue
-//   shadow : any-hit traversal of each queued path's shadow rays, adding the
-//            unoccluded contributions in the reference's slot order
+//            roulette; appends survivors to the next queue, paths with light
+//            contributions to the shadow path queue and each contribution
+//            that needs a visibility test to the shadow ray queue
+//   shadow : any-hit traversal of every queued shadow ray, marking occluded
+//            contributions
+//   fold   : adds each path's unoccluded contributions in the reference's
+//            slot order
 // followed, per wave, by `finish`, which folds each pixel's samples into the
 // accumulator in sample order.
 //
@@ -61,6 +65,50 @@ DEV uint32_t wave_append(uint32_t* counter, bool pred, uint32_t amount_if = 1) {
     if ((int)lane == leader) base = atomicAdd(counter, total);
     base = (uint32_t)__shfl((int)base, leader);
     return base + rank;
+}
+
+// Inclusive prefix sum over the lanes of a wave.
+DEV uint32_t wave_scan(uint32_t v) {
+    uint32_t lane = __lane_id();
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t u = (uint32_t)__shfl_up((int)v, off);
+        if ((int)lane >= off) v += u;
+    }
+    return v;
+}
+
+// Append a, b and c items per thread to the next-bounce, shadow path and
+// shadow ray queues with one atomic per queue for the whole 256-thread
+// block; returns each thread's first position in the three queues. Every
+// thread of the block must call it.
+DEV uint3 block_append3(uint32_t* counters, uint32_t a, uint32_t b, uint32_t c) {
+    __shared__ uint32_t tot[3][4];
+    __shared__ uint32_t base[3];
+    uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    uint32_t ia = wave_scan(a), ib = wave_scan(b), ic = wave_scan(c);
+    if (lane == 63) {
+        tot[0][wave] = ia;
+        tot[1][wave] = ib;
+        tot[2][wave] = ic;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        uint32_t q = threadIdx.x;
+        uint32_t sum = tot[q][0] + tot[q][1] + tot[q][2] + tot[q][3];
+        const uint32_t slot[3] = {C_NEXT, C_SHADOW, C_SRAYS};
+        base[q] = sum ? atomicAdd(counters + slot[q], sum) : 0u;
+    }
+    __syncthreads();
+    uint3 r;
+    r.x = base[0] + ia - a;
+    r.y = base[1] + ib - b;
+    r.z = base[2] + ic - c;
+    for (uint32_t w = 0; w < wave; ++w) {
+        r.x += tot[0][w];
+        r.y += tot[1][w];
+        r.z += tot[2][w];
+    }
+    return r;
 }
 
 // ---- traversal ------------------------------------------------------------------
@@ -176,13 +224,10 @@ struct Closest {
 // Test the four children of a Node4, then continue with the first hit
 // child and push the other hit children right to left, so children pop in
 // the reference's left-first order. Returns false when no child was hit.
-template <bool COUNT>
-DEV bool visit4(const SceneDev& S, uint32_t index, const Ray& R, float t, Stack& st, uint32_t& cur,
-                uint32_t& nodes_ctr) {
+DEV bool visit4(const SceneDev& S, uint32_t index, const Ray& R, float t, Stack& st, uint32_t& cur) {
     const float4* np = reinterpret_cast<const float4*>(S.nodes + index);
     float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
     uint4 rf = reinterpret_cast<const uint4*>(np)[6];
-    if (COUNT) nodes_ctr++;
     float n0, n1, n2, n3;
     bool h0, h1, h2, h3;
     if (R.fast) {
@@ -203,84 +248,50 @@ DEV bool visit4(const SceneDev& S, uint32_t index, const Ray& R, float t, Stack&
     return h0 || h1 || h2 || h3;
 }
 
-// Closest hit (trace.wgsl:999-1075 `scene`, with tri_hit :780-804) or, with
-// ANY, any hit (`occluded`, :1096-1168, tri_blocks :1076-1095). The leaves are
-// tested in the reference's order, so equal-distance ties resolve to the
-// same triangle.
-template <bool ANY, bool COUNT>
-DEV bool traverse(const SceneDev& S, v3 o, v3 d, float limit, Stack& st, Closest& h, uint32_t& nodes_ctr,
-                  uint32_t& prims_ctr) {
-    h.t = limit;
-    h.u = 0.0f;
-    h.v = 0.0f;
-    h.slot = 0xffffffffu;
-    h.rec = 0u;
-    const bool tame_bounds = S.bounds_tame != 0u;
-    const Ray world = make_ray(o, d, tame_bounds);
-    float nr;
-    if (S.root.ref == REF_NONE ||
-        !slab(S.root.lo[0], S.root.lo[1], S.root.lo[2], S.root.hi[0], S.root.hi[1], S.root.hi[2], world, limit, nr))
-        return false;
-    Ray R = world;
-    uint32_t base = 0, rec = 0, word = 0;
-    float inv_det = 1.0f;
-    int inst_top = -1;
-    st.top = 0;
-    uint32_t cur = S.root.ref;
-    for (;;) {
-        if ((cur & (REF_LEAF | REF_INST)) == 0u) {
-            if (visit4<COUNT>(S, cur, R, h.t, st, cur, nodes_ctr)) continue;
-        } else if (cur & REF_LEAF) {
-            bool skip = ANY ? (word & 2u) != 0u : (word & 1u) != 0u;  // casts-no-shadow / shadow-only
-            if (!skip) {
-                uint32_t first = base + (cur & REF_FIRST_MASK);
-                uint32_t count = ((cur >> 27) & 15u) + 1u;
-                for (uint32_t j = 0; j < count; ++j) {
-                    uint32_t i = first + j;
-                    if (COUNT) prims_ctr++;
-                    const float4* tp = reinterpret_cast<const float4*>(S.tris + i);
-                    float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-                    v3 A = mk(t0.x, t0.y, t0.z), e1 = mk(t0.w, t1.x, t1.y), e2 = mk(t1.z, t1.w, t2.x);
-                    v3 p = pfx::cross(R.d, e2);
-                    float det = dot(e1, p);
-                    float seen = det * inv_det;
-                    if (fabsf(seen) < 1e-9f) continue;
-                    if (!ANY && (word & ONE_SIDED) != 0u && seen < 0.0f) continue;
-                    float inv = 1.0f / det;
-                    v3 s = R.o - A;
-                    float u = dot(s, p) * inv;
-                    if (u < 0.0f || u > 1.0f) continue;
-                    v3 q = pfx::cross(s, e1);
-                    float v = dot(R.d, q) * inv;
-                    if (v < 0.0f || u + v > 1.0f) continue;
-                    float t = dot(e2, q) * inv;
-                    if (ANY) {
-                        if (t <= 0.0001f || t >= limit) continue;
-                        return true;
-                    }
-                    if (t > 0.0001f && t < h.t) {
-                        h.t = t;
-                        h.u = u;
-                        h.v = v;
-                        h.slot = i;
-                        h.rec = rec;
-                    }
-                }
-            }
-        } else {
-            const Group& g = S.groups[cur & REF_INDEX_MASK];
-            base = g.base;
-            rec = g.record;
-            word = g.word;
-            inv_det = g.inv_det;
-            if (g.has_xform) R = make_ray(to_object(g.inv, o, 1.0f), to_object(g.inv, d, 0.0f), tame_bounds);
-            inst_top = st.top;
-            if (slab(g.lo[0], g.lo[1], g.lo[2], g.hi[0], g.hi[1], g.hi[2], R, h.t, nr)) {
-                cur = g.root;
-                continue;
-            }
-        }
-        bool found = false;
+// The traversal of one ray as a resumable state: closest hit
+// (trace.wgsl:999-1075 `scene`, with tri_hit :780-804) or, with ANY, any hit
+// (`occluded`, :1096-1168, tri_blocks :1076-1095). Each ray walks the tree
+// depth first, left child first, with the same deferred pop-time distance
+// test, so it tests the reference's leaves in the reference's order and
+// equal-distance ties resolve to the same triangle. Keeping the walk in a
+// state lets a wave retire a finished ray and start a new one in its lane
+// while the other lanes keep walking.
+template <bool ANY>
+struct Walk {
+    Ray world, R;
+    Closest h;
+    float limit;
+    uint32_t cur, base, rec, word;
+    float inv_det;
+    int inst_top;
+
+    // Begin a ray; false when it misses the scene's root box.
+    DEV bool start(const SceneDev& S, v3 o, v3 d, float lim, Stack& st) {
+        limit = lim;
+        h.t = lim;
+        h.u = 0.0f;
+        h.v = 0.0f;
+        h.slot = 0xffffffffu;
+        h.rec = 0u;
+        world = make_ray(o, d, S.bounds_tame != 0u);
+        R = world;
+        base = 0;
+        rec = 0;
+        word = 0;
+        inv_det = 1.0f;
+        inst_top = -1;
+        st.top = 0;
+        cur = S.root.ref;
+        float nr;
+        return S.root.ref != REF_NONE && slab(S.root.lo[0], S.root.lo[1], S.root.lo[2], S.root.hi[0],
+                                              S.root.hi[1], S.root.hi[2], world, lim, nr);
+    }
+    DEV bool inner() const { return (cur & (REF_LEAF | REF_INST)) == 0u; }
+    DEV bool leafy() const { return (cur & REF_LEAF) != 0u; }
+    // Continue with the next deferred child whose entry distance is within
+    // the current hit distance, leaving an instance when its subtree is done;
+    // false when none is left.
+    DEV bool next(Stack& st) {
         for (;;) {
             if (st.top == inst_top) {
                 R = world;
@@ -290,30 +301,147 @@ DEV bool traverse(const SceneDev& S, v3 o, v3 d, float limit, Stack& st, Closest
                 inv_det = 1.0f;
                 inst_top = -1;
             }
-            if (st.top == 0) break;
+            if (st.top == 0) return false;
             uint2 e = st.pop();
             if (ubits(e.y) <= h.t) {
                 cur = e.x;
-                found = true;
-                break;
+                return true;
             }
         }
-        if (!found) break;
     }
-    return false;
+    // Test the current leaf's triangles in order; with ANY, true on the
+    // first blocking hit.
+    template <bool COUNT>
+    DEV bool leaf(const SceneDev& S, uint32_t& prims_ctr) {
+        bool skip = ANY ? (word & 2u) != 0u : (word & 1u) != 0u;  // casts-no-shadow / shadow-only
+        if (skip) return false;
+        uint32_t first = base + (cur & REF_FIRST_MASK);
+        uint32_t count = ((cur >> 27) & 15u) + 1u;
+        for (uint32_t j = 0; j < count; ++j) {
+            uint32_t i = first + j;
+            if (COUNT) prims_ctr++;
+            const float4* tp = reinterpret_cast<const float4*>(S.tris + i);
+            float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+            v3 A = mk(t0.x, t0.y, t0.z), e1 = mk(t0.w, t1.x, t1.y), e2 = mk(t1.z, t1.w, t2.x);
+            v3 p = pfx::cross(R.d, e2);
+            float det = dot(e1, p);
+            float seen = det * inv_det;
+            if (fabsf(seen) < 1e-9f) continue;
+            if (!ANY && (word & ONE_SIDED) != 0u && seen < 0.0f) continue;
+            float inv = 1.0f / det;
+            v3 s = R.o - A;
+            float u = dot(s, p) * inv;
+            if (u < 0.0f || u > 1.0f) continue;
+            v3 q = pfx::cross(s, e1);
+            float v = dot(R.d, q) * inv;
+            if (v < 0.0f || u + v > 1.0f) continue;
+            float t = dot(e2, q) * inv;
+            if (ANY) {
+                if (t <= 0.0001f || t >= limit) continue;
+                return true;
+            }
+            if (t > 0.0001f && t < h.t) {
+                h.t = t;
+                h.u = u;
+                h.v = v;
+                h.slot = i;
+                h.rec = rec;
+            }
+        }
+        return false;
+    }
+    // Enter the current instance: move the ray into its object space and
+    // continue at its bottom-level root; false when its box is missed.
+    DEV bool enter(const SceneDev& S, Stack& st) {
+        const Group& g = S.groups[cur & REF_INDEX_MASK];
+        base = g.base;
+        rec = g.record;
+        word = g.word;
+        inv_det = g.inv_det;
+        if (g.has_xform)
+            R = make_ray(to_object(g.inv, world.o, 1.0f), to_object(g.inv, world.d, 0.0f), S.bounds_tame != 0u);
+        inst_top = st.top;
+        float nr;
+        if (slab(g.lo[0], g.lo[1], g.lo[2], g.hi[0], g.hi[1], g.hi[2], R, h.t, nr)) {
+            cur = g.root;
+            return true;
+        }
+        return false;
+    }
+    // One step: an inner node, a leaf or an instance, then the next deferred
+    // child if needed. Returns false when the walk is over; `blocked` is
+    // set when an any-hit walk found an occluder.
+    template <bool COUNT>
+    DEV bool step(const SceneDev& S, Stack& st, bool& blocked, uint32_t& nodes_ctr, uint32_t& prims_ctr) {
+        if (inner()) {
+            if (COUNT) nodes_ctr++;
+            if (visit4(S, cur, R, h.t, st, cur)) return true;
+        } else if (leafy()) {
+            if (leaf<COUNT>(S, prims_ctr)) {
+                blocked = true;
+                return false;
+            }
+        } else if (enter(S, st)) {
+            return true;
+        }
+        return next(st);
+    }
+};
+
+// Whole walks for one ray (used where lanes do not refill).
+DEV Closest trace_closest(const SceneDev& S, v3 o, v3 d, float tmax, Stack& st) {
+    Walk<false> w;
+    uint32_t nodes = 0, prims = 0;
+    bool blocked = false;
+    if (w.start(S, o, d, tmax, st))
+        while (w.step<false>(S, st, blocked, nodes, prims)) {
+        }
+    return w.h;
 }
 
-template <bool COUNT>
-DEV Closest trace_closest(const SceneDev& S, v3 o, v3 d, float tmax, Stack& st, uint32_t& nodes_ctr,
-                          uint32_t& prims_ctr) {
-    Closest h;
-    traverse<false, COUNT>(S, o, d, tmax, st, h, nodes_ctr, prims_ctr);
-    return h;
+// Persistent waves refill finished lanes. The queue [0, n) is split into
+// PARTS contiguous parts with a counter each (on its own slot); a wave
+// starts on the part of its XCD (workgroups go to the XCDs round-robin) and
+// moves to the next part once its part is used up. Once at least `refill`
+// lanes of a wave are idle, the wave claims that many positions of its part
+// with one atomic and hands them out in lane order.
+struct Fetch {
+    uint32_t part;  // wave-uniform
+    uint32_t hops;  // parts found used up; PARTS means the queue is drained
+};
+DEV Fetch fetch_start() { return Fetch{blockIdx.x % PARTS, 0u}; }
+DEV bool fetch_drained(const Fetch& f) { return f.hops >= PARTS; }
+
+// Returns the queue position for an idle lane that got one, else
+// 0xffffffff. Every lane of the wave must call it.
+DEV uint32_t refill_lanes(bool idle, uint32_t refill, uint32_t n, uint32_t* counters, Fetch& f) {
+    unsigned long long mask = __ballot(idle);
+    uint32_t nidle = (uint32_t)__popcll(mask);
+    if (fetch_drained(f) || nidle < refill || nidle == 0u) return 0xffffffffu;
+    uint32_t lane = __lane_id();
+    unsigned long long below = mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+    uint32_t rank = (uint32_t)__popcll(below);
+    for (;;) {
+        uint32_t lo = (uint32_t)((uint64_t)n * f.part / PARTS);
+        uint32_t hi = (uint32_t)((uint64_t)n * (f.part + 1u) / PARTS);
+        uint32_t first = 0;
+        if (lane == 0) first = atomicAdd(counters + f.part * SLOT, nidle);
+        first = (uint32_t)__shfl((int)first, 0);
+        if (first < hi - lo) {
+            uint32_t k = lo + first + rank;
+            return (idle && k < hi) ? k : 0xffffffffu;
+        }
+        f.part = (f.part + 1u) % PARTS;
+        if (++f.hops >= PARTS) return 0xffffffffu;
+    }
 }
-template <bool COUNT>
-DEV bool trace_any(const SceneDev& S, v3 o, v3 d, float limit, Stack& st, uint32_t& nodes_ctr, uint32_t& prims_ctr) {
-    Closest h;
-    return traverse<true, COUNT>(S, o, d, limit, st, h, nodes_ctr, prims_ctr);
+
+// A wave runs one phase per round: inner-node steps when the lanes waiting on
+// inner nodes are at least two thirds as many as those waiting on leaves or
+// instances, else the leaf phase. Every lane calls it (wave-uniform result).
+DEV bool inner_round(bool in, bool out) {
+    uint32_t ni = (uint32_t)__popcll(__ballot(in)), nl = (uint32_t)__popcll(__ballot(out));
+    return ni * 3u >= nl * 2u;
 }
 
 // ---- surface at a hit (trace.wgsl:754-779 tri_fill, :681-694 world_hit) -----
@@ -834,26 +962,60 @@ __global__ __launch_bounds__(256) void k_camera(SceneDev S, WaveDev W, const uin
     W.nrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+DEV void store_closest(const WaveDev& W, uint32_t i, const Closest& h) {
+    W.hit[i] = make_float4(h.t, h.u, h.v, ubits(h.slot));
+    W.hit_rec[i] = h.rec;
+}
+
+// Closest hit of every queued path. Persistent waves refill finished lanes
+// from a shared counter, and each round walks inner nodes until every live
+// lane stands on a leaf or an instance, then handles those together, so
+// lanes diverge as little as the rays allow.
 template <bool COUNT>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev S, WaveDev W, const uint32_t* queue, uint32_t n) {
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
     uint32_t nodes = 0, prims = 0;
-    // persistent waves take 64 rays at a time from a shared counter, so a
-    // wave that drew short rays comes back for more
+    Walk<false> w;
+    uint32_t ray = 0;
+    bool live = false;
+    Fetch f = fetch_start();
     for (;;) {
-        uint32_t base = 0;
-        if (__lane_id() == 0) base = atomicAdd(W.counters + C_FETCH, 64u);
-        base = (uint32_t)__shfl((int)base, 0);
-        if (base >= n) break;
-        uint32_t k = base + __lane_id();
-        if (k >= n) continue;
-        uint32_t i = queue ? queue[k] : k;
-        float4 o4 = W.o[i], d4 = W.d[i];
-        Closest h = trace_closest<COUNT>(S, f3(o4), f3(d4), 1e20f, st, nodes, prims);
-        W.hit[i] = make_float4(h.t, h.u, h.v, ubits(h.slot));
-        W.hit_rec[i] = h.rec;
+        uint32_t k = refill_lanes(!live, W.refill, n, W.counters + C_FETCH, f);
+        if (k != 0xffffffffu) {
+            ray = queue ? queue[k] : k;
+            float4 o4 = W.o[ray], d4 = W.d[ray];
+            live = w.start(S, f3(o4), f3(d4), 1e20f, st);
+            if (!live) store_closest(W, ray, w.h);
+        }
+        if (!__any(live)) {
+            if (fetch_drained(f)) break;
+            continue;
+        }
+        // one phase per round: inner nodes, or leaves and instances, whichever
+        // has more lanes waiting (inner steps favoured 3:2)
+        bool in = live && w.inner();
+        bool out = live && !in;
+        if (inner_round(in, out)) {
+            if (in) {
+                if (COUNT) nodes++;
+                if (!visit4(S, w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) {
+                    store_closest(W, ray, w.h);
+                    live = false;
+                }
+            }
+        } else if (out) {
+            bool go = true;
+            if (w.leafy())
+                w.template leaf<COUNT>(S, prims);
+            else
+                go = !w.enter(S, st);
+            if (go && !w.next(st)) {
+                store_closest(W, ray, w.h);
+                live = false;
+            }
+        }
     }
     if (COUNT) {
         atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_NODES), (unsigned long long)nodes);
@@ -867,7 +1029,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, WaveDev W, const uint
     uint32_t k = blockIdx.x * 256u + threadIdx.x;
     bool valid = k < n;
     bool cont = false;
-    uint32_t entries = 0;
+    uint32_t entries = 0, tested = 0;
     uint32_t i = 0;
     if (valid) {
         i = queue ? queue[k] : k;
@@ -981,6 +1143,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, WaveDev W, const uint
                         W.e_o[e] = f4(origin, limit > 0.0f ? limit : 0.0f);
                         W.e_d[e] = f4(direction, 0.0f);
                         W.e_c[e] = f4(add, 0.0f);
+                        if (limit > 0.0f) tested |= 1u << entries;
                         entries++;
                     }
                 }
@@ -1039,48 +1202,78 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, WaveDev W, const uint
         W.thr[i] = f4(throughput, ubits(state));
         W.res[i] = f4(result, 0.0f);
     }
-    uint32_t at = wave_append(W.counters + C_NEXT, cont);
-    if (cont) W.q_next[at] = i;
     bool sh = valid && entries > 0u;
-    uint32_t as = wave_append(W.counters + C_SHADOW, sh);
-    if (sh) W.q_sh[as] = i | (entries << 28);
+    uint3 at = block_append3(W.counters, cont ? 1u : 0u, sh ? 1u : 0u, (uint32_t)__popc(tested));
+    if (cont) W.q_next[at.x] = i;
+    if (sh) W.q_sh[at.y] = i | (entries << 28);
+    for (uint32_t bits = tested, r = at.z; bits; bits &= bits - 1u, ++r)
+        W.q_sray[r] = ((uint32_t)(__ffs(bits) - 1) << 28) | i;
 }
 
-// Shadow rays of one path, in slot order (trace.wgsl:2465-2470).
+// Any-hit test of every queued shadow ray (one NEE entry each). An occluded
+// entry is marked in its contribution's w; k_fold adds the others.
 template <bool COUNT>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev S, WaveDev W, uint32_t n) {
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
-    uint32_t nodes = 0, prims = 0, rays = 0;
+    uint32_t nodes = 0, prims = 0;
+    Walk<true> w;
+    size_t at = 0;
+    bool live = false;
+    Fetch f = fetch_start();
     for (;;) {
-        uint32_t base = 0;
-        if (__lane_id() == 0) base = atomicAdd(W.counters + C_SFETCH, 64u);
-        base = (uint32_t)__shfl((int)base, 0);
-        if (base >= n) break;
-        uint32_t k = base + __lane_id();
-        if (k >= n) continue;
-        uint32_t q = W.q_sh[k];
-        uint32_t i = q & 0x0fffffffu, m = q >> 28;
-        float4 r = W.res[i];
-        v3 result = f3(r);
-        for (uint32_t e = 0; e < m; ++e) {
-            size_t at = (size_t)e * W.capacity + i;
-            float4 eo = W.e_o[at];
-            if (eo.w > 0.0f) {
-                rays++;
-                float4 ed = W.e_d[at];
-                if (trace_any<COUNT>(S, f3(eo), f3(ed), eo.w, st, nodes, prims)) continue;
-            }
-            result = result + f3(W.e_c[at]);
+        uint32_t k = refill_lanes(!live, W.refill, n, W.counters + C_SFETCH, f);
+        if (k != 0xffffffffu) {
+            uint32_t q = W.q_sray[k];
+            at = (size_t)(q >> 28) * W.capacity + (q & 0x0fffffffu);
+            float4 eo = W.e_o[at], ed = W.e_d[at];
+            live = w.start(S, f3(eo), f3(ed), eo.w, st);
         }
-        W.res[i] = f4(result, 0.0f);
+        if (!__any(live)) {
+            if (fetch_drained(f)) break;
+            continue;
+        }
+        bool in = live && w.inner();
+        bool out = live && !in;
+        if (inner_round(in, out)) {
+            if (in) {
+                if (COUNT) nodes++;
+                if (!visit4(S, w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) live = false;
+            }
+        } else if (out) {
+            bool go = true;
+            if (w.leafy()) {
+                if (w.template leaf<COUNT>(S, prims)) {
+                    reinterpret_cast<float*>(W.e_c + at)[3] = 1.0f;
+                    live = false;
+                    go = false;
+                }
+            } else {
+                go = !w.enter(S, st);
+            }
+            if (go && !w.next(st)) live = false;
+        }
     }
-    if (rays) atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_SHADOW_RAYS), (unsigned long long)rays);
     if (COUNT) {
         atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_SNODES), (unsigned long long)nodes);
         atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_SPRIMS), (unsigned long long)prims);
     }
+}
+
+// Add each queued path's unoccluded light contributions in slot order
+// (trace.wgsl:2465-2470).
+__global__ __launch_bounds__(256) void k_fold(WaveDev W, uint32_t n) {
+    uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= n) return;
+    uint32_t q = W.q_sh[k];
+    uint32_t i = q & 0x0fffffffu, m = q >> 28;
+    v3 result = f3(W.res[i]);
+    for (uint32_t e = 0; e < m; ++e) {
+        float4 c = W.e_c[(size_t)e * W.capacity + i];
+        if (c.w == 0.0f) result = result + f3(c);
+    }
+    W.res[i] = f4(result, 0.0f);
 }
 
 DEV void neumaier(float& s, float& c, float x) {
@@ -1226,7 +1419,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_probe(SceneDev S, WaveDev W, pf
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     uint32_t stride = gridDim.x * TRACE_BLOCK;
     Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
-    uint32_t pixels = S.width * S.height, dn = 0, dp = 0;
+    uint32_t pixels = S.width * S.height;
     v3 fwd = mk(S.forward[0], S.forward[1], S.forward[2]), right = mk(S.right[0], S.right[1], S.right[2]),
        up = mk(S.up[0], S.up[1], S.up[2]);
     for (uint32_t pix = gtid; pix < pixels; pix += stride) {
@@ -1235,7 +1428,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_probe(SceneDev S, WaveDev W, pf
         float ndcy = ((float)(y + S.window[3]) + 0.5f) / (float)S.window[1] * 2.0f - 1.0f;
         v3 d = normalize(fwd + right * (ndcx + S.lens[0]) + up * (-ndcy + S.lens[1]));
         v3 o = mk(S.origin[0], S.origin[1], S.origin[2]);
-        Closest h = trace_closest<false>(S, o, d, 1e20f, st, dn, dp);
+        Closest h = trace_closest(S, o, d, 1e20f, st);
         pfxh_probe_hit r;
         r.depth = 0.0f;
         r.distance = 0.0f;
@@ -1347,6 +1540,10 @@ void launch_shadow(hipStream_t s, const SceneDev& S, const WaveDev& W, uint32_t 
         hipLaunchKernelGGL(k_shadow<true>, dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, W, n);
     else
         hipLaunchKernelGGL(k_shadow<false>, dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, W, n);
+}
+void launch_fold(hipStream_t s, const WaveDev& W, uint32_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fold, dim3(blocks_for(n, 256)), dim3(256), 0, s, W, n);
 }
 void launch_finish(hipStream_t s, const WaveDev& W, const AccumDev& A, const uint32_t* active, uint32_t npix,
                    uint32_t nsamples, float count_after) {

@@ -7,6 +7,7 @@
 // submissions (gpu.rs:2992-3107) has no counterpart: the compute queue is
 // not shared with a display, so a call runs its waves back to back.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -95,6 +96,14 @@ struct pfxh_trace {
 };
 
 namespace {
+
+// Idle lanes out of 64 at which a traversal wave fetches new rays; a tuning
+// knob (PFXH_REFILL), clamped to 1..64.
+uint32_t refill_lanes() {
+    const char* v = std::getenv("PFXH_REFILL");
+    long n = v ? std::strtol(v, nullptr, 10) : 16;
+    return (uint32_t)std::min(64L, std::max(1L, n));
+}
 
 uint32_t spill_threads_for() {
     int device = 0, cus = 256;
@@ -198,9 +207,11 @@ void setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& detail, ui
     W.e_c = (float4*)t->alloc(cap * entries * 16, "shadow ray contributions");
     t->qa = (uint32_t*)t->alloc(cap * 4, "ray queue");
     t->qb = (uint32_t*)t->alloc(cap * 4, "ray queue");
-    W.q_sh = (uint32_t*)t->alloc(cap * 4, "shadow queue");
+    W.q_sh = (uint32_t*)t->alloc(cap * 4, "shadow path queue");
+    W.q_sray = (uint32_t*)t->alloc(cap * entries * 4, "shadow ray queue");
     W.counters = (uint32_t*)t->alloc(C_COUNT * 4, "counters");
     W.spill_threads = spill_threads_for();
+    W.refill = refill_lanes();
     W.spill = (uint2*)t->alloc((size_t)W.spill_threads * (STACK_DEPTH - STACK_SHORT) * 8, "traversal stacks");
     t->active = (uint32_t*)t->alloc(pixels * 4, "active pixels");
     AccumDev& A = t->A;
@@ -261,14 +272,15 @@ void sample(pfxh_trace* t, uint32_t count, uint32_t seed) {
         for (uint32_t bounce = 0; bounce <= t->S.total + 1u && n > 0; ++bounce) {
             WaveDev W = t->W;
             W.q_next = outq;
-            launch_zero(s, W.counters + C_FETCH, 2);
+            launch_zero(s, W.counters + C_FETCH, PARTS * SLOT);
             HIP_TRY(hipEventRecord(t->ev[0], s));
             launch_extend(s, t->S, W, queue, n, t->counting, grid);
             HIP_TRY(hipEventRecord(t->ev[1], s));
-            launch_zero(s, W.counters, 2);
+            launch_zero(s, W.counters, C_SRAYS + 1);
             launch_shade(s, t->S, W, queue, n, bounce);
             HIP_TRY(hipEventRecord(t->ev[2], s));
-            HIP_TRY(hipMemcpyAsync(t->host_counters, W.counters, 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(t->host_counters, W.counters, (C_SRAYS + 1) * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             HIP_TRY(hipGetLastError());
             t->stats.closest_rays += n;
@@ -278,11 +290,14 @@ void sample(pfxh_trace* t, uint32_t count, uint32_t seed) {
             if (shadow_pending) t->stats.shadow_ms += elapsed(t->ev[3], t->ev[4]);
             shadow_pending = false;
             uint32_t n_next = t->host_counters[C_NEXT], n_sh = t->host_counters[C_SHADOW];
+            uint32_t n_rays = t->host_counters[C_SRAYS];
             if (n_sh) {
-                launch_zero(s, W.counters + C_SFETCH, 1);
+                launch_zero(s, W.counters + C_SFETCH, PARTS * SLOT);
                 HIP_TRY(hipEventRecord(t->ev[3], s));
-                launch_shadow(s, t->S, W, n_sh, t->counting, grid);
+                launch_shadow(s, t->S, W, n_rays, t->counting, grid);
+                launch_fold(s, W, n_sh);
                 HIP_TRY(hipEventRecord(t->ev[4], s));
+                t->stats.shadow_rays += n_rays;
                 shadow_pending = true;
             }
             queue = outq;
@@ -521,7 +536,6 @@ pfxh_status pfxh_trace_stats(const pfxh_trace* t, pfxh_stats* out) {
         return PFXH_DEVICE;
     }
     auto u64 = [&](int k) { return (uint64_t)c[k] | ((uint64_t)c[k + 1] << 32); };
-    out->shadow_rays = u64(C_SHADOW_RAYS);
     out->nodes_visited = u64(C_NODES);
     out->prims_tested = u64(C_PRIMS);
     out->shadow_nodes = u64(C_SNODES);
