@@ -36,11 +36,12 @@ WIDTH, HEIGHT = 3840, 2160
 ADAPTIVE = Adaptive(threshold=0.01, min_samples=16, max_samples=64, growth=1.5)
 BOUNCES = Bounces.deep(6)
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes of the closest-hit kernel: one 128-byte 4-wide node per
-# node visit, one 48-byte edge-form triangle per triangle test, and per ray
-# its queue entry (4), origin and direction (32), hit record (16) and hit
-# instance record (4)
-NODE_BYTES, PRIM_BYTES, RAY_IO_BYTES = 128, 48, 4 + 32 + 16 + 4
+# algorithmic bytes of the closest-hit kernel: per node visit the six box
+# planes and four child references of a 4-wide node (112 of its 128 bytes),
+# per triangle test one 48-byte edge-form triangle, and per ray its queue
+# entry (4), origin and direction (32), hit record (16) and hit instance
+# record (4)
+NODE_BYTES, PRIM_BYTES, RAY_IO_BYTES = 112, 48, 4 + 32 + 16 + 4
 # HBM bytes per closest-hit ray measured with rocprofv3 counters (FETCH_SIZE
 # and WRITE_SIZE of k_extend, calibrated on a copy kernel); absent -> null
 TRAFFIC_FILE = ROOT / "profiles" / "k_extend_traffic.json"

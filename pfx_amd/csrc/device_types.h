@@ -1,12 +1,12 @@
 // Device record formats, shared by the host layout code and the kernels.
 // Every record is a whole number of 16-byte vectors so a wave fetches it
-// with dwordx4 loads; node and triangle records are 64 and 48 bytes.
+// with dwordx4 loads; node and triangle records are 128 and 48 bytes.
 #pragma once
 #include <stdint.h>
 
 namespace pfxh {
 
-// Child reference encoding used by WideNode, RootEntry and Group.
+// Child reference encoding used by Node4, RootEntry and Group.
 //   inner node : index into the node array           (bits 31,30 clear)
 //   triangles  : bit 31 | (count - 1) << 27 | first  (count <= 16, first < 2^27,
 //                first relative to the group's triangle base)

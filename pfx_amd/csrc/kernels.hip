@@ -971,8 +971,11 @@ DEV void store_closest(const WaveDev& W, uint32_t i, const Closest& h) {
 // from a shared counter, and each round walks inner nodes until every live
 // lane stands on a leaf or an instance, then handles those together, so
 // lanes diverge as little as the rays allow.
+// Six waves per SIMD: the walk is bound by dependent node fetches, and the
+// extra resident waves hide more of that latency than the few registers
+// spilled at this limit cost.
 template <bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev S, WaveDev W, const uint32_t* queue, uint32_t n) {
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(SceneDev S, WaveDev W, const uint32_t* queue, uint32_t n) {
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
@@ -1213,7 +1216,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, WaveDev W, const uint
 // Any-hit test of every queued shadow ray (one NEE entry each). An occluded
 // entry is marked in its contribution's w; k_fold adds the others.
 template <bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev S, WaveDev W, uint32_t n) {
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(SceneDev S, WaveDev W, uint32_t n) {
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
