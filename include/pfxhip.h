@@ -149,6 +149,9 @@ typedef struct pfxh_bounces {
 
 enum { PFXH_ACCUM_COMPENSATED = 0, PFXH_ACCUM_PLAIN = 1 };
 
+/* gpu::Projection (crates/trace/src/gpu.rs:181-189) */
+enum { PFXH_PROJ_PERSPECTIVE = 0, PFXH_PROJ_ORTHOGRAPHIC = 1, PFXH_PROJ_EQUIRECT = 2 };
+
 /* The parts of detail::Detail (crates/trace/src/detail.rs:374-398) the
  * path uses. The sampler is PCG (`Sampler::Pcg`, sobol.rs:13-23). */
 typedef struct pfxh_detail {
@@ -159,6 +162,8 @@ typedef struct pfxh_detail {
     float sun_radius_deg;
     uint32_t accumulate; /* PFXH_ACCUM_COMPENSATED (Neumaier) or PFXH_ACCUM_PLAIN */
     uint32_t wave_paths; /* paths in flight per wave; 0 picks a default */
+    uint32_t projection; /* PFXH_PROJ_*; Detail::projection */
+    float ortho_size[2]; /* orthographic view width and height (PFXH_PROJ_ORTHOGRAPHIC) */
 } pfxh_detail;
 
 /* adaptive::Adaptive (crates/trace/src/adaptive.rs:9-26) */
@@ -234,6 +239,11 @@ pfxh_status pfxh_trace_set_sun(pfxh_trace* trace, const pfxh_sun* sun, char* err
 /* Trace::set_lens (gpu.rs:2858-2870); resets the sample count */
 pfxh_status pfxh_trace_set_lens(pfxh_trace* trace, float shift_x, float shift_y, float aperture,
                                 float focus_distance, char* err, size_t err_len);
+/* Trace::set_projection (gpu.rs:2872-2890): `kind` is a PFXH_PROJ_* value,
+ * width/height the orthographic view size (ignored otherwise); resets the
+ * sample count */
+pfxh_status pfxh_trace_set_projection(pfxh_trace* trace, uint32_t kind, float width, float height,
+                                      char* err, size_t err_len);
 /* Trace::samples (gpu.rs:3232-3234) */
 uint32_t pfxh_trace_samples(const pfxh_trace* trace);
 /* Trace::gpu_bytes (gpu.rs:2360) */
@@ -284,6 +294,52 @@ pfxh_status pfxh_layout_lights(const pfxh_light* lights, uint32_t count, float* 
  * header first) and the CDF. Call with entries == NULL to learn the count. */
 pfxh_status pfxh_layout_emitters(const pfxh_scene* scene, float* entries, float* cdf,
                                  uint32_t* emitter_count, char* err, size_t err_len);
+
+/* ---- checkpoints (crates/trace/src/checkpoint.rs) ------------------------ */
+
+#define PFXH_NO_ROUNDS 0xffffffffu
+
+/* checkpoint::Saved (checkpoint.rs:13-28): sizes and pointers to the blocks.
+ * accum is 48 bytes per pixel (Accum: color.xyz + sample count, albedo,
+ * normal sums), color/albedo/normal 16 bytes per pixel (RGBA32F), snaps 32
+ * bytes per pixel and present only with adaptive rounds. */
+typedef struct pfxh_checkpoint {
+    uint32_t width;
+    uint32_t height;
+    uint32_t samples;
+    uint32_t round_count; /* PFXH_NO_ROUNDS when the checkpoint has no adaptive rounds */
+    const uint8_t* accum;
+    const uint8_t* color;
+    const uint8_t* albedo;
+    const uint8_t* normal;
+    const uint32_t* round_samples;
+    const uint64_t* round_active;
+    const uint8_t* snaps;
+} pfxh_checkpoint;
+
+/* Saved::write_to (checkpoint.rs:85-110): the PFXCKPT1 bytes under `key`.
+ * With out == NULL only *len is set. Host only. */
+pfxh_status pfxh_checkpoint_write(const pfxh_checkpoint* checkpoint, const char* key, uint8_t* out,
+                                  size_t cap, size_t* len, char* err, size_t err_len);
+/* Saved::read_from (checkpoint.rs:112-158): parse `bytes` made under `key`;
+ * the block pointers of *out point into `bytes`, and up to round_cap rounds
+ * are copied to round_samples/round_active. Host only. */
+pfxh_status pfxh_checkpoint_read(const uint8_t* bytes, size_t len, const char* key,
+                                 pfxh_checkpoint* out, uint32_t* round_samples,
+                                 uint64_t* round_active, uint32_t round_cap, char* err,
+                                 size_t err_len);
+/* Trace::save (checkpoint.rs:163-174) + Saved::write_to: the trace's
+ * accumulator, images and sample count as PFXCKPT1 bytes (no adaptive
+ * rounds). A compensated or FP64 accumulator is saved as its FP32 sum.
+ * With out == NULL only *len is set. */
+pfxh_status pfxh_trace_save(pfxh_trace* trace, const char* key, uint8_t* out, size_t cap,
+                            size_t* len, char* err, size_t err_len);
+/* Saved::read_from + Trace::restore (checkpoint.rs:176-200): continue from
+ * the checkpoint's accumulator and sample count; the images are developed
+ * again from the accumulator at readback. Adaptive rounds in the
+ * checkpoint are read and ignored, as Trace::restore does. */
+pfxh_status pfxh_trace_restore(pfxh_trace* trace, const char* key, const uint8_t* bytes, size_t len,
+                               char* err, size_t err_len);
 
 /* Library version string. */
 const char* pfxh_version(void);

@@ -61,6 +61,8 @@ struct Scene {
     float sun_intensity = 0.0f, sun_w = 1.0f;
     float lens[4] = {0, 0, 0, 0};
     uint32_t window[4] = {0, 0, 0, 0};
+    uint32_t projection = 0;            // 0 perspective, 1 orthographic, 2 equirect
+    float half_view[2] = {0.0f, 0.0f};  // orthographic half extents
     uint32_t width = 0, height = 0;
     pfxh_bounces bounces{16, 4, 8, 12};
     uint32_t accumulate = ACC_COMP;
@@ -894,7 +896,18 @@ static void camera_ray(const Scene& sc, uint32_t px, uint32_t py, float jx, floa
     float ndcy = ((float)py + jy) / (float)sc.window[1] * 2.0f - 1.0f;
     v3 direction = normalize(sc.forward + sc.right * (ndcx + sc.lens[0]) + sc.up * (-ndcy + sc.lens[1]));
     v3 origin = sc.origin;
-    if (!centre && sc.lens[2] > 0.0f) {
+    if (sc.projection == 1u) {
+        // trace.wgsl:2546-2548: orthographic, rays along forward from the view rectangle
+        direction = normalize(sc.forward);
+        origin = origin + (normalize(sc.right) * ((ndcx + sc.lens[0]) * sc.half_view[0]) +
+                           normalize(sc.up) * ((-ndcy + sc.lens[1]) * sc.half_view[1]));
+    } else if (sc.projection == 2u) {
+        // trace.wgsl:2549-2553: equirect, direction from longitude and latitude
+        float phi = ((ndcx * 0.5f) * 2.0f) * PI;
+        float theta = (ndcy * 0.5f + 0.5f) * PI;
+        v3 local = mk(fm_sin(theta) * fm_sin(phi), fm_cos(theta), -fm_sin(theta) * fm_cos(phi));
+        direction = normalize(normalize(sc.right) * local.x + normalize(sc.up) * local.y - normalize(sc.forward) * local.z);
+    } else if (!centre && sc.lens[2] > 0.0f) {
         v3 focus = origin + direction * (sc.lens[3] / dot(direction, normalize(sc.forward)));
         float radius = sqrtf(pfx::rnd(state)) * sc.lens[2];
         float phi = 2.0f * PI * pfx::rnd(state);
@@ -1136,6 +1149,9 @@ static std::string prepare(Scene& sc, const pfxh_scene& in, const pfxh_detail& d
     sc.lens[1] = detail.lens_shift[1];
     sc.lens[2] = detail.aperture * 0.5f;
     sc.lens[3] = detail.focus_distance;
+    sc.projection = detail.projection;
+    sc.half_view[0] = detail.projection == 1u ? detail.ortho_size[0] * 0.5f : 0.0f;  // gpu.rs:210-217
+    sc.half_view[1] = detail.projection == 1u ? detail.ortho_size[1] * 0.5f : 0.0f;
     sc.width = w;
     sc.height = h;
     sc.window[0] = w;
@@ -1216,7 +1232,7 @@ void or_probe(void* h, pfxh_probe_hit* out) {
             r.primitive = 0xffffffffu;
             if (hit.t >= 1e19f) continue;
             v3 n = dot(hit.n, d) > 0.0f ? -hit.n : hit.n;
-            r.depth = hit.t * dot(d, normalize(sc->forward));
+            r.depth = sc->projection == 2u ? hit.t : hit.t * dot(d, normalize(sc->forward));  // probe.wgsl:33-34
             r.distance = hit.t;
             r.material_id = hit.material + 1u;
             r.primitive = sc->laid.slot_prim[hit.surface];

@@ -5,10 +5,14 @@ The product is libpfxhip.so (include/pfxhip.h); this package is its Python
 host: ctypes bindings, the `Trace` mirror, scene inputs and the seeded
 workload generators.
 """
-from .scene import Adaptive, Bounces, Camera, Detail, Scene, Sun, material, point_light, spot_light
+from .scene import (EQUIRECT, ORTHOGRAPHIC, PERSPECTIVE, Adaptive, Bounces, Camera, Detail, Scene, Sun, material,
+                    point_light, spot_light)
 from .trace import Converged, Output, Probe, Trace, TraceError
 
 __all__ = [
+    "EQUIRECT",
+    "ORTHOGRAPHIC",
+    "PERSPECTIVE",
     "Adaptive",
     "Bounces",
     "Camera",

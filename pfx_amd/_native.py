@@ -64,6 +64,7 @@ assert MATERIAL.itemsize == 52 and LIGHT.itemsize == 64 and PROBE_HIT.itemsize =
 
 LIGHT_POINT, LIGHT_SPOT = 0, 1
 ACCUM_COMPENSATED, ACCUM_PLAIN, ACCUM_FP64 = 0, 1, 2
+PROJ_PERSPECTIVE, PROJ_ORTHOGRAPHIC, PROJ_EQUIRECT = 0, 1, 2
 
 OK, REFUSED, SETUP, INVALID, DEVICE = 0, 1, 2, 3, 4
 MAX_ROUNDS = 64
@@ -126,6 +127,8 @@ class DetailC(C.Structure):
         ("sun_radius_deg", C.c_float),
         ("accumulate", C.c_uint32),
         ("wave_paths", C.c_uint32),
+        ("projection", C.c_uint32),
+        ("ortho_size", C.c_float * 2),
     ]
 
 
@@ -164,6 +167,24 @@ class StatsC(C.Structure):
 
 
 # Every symbol include/pfxhip.h declares, with its ctypes signature.
+class CheckpointC(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("samples", C.c_uint32),
+        ("round_count", C.c_uint32),
+        ("accum", C.c_void_p),
+        ("color", C.c_void_p),
+        ("albedo", C.c_void_p),
+        ("normal", C.c_void_p),
+        ("round_samples", C.c_void_p),
+        ("round_active", C.c_void_p),
+        ("snaps", C.c_void_p),
+    ]
+
+
+NO_ROUNDS = 0xFFFFFFFF
+
 _P = C.c_void_p
 _U = C.c_uint32
 _S = C.c_char_p
@@ -177,6 +198,11 @@ SIGNATURES = {
     "pfxh_trace_set_window": (C.c_int, [_P, _U, _U, _U, _U, _S, _Z]),
     "pfxh_trace_set_sun": (C.c_int, [_P, _P, _S, _Z]),
     "pfxh_trace_set_lens": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_float, _S, _Z]),
+    "pfxh_trace_set_projection": (C.c_int, [_P, C.c_uint32, C.c_float, C.c_float, _S, _Z]),
+    "pfxh_checkpoint_write": (C.c_int, [_P, _S, _P, _Z, C.POINTER(C.c_size_t), _S, _Z]),
+    "pfxh_checkpoint_read": (C.c_int, [_P, _Z, _S, _P, _P, _P, _U, _S, _Z]),
+    "pfxh_trace_save": (C.c_int, [_P, _S, _P, _Z, C.POINTER(C.c_size_t), _S, _Z]),
+    "pfxh_trace_restore": (C.c_int, [_P, _S, _P, _Z, _S, _Z]),
     "pfxh_trace_samples": (_U, [_P]),
     "pfxh_trace_gpu_bytes": (C.c_uint64, [_P]),
     "pfxh_trace_probe": (C.c_int, [_P, _P, _S, _Z]),

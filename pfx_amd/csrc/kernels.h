@@ -31,6 +31,8 @@ struct SceneDev {
     uint32_t total, diffuse, glossy, transmission;
     float sun_dir[3], sun_color[3], sun_intensity, sun_w;
     float origin[3], forward[3], right[3], up[3], lens[4];
+    uint32_t projection;  // PFXH_PROJ_*
+    float half_view[2];   // orthographic half width and half height
     uint32_t window[4];
     uint32_t width, height;
 };

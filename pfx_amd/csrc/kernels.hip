@@ -929,6 +929,23 @@ DEV float sky_pdf(const SceneDev& S, v3 direction) {
 
 // ---- kernels ------------------------------------------------------------------------
 // Camera rays (trace.wgsl:2504-2561): one path per (sample, active pixel).
+// Orthographic and equirect cameras (trace.wgsl:2546-2555, probe.wgsl:18-26):
+// an orthographic ray starts on the view rectangle and runs along forward;
+// an equirect ray takes its direction from longitude and latitude.
+DEV void project(const SceneDev& S, float ndcx, float ndcy, v3 fwd, v3 right, v3 up, v3& origin, v3& direction) {
+    if (S.projection == 1u) {
+        direction = normalize(fwd);
+        origin = origin + (normalize(right) * ((ndcx + S.lens[0]) * S.half_view[0]) +
+                           normalize(up) * ((-ndcy + S.lens[1]) * S.half_view[1]));
+    } else {
+        float phi = ((ndcx * 0.5f) * 2.0f) * PI;
+        float theta = (ndcy * 0.5f + 0.5f) * PI;
+        float st = fm_sin(theta);
+        v3 local = mk(st * fm_sin(phi), fm_cos(theta), -st * fm_cos(phi));
+        direction = normalize(normalize(right) * local.x + normalize(up) * local.y - normalize(fwd) * local.z);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_camera(SceneDev S, WaveDev W, const uint32_t* active, uint32_t npix,
                                                 uint32_t nsamples, uint32_t acc, uint32_t seed) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -947,7 +964,9 @@ __global__ __launch_bounds__(256) void k_camera(SceneDev S, WaveDev W, const uin
        up = mk(S.up[0], S.up[1], S.up[2]);
     v3 direction = normalize(fwd + right * (ndcx + S.lens[0]) + up * (-ndcy + S.lens[1]));
     v3 origin = mk(S.origin[0], S.origin[1], S.origin[2]);
-    if (S.lens[2] > 0.0f) {
+    if (S.projection != 0u) {
+        project(S, ndcx, ndcy, fwd, right, up, origin, direction);
+    } else if (S.lens[2] > 0.0f) {
         v3 focus = origin + direction * (S.lens[3] / dot(direction, normalize(fwd)));
         float radius = sqrtf(pfx::rnd(&state)) * S.lens[2];
         float phi = 2.0f * PI * pfx::rnd(&state);
@@ -1431,6 +1450,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_probe(SceneDev S, WaveDev W, pf
         float ndcy = ((float)(y + S.window[3]) + 0.5f) / (float)S.window[1] * 2.0f - 1.0f;
         v3 d = normalize(fwd + right * (ndcx + S.lens[0]) + up * (-ndcy + S.lens[1]));
         v3 o = mk(S.origin[0], S.origin[1], S.origin[2]);
+        if (S.projection != 0u) project(S, ndcx, ndcy, fwd, right, up, o, d);
         Closest h = trace_closest(S, o, d, 1e20f, st);
         pfxh_probe_hit r;
         r.depth = 0.0f;
@@ -1442,7 +1462,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_probe(SceneDev S, WaveDev W, pf
         if (h.t < 1e19f) {
             Surf sf = surface_at(S, h.slot, h.rec, h.u, h.v);
             v3 n = dot(sf.n, d) > 0.0f ? -sf.n : sf.n;
-            r.depth = h.t * dot(d, normalize(fwd));
+            r.depth = S.projection == 2u ? h.t : h.t * dot(d, normalize(fwd));
             r.distance = h.t;
             r.material_id = sf.material + 1u;
             r.primitive = S.slot_prim[h.slot];

@@ -117,6 +117,24 @@ uint32_t spill_threads_for() {
     return (uint32_t)cus * 16u * (uint32_t)TRACE_BLOCK;
 }
 
+// Projection::valid and Projection::fits (gpu.rs:191-208)
+bool projection_valid(uint32_t kind, float w, float h) {
+    if (kind == PFXH_PROJ_ORTHOGRAPHIC) return std::isfinite(w) && std::isfinite(h) && w > 0.0f && h > 0.0f;
+    return kind == PFXH_PROJ_PERSPECTIVE || kind == PFXH_PROJ_EQUIRECT;
+}
+bool projection_fits(uint32_t kind, const float* forward, const float* right, const float* up) {
+    if (kind == PFXH_PROJ_PERSPECTIVE) return true;
+    for (const float* a : {forward, right, up})
+        if (!(a[0] * a[0] + a[1] * a[1] + a[2] * a[2] > 1e-12f)) return false;
+    return true;
+}
+// Projection::place (gpu.rs:210-217)
+void place_projection(SceneDev& S, uint32_t kind, float w, float h) {
+    S.projection = kind;
+    S.half_view[0] = kind == PFXH_PROJ_ORTHOGRAPHIC ? w * 0.5f : 0.0f;
+    S.half_view[1] = kind == PFXH_PROJ_ORTHOGRAPHIC ? h * 0.5f : 0.0f;
+}
+
 void setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& detail, uint32_t width, uint32_t height) {
     std::string e = build_scene(scene, t->host);
     if (!e.empty()) throw DeviceError{PFXH_SETUP, e};
@@ -173,6 +191,7 @@ void setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& detail, ui
     S.lens[1] = detail.lens_shift[1];
     S.lens[2] = detail.aperture * 0.5f;
     S.lens[3] = detail.focus_distance;
+    place_projection(S, detail.projection, detail.ortho_size[0], detail.ortho_size[1]);
     S.window[0] = width;
     S.window[1] = height;
     S.window[2] = 0;
@@ -330,6 +349,7 @@ pfxh_status guarded(char* err, size_t err_len, F&& f) {
 
 bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
 
+
 }  // namespace
 
 extern "C" {
@@ -351,6 +371,14 @@ pfxh_status pfxh_trace_new(const pfxh_scene* scene, const pfxh_detail* detail, u
     if ((uint64_t)width * height > 0xffffffffull) {
         put_err(err, err_len, "the frame is too large");
         return PFXH_REFUSED;
+    }
+    if (!projection_valid(detail->projection, detail->ortho_size[0], detail->ortho_size[1])) {
+        put_err(err, err_len, "invalid trace detail");
+        return PFXH_SETUP;
+    }
+    if (!projection_fits(detail->projection, scene->camera.forward, scene->camera.right, scene->camera.up)) {
+        put_err(err, err_len, "an orthographic or equirect camera needs forward, right and up");
+        return PFXH_SETUP;
     }
     pfxh_trace* t = new pfxh_trace();
     pfxh_status st = guarded(err, err_len, [&] { setup(t, *scene, *detail, width, height); });
@@ -445,6 +473,10 @@ pfxh_status pfxh_trace_set_camera(pfxh_trace* t, const pfxh_camera* c, char* err
         put_err(err, err_len, "camera contains invalid parameters");
         return PFXH_INVALID;
     }
+    if (!projection_fits(t->S.projection, c->forward, c->right, c->up)) {
+        put_err(err, err_len, "an orthographic or equirect camera needs forward, right and up");
+        return PFXH_INVALID;
+    }
     for (int k = 0; k < 3; ++k) {
         t->S.origin[k] = c->origin[k];
         t->S.forward[k] = c->forward[k];
@@ -508,6 +540,106 @@ pfxh_status pfxh_trace_set_lens(pfxh_trace* t, float sx, float sy, float apertur
     t->S.lens[2] = aperture * 0.5f;
     t->S.lens[3] = focus;
     t->samples = 0;
+    return PFXH_OK;
+}
+
+pfxh_status pfxh_trace_set_projection(pfxh_trace* t, uint32_t kind, float w, float h, char* err, size_t err_len) {
+    if (!t) return PFXH_INVALID;
+    if (!projection_valid(kind, w, h)) {
+        put_err(err, err_len, "invalid projection");
+        return PFXH_INVALID;
+    }
+    if (!projection_fits(kind, t->S.forward, t->S.right, t->S.up)) {
+        put_err(err, err_len, "an orthographic or equirect camera needs forward, right and up");
+        return PFXH_INVALID;
+    }
+    place_projection(t->S, kind, w, h);
+    t->samples = 0;
+    return PFXH_OK;
+}
+
+pfxh_status pfxh_trace_save(pfxh_trace* t, const char* key, uint8_t* out, size_t cap, size_t* len, char* err,
+                            size_t err_len) {
+    if (!t || !key || !len) return PFXH_INVALID;
+    size_t pixels = (size_t)t->width * t->height;
+    // sizing needs no device work
+    std::vector<uint8_t> none(1);
+    pfxh_checkpoint c{};
+    c.width = t->width;
+    c.height = t->height;
+    c.samples = t->samples;
+    c.round_count = PFXH_NO_ROUNDS;
+    c.accum = c.color = c.albedo = c.normal = none.data();
+    if (!out) return pfxh_checkpoint_write(&c, key, nullptr, 0, len, err, err_len);
+    std::vector<float> accum(pixels * 12), images(pixels * 12);
+    pfxh_status st = guarded(err, err_len, [&] {
+        if (!pixels) return;
+        HIP_TRY(hipMemcpyAsync(accum.data(), t->A.sum, pixels * 48, hipMemcpyDeviceToHost, t->stream));
+        std::vector<float> comp;
+        std::vector<double> wide;
+        if (t->A.mode == PFXH_ACCUM_COMPENSATED) {
+            comp.resize(pixels * 12);
+            HIP_TRY(hipMemcpyAsync(comp.data(), t->A.comp, pixels * 48, hipMemcpyDeviceToHost, t->stream));
+        } else if (t->A.mode == 2u) {
+            wide.resize(pixels * 12);
+            HIP_TRY(hipMemcpyAsync(wide.data(), t->A.sum64, pixels * 96, hipMemcpyDeviceToHost, t->stream));
+        }
+        float4* ci = t->images;
+        launch_develop(t->stream, t->A, (uint32_t)pixels, ci, ci + pixels, ci + 2 * pixels);
+        HIP_TRY(hipMemcpyAsync(images.data(), ci, pixels * 48, hipMemcpyDeviceToHost, t->stream));
+        HIP_TRY(hipStreamSynchronize(t->stream));
+        HIP_TRY(hipGetLastError());
+        // fold the compensation (or the FP64 sums) into the FP32 sums; the
+        // sample count (color.w) is stored as is
+        for (size_t i = 0; i < pixels * 12; ++i) {
+            if (i % 12 == 3) continue;
+            if (!comp.empty()) accum[i] = accum[i] + comp[i];
+            if (!wide.empty()) accum[i] = (float)wide[i];
+        }
+    });
+    if (st != PFXH_OK) return st;
+    c.accum = (const uint8_t*)accum.data();
+    c.color = (const uint8_t*)images.data();
+    c.albedo = c.color + pixels * 16;
+    c.normal = c.color + pixels * 32;
+    return pfxh_checkpoint_write(&c, key, out, cap, len, err, err_len);
+}
+
+pfxh_status pfxh_trace_restore(pfxh_trace* t, const char* key, const uint8_t* bytes, size_t len, char* err,
+                               size_t err_len) {
+    if (!t || !key || !bytes) return PFXH_INVALID;
+    pfxh_checkpoint c{};
+    pfxh_status st = pfxh_checkpoint_read(bytes, len, key, &c, nullptr, nullptr, 0, err, err_len);
+    if (st != PFXH_OK) return st;
+    if (c.width != t->width || c.height != t->height) {
+        put_err(err, err_len, "the checkpoint is " + std::to_string(c.width) + "x" + std::to_string(c.height) +
+                                  " and the trace " + std::to_string(t->width) + "x" + std::to_string(t->height));
+        return PFXH_INVALID;
+    }
+    size_t pixels = (size_t)t->width * t->height;
+    st = guarded(err, err_len, [&] {
+        if (!pixels) return;
+        HIP_TRY(hipMemcpy(t->A.sum, c.accum, pixels * 48, hipMemcpyHostToDevice));
+        if (t->A.mode == PFXH_ACCUM_COMPENSATED) HIP_TRY(hipMemset(t->A.comp, 0, pixels * 48));
+        if (t->A.mode == 2u) {
+            std::vector<double> wide(pixels * 12);
+            for (size_t i = 0; i < wide.size(); ++i) {
+                float v;
+                memcpy(&v, c.accum + i * 4, 4);
+                wide[i] = (double)v;
+            }
+            HIP_TRY(hipMemcpy(t->A.sum64, wide.data(), pixels * 96, hipMemcpyHostToDevice));
+        }
+    });
+    if (st != PFXH_OK) return st;
+    t->samples = c.samples;
+    // a negative sample count marks a pixel frozen by adaptive sampling
+    t->any_frozen = false;
+    for (size_t p = 0; p < pixels && !t->any_frozen; ++p) {
+        float w;
+        memcpy(&w, c.accum + p * 48 + 12, 4);
+        t->any_frozen = w < 0.0f;
+    }
     return PFXH_OK;
 }
 

@@ -78,6 +78,9 @@ class Bounces:
 ACCUM_COMPENSATED = N.ACCUM_COMPENSATED
 ACCUM_PLAIN = N.ACCUM_PLAIN
 ACCUM_FP64 = N.ACCUM_FP64
+PERSPECTIVE = N.PROJ_PERSPECTIVE
+ORTHOGRAPHIC = N.PROJ_ORTHOGRAPHIC
+EQUIRECT = N.PROJ_EQUIRECT
 
 
 @dataclass
@@ -93,6 +96,8 @@ class Detail:
     sun_radius_deg: float = 0.0
     accumulate: int = ACCUM_COMPENSATED
     wave_paths: int = 0
+    projection: int = PERSPECTIVE  # gpu::Projection
+    ortho_size: tuple = (0.0, 0.0)  # orthographic view width, height
 
     def to_c(self) -> N.DetailC:
         d = N.DetailC()
@@ -103,6 +108,8 @@ class Detail:
         d.sun_radius_deg = float(self.sun_radius_deg)
         d.accumulate = int(self.accumulate)
         d.wave_paths = int(self.wave_paths)
+        d.projection = int(self.projection)
+        d.ortho_size[:] = [float(v) for v in self.ortho_size]
         return d
 
 

@@ -160,6 +160,30 @@ class Trace:
         st = N.lib().pfxh_trace_set_lens(self._handle(), shift[0], shift[1], aperture, focus_distance, err, len(err))
         self._check(st, err)
 
+    def set_projection(self, kind: int, width: float = 0.0, height: float = 0.0) -> None:
+        """Trace::set_projection (gpu.rs:2872-2890): PERSPECTIVE, ORTHOGRAPHIC
+        (view width/height) or EQUIRECT; resets the sample count."""
+        err = N.err_buffer()
+        st = N.lib().pfxh_trace_set_projection(self._handle(), int(kind), float(width), float(height), err, len(err))
+        self._check(st, err)
+
+    def save(self, key: str) -> bytes:
+        """Trace::save + Saved::write_to (checkpoint.rs:85-110, 163-174): the
+        PFXCKPT1 bytes of this trace's accumulator, images and sample count."""
+        err = N.err_buffer()
+        length = C.c_size_t()
+        k = key.encode()
+        self._check(N.lib().pfxh_trace_save(self._handle(), k, None, 0, C.byref(length), err, len(err)), err)
+        out = C.create_string_buffer(length.value)
+        self._check(N.lib().pfxh_trace_save(self._handle(), k, out, length.value, C.byref(length), err, len(err)), err)
+        return out.raw[: length.value]
+
+    def restore(self, key: str, data: bytes) -> None:
+        """Saved::read_from + Trace::restore (checkpoint.rs:112-200)."""
+        err = N.err_buffer()
+        buf = C.create_string_buffer(data, len(data))
+        self._check(N.lib().pfxh_trace_restore(self._handle(), key.encode(), buf, len(data), err, len(err)), err)
+
     def samples(self) -> int:
         return int(N.lib().pfxh_trace_samples(self._handle()))
 

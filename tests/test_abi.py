@@ -34,7 +34,7 @@ def test_struct_sizes_match_the_header():
     assert C.sizeof(N.Camera) == 48
     assert C.sizeof(N.Sun) == 28
     assert C.sizeof(N.Environment) == 16
-    assert C.sizeof(N.DetailC) == 44
+    assert C.sizeof(N.DetailC) == 56
     assert C.sizeof(N.AdaptiveC) == 16
     assert C.sizeof(N.ConvergedC) == 4 + 4 * 64 + 4 + 8 * 64 + 8
     assert C.sizeof(N.SceneC) == 7 * 16 + 16 + 48 + 28 + 4
@@ -55,3 +55,26 @@ def test_invalid_inputs_are_refused_without_a_device(native_lib):
     assert native_lib.pfxh_trace_new(None, None, 4, 4, C.byref(out), err, 256) == N.INVALID
     assert native_lib.pfxh_trace_samples(None) == 0
     native_lib.pfxh_trace_free(None)
+
+
+def test_invalid_projection_is_refused_before_any_device_work(native_lib):
+    """Projection::valid / fits (gpu.rs:191-208): an orthographic view needs a
+    positive finite size, and orthographic or equirect cameras need nonzero
+    forward, right and up."""
+    from pfx_amd import EQUIRECT, ORTHOGRAPHIC, Detail, scenes
+
+    scene = scenes.box_spheres(width=8, height=8)
+    err = C.create_string_buffer(256)
+    out = C.c_void_p()
+    sc, keep = scene.to_c()
+    bad = Detail(projection=ORTHOGRAPHIC, ortho_size=(0.0, 1.0)).to_c()
+    assert native_lib.pfxh_trace_new(C.byref(sc), C.byref(bad), 8, 8, C.byref(out), err, 256) == N.SETUP
+    assert b"invalid trace detail" in err.value
+    unknown = Detail(projection=7).to_c()
+    assert native_lib.pfxh_trace_new(C.byref(sc), C.byref(unknown), 8, 8, C.byref(out), err, 256) == N.SETUP
+    scene.camera.up = (0.0, 0.0, 0.0)
+    sc, keep = scene.to_c()
+    flat = Detail(projection=EQUIRECT).to_c()
+    assert native_lib.pfxh_trace_new(C.byref(sc), C.byref(flat), 8, 8, C.byref(out), err, 256) == N.SETUP
+    assert b"needs forward, right and up" in err.value
+    assert not out.value

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from oracle import Oracle
-from pfx_amd import Bounces, Detail, Trace, scenes
+from pfx_amd import EQUIRECT, ORTHOGRAPHIC, PERSPECTIVE, Bounces, Detail, Trace, scenes
 from pfx_amd.scene import ACCUM_COMPENSATED, ACCUM_PLAIN
 
 pytestmark = pytest.mark.gpu
@@ -131,3 +131,77 @@ def test_adaptive_rounds_match_oracle():
     assert [(r.samples, r.active) for r in conv.rounds] == ref_rounds
     assert conv.pixels == pixels
     assert rel_mse(got[..., :3], rc[..., :3]) <= 1e-4
+
+
+PROJECTIONS = {"orthographic": (ORTHOGRAPHIC, (0.9, 0.9)), "equirect": (EQUIRECT, (0.0, 0.0))}
+
+
+@pytest.mark.parametrize("name", sorted(PROJECTIONS))
+def test_projections_match_oracle(name):
+    """Orthographic and equirect cameras (trace.wgsl:2546-2553, probe.wgsl:18-34):
+    probe ids and distances bit-exact, images at equal spp as for perspective,
+    and set_projection on a perspective trace equals a trace built with it."""
+    kind, size = PROJECTIONS[name]
+    w, h = 64, 48
+    scene = scenes.box_spheres(width=w, height=h)
+    detail = Detail(bounces=Bounces.deep(4), accumulate=ACCUM_PLAIN, projection=kind, ortho_size=size)
+    print(f"projection parity {name} {w}x{h}", flush=True)
+    o = Oracle(scene, detail, w, h)
+    ref = o.probe()
+    o.sample(4, 5)
+    rc, _, _ = o.readback()
+    with Trace.new_detailed(scene, detail, w, h) as t:
+        got = t.probe()
+        t.sample(4, 5)
+        color = t.readback().color
+    assert np.array_equal(got.primitive, ref["primitive"])
+    np.testing.assert_array_equal(got.distance, ref["distance"])
+    np.testing.assert_array_equal(got.depth, ref["depth"])
+    assert rel_mse(color[..., :3], rc[..., :3]) <= 1e-4
+    assert float(np.mean(color == rc)) > 0.99
+    with Trace.new_detailed(scene, Detail(bounces=Bounces.deep(4), accumulate=ACCUM_PLAIN), w, h) as t:
+        t.sample(1, 1)
+        t.set_projection(kind, *size)
+        assert t.samples() == 0
+        t.sample(4, 5)
+        assert np.array_equal(t.readback().color, color)
+        t.set_projection(PERSPECTIVE)
+        assert t.samples() == 0
+
+
+@pytest.mark.parametrize("accumulate", [ACCUM_PLAIN, ACCUM_COMPENSATED])
+def test_checkpoint_resume_continues_the_render(accumulate):
+    """Trace::save / Trace::restore (checkpoint.rs:163-200): 4 samples, save,
+    restore into a fresh trace, 4 more samples equals 8 uninterrupted ones
+    (byte for byte with the plain sum; the compensated sum restarts its
+    compensation from the saved FP32 sum)."""
+    w, h = 48, 40
+    scene = scenes.box_spheres(width=w, height=h)
+    detail = Detail(bounces=Bounces.deep(4), accumulate=accumulate)
+    print(f"checkpoint resume accumulate={accumulate}", flush=True)
+    with Trace.new_detailed(scene, detail, w, h) as t:
+        t.sample(4, 3)
+        data = t.save("still")
+        first = t.readback()
+        t.sample(4, 3)
+        full = t.readback()
+    from pfx_amd.checkpoint import Saved
+
+    s = Saved.read_from("still", data)
+    assert s.size == (w, h) and s.samples == 4
+    assert s.color == first.color.tobytes() and s.normal == first.normal.tobytes()
+    with Trace.new_detailed(scene, detail, w, h) as t:
+        t.restore("still", data)
+        assert t.samples() == 4
+        assert np.array_equal(t.readback().color, first.color) or accumulate != ACCUM_PLAIN
+        t.sample(4, 3)
+        resumed = t.readback()
+    if accumulate == ACCUM_PLAIN:
+        assert np.array_equal(resumed.color, full.color)
+    else:
+        assert rel_mse(resumed.color[..., :3], full.color[..., :3]) <= 1e-10
+    with Trace.new_detailed(scene, detail, 40, 40) as t:
+        with pytest.raises(RuntimeError, match="the checkpoint is 48x40"):
+            t.restore("still", data)
+        with pytest.raises(RuntimeError, match="another render"):
+            t.restore("other", data)
