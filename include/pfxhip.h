@@ -164,6 +164,8 @@ typedef struct pfxh_detail {
     uint32_t wave_paths; /* paths in flight per wave; 0 picks a default */
     uint32_t projection; /* PFXH_PROJ_*; Detail::projection */
     float ortho_size[2]; /* orthographic view width and height (PFXH_PROJ_ORTHOGRAPHIC) */
+    uint32_t no_emitter_sampling; /* 1: emissive triangles are not light-sampled, as
+                                   * Trace::build with emitter_sampling = false (gpu.rs:781) */
 } pfxh_detail;
 
 /* adaptive::Adaptive (crates/trace/src/adaptive.rs:9-26) */

@@ -1117,7 +1117,7 @@ static std::string prepare(Scene& sc, const pfxh_scene& in, const pfxh_detail& d
         v3 e1 = t3(t.b) - t3(t.a), e2 = t3(t.c) - t3(t.a);
         v3 cr = pfx::cross(e1, e2);
         float area = 0.5f * sqrtf(dot(cr, cr));
-        if (emits(pm.emission) && area > 1e-12f) {
+        if (detail.no_emitter_sampling == 0u && emits(pm.emission) && area > 1e-12f) {  // gpu.rs:781
             EmitterCandidate c;
             memcpy(c.vertices, verts, sizeof verts);
             c.record = 0;
@@ -1127,7 +1127,7 @@ static std::string prepare(Scene& sc, const pfxh_scene& in, const pfxh_detail& d
         }
     }
     for (size_t slot = world_slots; slot < sc.laid.tris.size(); ++slot)
-        if (emits(in.materials[(uint32_t)sc.laid.tris[slot].a[3]].emission))
+        if (detail.no_emitter_sampling == 0u && emits(in.materials[(uint32_t)sc.laid.tris[slot].a[3]].emission))
             return "emissive instanced meshes are not supported";
     sc.lights = tree_lay(in.lights, in.light_count);
     sc.emitters = build_emitters(cands, sc.lights.looped, sc.lights.tree);

@@ -129,6 +129,7 @@ class DetailC(C.Structure):
         ("wave_paths", C.c_uint32),
         ("projection", C.c_uint32),
         ("ortho_size", C.c_float * 2),
+        ("no_emitter_sampling", C.c_uint32),
     ]
 
 

@@ -172,8 +172,12 @@ PFX_HD float fm_log2(float x) {
 }
 
 // pow for the non-negative bases the shading code passes (x^y = 2^(y log2 x))
+// x^y for x >= 0. WGSL's pow leaves a negative base undefined; shading
+// reaches one only through rounding (1 - cos with cos a hair above 1), and
+// such a base is taken as 0 so the result stays finite.
 PFX_HD float fm_pow(float x, float y) {
     if (y == 0.0f) return 1.0f;
+    if (x < 0.0f) x = 0.0f;
     if (x == 0.0f) return y > 0.0f ? 0.0f : u2f(0x7f800000u);
     if (x == 1.0f) return 1.0f;
     return fm_exp2(y * fm_log2(x));

@@ -992,7 +992,7 @@ void build_emitters(const std::vector<float>& tris12, const std::vector<uint32_t
     if (n) cdf[n - 1] = 1.0f;
 }
 
-std::string build_scene(const pfxh_scene& sc, HostScene& out) {
+std::string build_scene(const pfxh_scene& sc, HostScene& out, bool emitter_sampling) {
     if (sc.material_count == 0) return "invalid trace scene detail";
     for (uint32_t i = 0; i < sc.light_count; ++i)
         if (!light_valid(sc.lights[i])) return "invalid trace light";
@@ -1049,7 +1049,8 @@ std::string build_scene(const pfxh_scene& sc, HostScene& out) {
     for (auto& m : laid.meshes) {
         std::vector<Tri9> ts = laid.mesh_tris(sc, m.source);
         for (uint32_t j : m.bvh.order) {
-            if (out.materials[ts[j].material].emits) return "emissive instanced meshes are not supported";
+            if (emitter_sampling && out.materials[ts[j].material].emits)
+                return "emissive instanced meshes are not supported";
             put(ts[j], mesh_base + sc.meshes[m.source].first + j);
         }
     }
@@ -1059,7 +1060,7 @@ std::string build_scene(const pfxh_scene& sc, HostScene& out) {
     std::vector<uint32_t> cand_rec;
     for (uint32_t slot = 0; slot < laid.world.size(); ++slot) {
         const Tri9& t = laid.world[laid.world_bvh.order[slot]];
-        if (out.materials[t.material].emits && tri_area(t.v) > 1e-12f) {
+        if (emitter_sampling && out.materials[t.material].emits && tri_area(t.v) > 1e-12f) {
             cand.insert(cand.end(), t.v, t.v + 9);
             cand.insert(cand.end(), sc.materials[t.material].emission, sc.materials[t.material].emission + 3);
             cand_rec.push_back(0);

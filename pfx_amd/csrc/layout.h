@@ -54,7 +54,9 @@ struct HostScene {
     bool bounds_tame = true;  // every node box bound is 0 or >= 2^-72 in magnitude
 };
 
-std::string build_scene(const pfxh_scene& scene, HostScene& out);
+// `emitter_sampling` false leaves emissive triangles unmarked, so they are
+// reached only by BSDF sampling (gpu.rs:781).
+std::string build_scene(const pfxh_scene& scene, HostScene& out, bool emitter_sampling = true);
 std::string lay_levels(const pfxh_scene& scene, RefLevels& out, std::vector<uint32_t>* slot_prim);
 
 // tree::lay (crates/trace/src/tree.rs:220-243)

@@ -136,7 +136,7 @@ void place_projection(SceneDev& S, uint32_t kind, float w, float h) {
 }
 
 void setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& detail, uint32_t width, uint32_t height) {
-    std::string e = build_scene(scene, t->host);
+    std::string e = build_scene(scene, t->host, detail.no_emitter_sampling == 0u);
     if (!e.empty()) throw DeviceError{PFXH_SETUP, e};
     const HostScene& h = t->host;
     HIP_TRY(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));

@@ -98,6 +98,7 @@ class Detail:
     wave_paths: int = 0
     projection: int = PERSPECTIVE  # gpu::Projection
     ortho_size: tuple = (0.0, 0.0)  # orthographic view width, height
+    emitter_sampling: bool = True  # False: emissive triangles only through BSDF sampling (gpu.rs:781)
 
     def to_c(self) -> N.DetailC:
         d = N.DetailC()
@@ -110,6 +111,7 @@ class Detail:
         d.wave_paths = int(self.wave_paths)
         d.projection = int(self.projection)
         d.ortho_size[:] = [float(v) for v in self.ortho_size]
+        d.no_emitter_sampling = 0 if self.emitter_sampling else 1
         return d
 
 

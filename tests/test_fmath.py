@@ -130,3 +130,12 @@ def test_spawn_point_half_millimetre_gap():
     O.lib().or_spawn_point(p.ctypes.data, d.ctypes.data, np.float32([0, 2600, 0]).ctypes.data, 2600.0,
                            out.ctypes.data)
     assert out[1] + np.float32(0.05) > 0.001
+
+
+def test_pow_of_a_rounding_negative_base_is_finite():
+    """1 - cos with cos a hair above 1 (a normalised vector's z rounding past
+    1) is a tiny negative base: pow takes it as 0 instead of NaN, for the
+    integer and non-integer exponents shading uses."""
+    for y in (4.0, 5.0, 2.5):
+        x = np.float32([-1.1920929e-07, -0.0, 0.0])
+        assert np.array_equal(run(POW, x, np.full_like(x, y)), np.zeros(3, np.float32))

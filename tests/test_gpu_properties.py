@@ -1,0 +1,65 @@
+"""Size-independent properties at the full terrain configuration (3840x2160,
+1,048,576 heightfield triangles, 256 instances of 8 meshes, 128 point/spot
+lights, 6 bounces), where the oracle is too slow to compare pixel by pixel:
+every output is finite after a full adaptive render, the adaptive rounds
+follow adaptive.rs's schedule, four 1920x1080 windows equal the full frame
+byte for byte (docs/trace.md:539-541), split sampling equals one call, and
+reruns are identical."""
+import numpy as np
+import pytest
+
+from pfx_amd import Adaptive, Bounces, Detail, Trace, scenes
+from pfx_amd.scene import ACCUM_PLAIN
+
+pytestmark = pytest.mark.gpu
+
+W, H = 3840, 2160
+
+
+@pytest.fixture(scope="module")
+def terrain():
+    return scenes.terrain(width=W, height=H)
+
+
+def test_adaptive_render_is_finite_and_follows_the_rounds(terrain):
+    print("4K adaptive <=64 spp", flush=True)
+    with Trace.new_detailed(terrain, Detail(bounces=Bounces.deep(6)), W, H) as t:
+        conv = t.sample_adaptive(Adaptive(threshold=0.01, min_samples=16, max_samples=64, growth=1.5), 1)
+        out = t.readback()
+    assert [r.samples for r in conv.rounds] == [8, 16, 24, 36, 54, 64]  # adaptive.rs:46-63
+    active = [r.active for r in conv.rounds]
+    assert active[0] == W * H and all(a >= b for a, b in zip(active, active[1:]))
+    for img in (out.color, out.albedo, out.normal):
+        assert np.isfinite(img).all()
+    assert (out.color[..., :3] >= 0.0).all()
+
+
+def test_windows_tile_the_full_frame(terrain):
+    detail = Detail(bounces=Bounces.deep(6), accumulate=ACCUM_PLAIN)
+    print("4K full frame vs four windows, 4 spp", flush=True)
+    with Trace.new_detailed(terrain, detail, W, H) as t:
+        t.sample(4, 9)
+        full = t.readback().color
+    hw, hh = W // 2, H // 2
+    with Trace.new_detailed(terrain, detail, hw, hh) as t:
+        for x0, y0 in ((0, 0), (hw, 0), (0, hh), (hw, hh)):
+            t.set_window((W, H), (x0, y0))
+            t.sample(4, 9)
+            assert np.array_equal(t.readback().color, full[y0:y0 + hh, x0:x0 + hw])
+
+
+def test_split_sampling_and_reruns_are_identical(terrain):
+    detail = Detail(bounces=Bounces.deep(6), accumulate=ACCUM_PLAIN)
+    print("4K sample(2)+sample(2) vs sample(4), and a rerun", flush=True)
+    with Trace.new_detailed(terrain, detail, W, H) as t:
+        t.sample(2, 5)
+        t.sample(2, 5)
+        split = t.readback().color
+        t.set_camera(terrain.camera)
+        t.sample(4, 5)
+        whole = t.readback().color
+    with Trace.new_detailed(terrain, detail, W, H) as t:
+        t.sample(4, 5)
+        again = t.readback().color
+    assert np.array_equal(split, whole)
+    assert np.array_equal(whole, again)
