@@ -343,6 +343,51 @@ pfxh_status pfxh_trace_save(pfxh_trace* trace, const char* key, uint8_t* out, si
 pfxh_status pfxh_trace_restore(pfxh_trace* trace, const char* key, const uint8_t* bytes, size_t len,
                                char* err, size_t err_len);
 
+/* ---- the repo denoiser (crates/denoise: OIDN 2.5.1's RT filter) ------------ */
+
+typedef struct pfxh_denoiser pfxh_denoiser;
+
+/* tile::Tile (crates/denoise/src/tile.rs:4-12), [y, x] pairs */
+typedef struct pfxh_denoise_tile {
+    uint32_t src[2];
+    uint32_t dst[2];
+    uint32_t size[2];
+    uint32_t out_src[2];
+    uint32_t out_dst[2];
+    uint32_t out_size[2];
+} pfxh_denoise_tile;
+
+/* Denoiser::new (crates/denoise/src/lib.rs:82-112): parse OIDN's RT weights
+ * (the TZA blob pfx_amd/weights/rt_hdr_alb_nrm.tza, tza.rs:50-111), build
+ * the U-Net (net.rs:202-232) and upload it. */
+pfxh_status pfxh_denoiser_new(const uint8_t* weights, size_t len, pfxh_denoiser** out, char* err,
+                              size_t err_len);
+/* Denoiser::denoise (lib.rs:121-144): RGBA32F colour, albedo and normal
+ * (row-major, width*height*4 floats each) in, the denoised RGBA32F image
+ * out (alpha copied from the colour). */
+pfxh_status pfxh_denoise(pfxh_denoiser* denoiser, uint32_t width, uint32_t height, const float* color,
+                         const float* albedo, const float* normal, float* out, char* err,
+                         size_t err_len);
+/* Denoiser::denoise_textures on a trace's current colour, albedo and normal
+ * (still::developed, crates/still/src/trace.rs:1299-1339), without a host
+ * round trip for the inputs. `out` holds width*height*4 floats. */
+pfxh_status pfxh_trace_denoise(pfxh_trace* trace, pfxh_denoiser* denoiser, float* out, char* err,
+                               size_t err_len);
+/* Largest tile in pixels (0, the default: one tile per image; the
+ * reference uses 2160*2160, tile.rs:2). Results do not depend on it: tiles
+ * overlap by more than half the receptive field and align to 16. */
+void pfxh_denoiser_set_max_tile(pfxh_denoiser* denoiser, uint64_t pixels);
+/* Layer and parameter counts of a weights blob (host only). */
+pfxh_status pfxh_denoiser_layers(const uint8_t* weights, size_t len, uint32_t* layers,
+                                 uint64_t* params, char* err, size_t err_len);
+/* tile::plan (tile.rs:31-101) for a height x width image: the tile size and
+ * up to `cap` tiles; *count gets the tile count (host only). */
+pfxh_status pfxh_denoise_plan(uint32_t height, uint32_t width, uint64_t max_tile_pixels,
+                              uint32_t* tile_h, uint32_t* tile_w, pfxh_denoise_tile* tiles,
+                              uint32_t cap, uint32_t* count);
+/* Drop the denoiser and its device memory. NULL is allowed. */
+void pfxh_denoiser_free(pfxh_denoiser* denoiser);
+
 /* Library version string. */
 const char* pfxh_version(void);
 

@@ -33,6 +33,9 @@ def lib() -> C.CDLL:
         h = C.CDLL(str(LIB))
         sig = {
             "or_scene_new": (_P, [_P, _P, _U, _U, C.c_char_p, C.c_size_t]),
+            "or_tza_count": (C.c_int, [C.c_char_p, C.c_size_t]),
+            "or_denoise_plan": (C.c_int, [_U, _U, C.c_uint64, C.POINTER(_U), C.POINTER(_U), _P, _U]),
+            "or_denoise": (C.c_int, [C.c_char_p, C.c_size_t, _U, _U, _P, _P, _P, _P, C.c_uint64]),
             "or_scene_free": (None, [_P]),
             "or_set_camera": (None, [_P, _P]),
             "or_set_window": (C.c_int, [_P, _U, _U, _U, _U]),

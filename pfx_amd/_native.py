@@ -168,6 +168,10 @@ class StatsC(C.Structure):
 
 
 # Every symbol include/pfxhip.h declares, with its ctypes signature.
+class DenoiseTile(C.Structure):
+    _fields_ = [(name, C.c_uint32 * 2) for name in ("src", "dst", "size", "out_src", "out_dst", "out_size")]
+
+
 class CheckpointC(C.Structure):
     _fields_ = [
         ("width", C.c_uint32),
@@ -203,6 +207,13 @@ SIGNATURES = {
     "pfxh_checkpoint_write": (C.c_int, [_P, _S, _P, _Z, C.POINTER(C.c_size_t), _S, _Z]),
     "pfxh_checkpoint_read": (C.c_int, [_P, _Z, _S, _P, _P, _P, _U, _S, _Z]),
     "pfxh_trace_save": (C.c_int, [_P, _S, _P, _Z, C.POINTER(C.c_size_t), _S, _Z]),
+    "pfxh_denoiser_new": (C.c_int, [_S, _Z, C.POINTER(_P), _S, _Z]),
+    "pfxh_denoise": (C.c_int, [_P, _U, _U, _P, _P, _P, _P, _S, _Z]),
+    "pfxh_trace_denoise": (C.c_int, [_P, _P, _P, _S, _Z]),
+    "pfxh_denoiser_set_max_tile": (None, [_P, C.c_uint64]),
+    "pfxh_denoiser_layers": (C.c_int, [_S, _Z, C.POINTER(_U), C.POINTER(C.c_uint64), _S, _Z]),
+    "pfxh_denoise_plan": (C.c_int, [_U, _U, C.c_uint64, C.POINTER(_U), C.POINTER(_U), _P, _U, C.POINTER(_U)]),
+    "pfxh_denoiser_free": (None, [_P]),
     "pfxh_trace_restore": (C.c_int, [_P, _S, _P, _Z, _S, _Z]),
     "pfxh_trace_samples": (_U, [_P]),
     "pfxh_trace_gpu_bytes": (C.c_uint64, [_P]),

@@ -1,0 +1,250 @@
+// Kernels of the repo denoiser (OIDN 2.5.1 RT filter, crates/denoise) for
+// gfx950.
+//
+//   k_conv       3x3 convolution as an implicit GEMM on FP32 matrix cores
+//                (v_mfma_f32_16x16x4f32): a 256-thread block computes 4 rows
+//                of 16 output pixels for every output channel; each wave owns
+//                one row (M = 16 pixels) and all N tiles of 16 channels. The
+//                K loop walks 16-channel chunks of the input (the upsampled
+//                coarser tensor first, then the skip tensor) staged with a
+//                1-pixel halo in LDS, and the nine taps. Bias, ReLU and the
+//                2x2 max pool are fused into the epilogue; nearest
+//                upsampling and concatenation are fused into the staging.
+//   k_bins / k_scale   autoexposure (process.wgsl:104-157)
+//   k_input / k_output PU transfer in and out of the network
+//                      (process.wgsl:33-102)
+#include <hip/hip_runtime.h>
+
+#include "denoise.h"
+
+namespace pfxd {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int TW = 16, TH = 4, PW = TW + 2, PH = TH + 2;
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv(ConvDev c) {
+    __shared__ float4 patch[PH * PW * 4];    // [row][col][16 channels]
+    __shared__ float pooled[2][NT][64][2];   // odd rows' x-pooled values
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 15, kk = lane >> 4;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const int h = (int)c.h, w = (int)c.w_;
+    const int sw = c.upsample ? w / 2 : w;
+    v4f acc[NT];
+    for (int t = 0; t < NT; ++t) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    const uint32_t chunks = (c.c0 + c.c1) / 16u;
+    for (uint32_t ch = 0; ch < chunks; ++ch) {
+        __syncthreads();
+        for (int e = tid; e < PH * PW * 4; e += 256) {
+            int q = e & 3, p = e >> 2;
+            int py = p / PW, px = p - py * PW;
+            int y = y0 + py - 1, x = x0 + px - 1;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (y >= 0 && y < h && x >= 0 && x < w) {
+                uint32_t cc = ch * 16u + (uint32_t)q * 4u;
+                if (cc < c.c0) {
+                    int yy = c.upsample ? y >> 1 : y, xx = c.upsample ? x >> 1 : x;
+                    v = *reinterpret_cast<const float4*>(c.src0 + ((size_t)yy * sw + xx) * c.c0 + cc);
+                } else {
+                    v = *reinterpret_cast<const float4*>(c.src1 + ((size_t)y * w + x) * c.c1 + (cc - c.c0));
+                }
+            }
+            patch[e] = v;
+        }
+        __syncthreads();
+        const float* wb = c.w + (size_t)ch * 9u * c.n_pad * 16u;
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap - ky * 3;
+            const float4 a = patch[((wave + ky) * PW + i + kx) * 4 + kk];
+            const float* wt = wb + (size_t)tap * c.n_pad * 16u;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const float4 b = *reinterpret_cast<const float4*>(wt + (size_t)(16 * t + i) * 16u + 4 * kk);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b.z, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    // D[pixel 4*kk + v][channel i] of each N tile; bias, ReLU
+    const int y = y0 + wave;
+    if (!c.pool) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int n = 16 * t + i;
+            const float bias = c.bias[n];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int x = x0 + 4 * kk + v;
+                if (y < h && x < w) c.dst[((size_t)y * w + x) * c.n_pad + n] = fmaxf(acc[t][v] + bias, 0.0f);
+            }
+        }
+        return;
+    }
+    // 2x2 max pool: x pairs sit in one lane, row pairs in waves (0,1), (2,3)
+    float m[NT][2];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const float bias = c.bias[16 * t + i];
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+            m[t][p] = fmaxf(fmaxf(acc[t][2 * p] + bias, acc[t][2 * p + 1] + bias), 0.0f);
+    }
+    if (wave & 1)
+        for (int t = 0; t < NT; ++t)
+            for (int p = 0; p < 2; ++p) pooled[wave >> 1][t][lane][p] = m[t][p];
+    __syncthreads();
+    if (wave & 1) return;
+    const int oy = (y0 + wave) / 2, ho = h / 2, wo = w / 2;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int ox = x0 / 2 + 2 * kk + p;
+            const float v = fmaxf(m[t][p], pooled[wave >> 1][t][lane][p]);
+            if (oy < ho && ox < wo) c.dst[((size_t)oy * wo + ox) * c.n_pad + 16 * t + i] = v;
+        }
+}
+
+void launch_conv(hipStream_t s, const ConvDev& c) {
+    dim3 grid((c.w_ + TW - 1) / TW, (c.h + TH - 1) / TH);
+    switch (c.n_pad / 16u) {
+        case 1: hipLaunchKernelGGL(k_conv<1>, grid, dim3(256), 0, s, c); break;
+        case 2: hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), 0, s, c); break;
+        case 3: hipLaunchKernelGGL(k_conv<3>, grid, dim3(256), 0, s, c); break;
+        case 4: hipLaunchKernelGGL(k_conv<4>, grid, dim3(256), 0, s, c); break;
+        case 5: hipLaunchKernelGGL(k_conv<5>, grid, dim3(256), 0, s, c); break;
+        case 6: hipLaunchKernelGGL(k_conv<6>, grid, dim3(256), 0, s, c); break;
+        case 7: hipLaunchKernelGGL(k_conv<7>, grid, dim3(256), 0, s, c); break;
+        default: break;  // the host refuses other widths
+    }
+}
+
+// ---- autoexposure (process.wgsl:104-157) --------------------------------------
+constexpr float FLT_MAX_ = 3.4028234663852886e38f;
+__device__ __forceinline__ float nan_to_zero(float v) { return v != v ? 0.0f : v; }
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+
+__global__ __launch_bounds__(256) void k_bins(const float4* color, uint32_t w, uint32_t h, float* bins) {
+    __shared__ float sums[256];
+    const uint32_t nx = gridDim.x, ny = gridDim.y;
+    const uint32_t h0 = blockIdx.y * h / ny, h1 = (blockIdx.y + 1) * h / ny;
+    const uint32_t w0 = blockIdx.x * w / nx, w1 = (blockIdx.x + 1) * w / nx;
+    const uint32_t yy = h0 + threadIdx.x / 16u, xx = w0 + threadIdx.x % 16u;
+    float l = 0.0f;
+    if (yy < h1 && xx < w1) {
+        float4 c = color[(size_t)yy * w + xx];
+        float r = clampf(nan_to_zero(c.x), 0.0f, FLT_MAX_), g = clampf(nan_to_zero(c.y), 0.0f, FLT_MAX_),
+              b = clampf(nan_to_zero(c.z), 0.0f, FLT_MAX_);
+        l = 0.212671f * r + 0.715160f * g + 0.072169f * b;
+    }
+    sums[threadIdx.x] = l;
+    for (uint32_t k = 128; k > 0; k >>= 1) {
+        __syncthreads();
+        if (threadIdx.x < k) sums[threadIdx.x] += sums[threadIdx.x + k];
+    }
+    if (threadIdx.x == 0) bins[blockIdx.y * nx + blockIdx.x] = sums[0] / (float)((h1 - h0) * (w1 - w0));
+}
+
+__global__ __launch_bounds__(256) void k_scale(const float* bins, uint32_t count, float* scale) {
+    __shared__ float sums[256];
+    __shared__ uint32_t counts[256];
+    float sum = 0.0f;
+    uint32_t n = 0;
+    for (uint32_t i = threadIdx.x; i < count; i += 256u) {
+        float l = bins[i];
+        if (l > 1e-8f) {
+            sum += log2f(l);
+            n += 1;
+        }
+    }
+    sums[threadIdx.x] = sum;
+    counts[threadIdx.x] = n;
+    for (uint32_t k = 128; k > 0; k >>= 1) {
+        __syncthreads();
+        if (threadIdx.x < k) {
+            sums[threadIdx.x] += sums[threadIdx.x + k];
+            counts[threadIdx.x] += counts[threadIdx.x + k];
+        }
+    }
+    if (threadIdx.x == 0) scale[0] = counts[0] > 0 ? 0.18f / exp2f(sums[0] / (float)counts[0]) : 1.0f;
+}
+
+void launch_exposure(hipStream_t s, const float4* color, uint32_t w, uint32_t h, float* bins, float* scale) {
+    dim3 grid((w + 15) / 16, (h + 15) / 16);
+    hipLaunchKernelGGL(k_bins, grid, dim3(256), 0, s, color, w, h, bins);
+    hipLaunchKernelGGL(k_scale, dim3(1), dim3(256), 0, s, bins, grid.x * grid.y, scale);
+}
+
+// ---- PU transfer in and out (process.wgsl:33-102) ------------------------------
+__device__ __forceinline__ float pu_forward(float y) {
+    if (y <= 1.57945760e-06f) return 1.41283765e+03f * y;
+    if (y <= 3.22087631e-02f) return 1.64593172e+00f * powf(y, 4.31384981e-01f) + -2.94139609e-03f;
+    return 1.92653254e-01f * logf(y + 6.26026094e-03f) + 9.98620152e-01f;
+}
+__device__ __forceinline__ float pu_inverse(float x) {
+    if (x <= 2.23151711e-03f) return x / 1.41283765e+03f;
+    if (x <= 3.70974749e-01f) return powf((x - -2.94139609e-03f) / 1.64593172e+00f, 1.0f / 4.31384981e-01f);
+    return expf((x - 9.98620152e-01f) / 1.92653254e-01f) - 6.26026094e-03f;
+}
+
+// the network input: colour (PU, scaled), albedo, normal (to [0, 1]); 16 channels, 9 used
+__global__ __launch_bounds__(256) void k_input(const float4* color, const float4* albedo, const float4* normal,
+                                               const float* scale, uint32_t image_w, TileDev t, uint32_t tile_w,
+                                               uint32_t tile_h, float norm, float* tensor) {
+    uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p >= tile_w * tile_h) return;
+    int y = (int)(p / tile_w), x = (int)(p % tile_w);
+    int sx = x - t.dst[1], sy = y - t.dst[0];
+    float v[16];
+    for (int k = 0; k < 16; ++k) v[k] = 0.0f;
+    if (sx >= 0 && sx < t.size[1] && sy >= 0 && sy < t.size[0]) {
+        size_t at = (size_t)(sy + t.src[0]) * image_w + (uint32_t)(sx + t.src[1]);
+        float s = scale[0];
+        float4 c = color[at], a = albedo[at], n = normal[at];
+        float cv[3] = {c.x, c.y, c.z}, av[3] = {a.x, a.y, a.z}, nv[3] = {n.x, n.y, n.z};
+        for (int k = 0; k < 3; ++k) {
+            v[k] = pu_forward(clampf(nan_to_zero(cv[k] * s), 0.0f, FLT_MAX_)) * norm;
+            v[3 + k] = clampf(nan_to_zero(av[k]), 0.0f, 1.0f);
+            v[6 + k] = clampf(nan_to_zero(nv[k]), -1.0f, 1.0f) * 0.5f + 0.5f;
+        }
+    }
+    float4* out = reinterpret_cast<float4*>(tensor + (size_t)p * 16u);
+    for (int q = 0; q < 4; ++q) out[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+}
+
+__global__ __launch_bounds__(256) void k_output(const float* tensor, uint32_t tile_w, const float* scale,
+                                                const float4* color, uint32_t image_w, TileDev t, float rcp_norm,
+                                                float4* out) {
+    uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    uint32_t ow = (uint32_t)t.out_size[1];
+    if (p >= ow * (uint32_t)t.out_size[0]) return;
+    uint32_t y = p / ow, x = p % ow;
+    const float* v = tensor + ((size_t)(y + t.out_src[0]) * tile_w + x + t.out_src[1]) * 16u;
+    float s = scale[0];
+    float inv = s != 0.0f ? 1.0f / s : 0.0f;
+    float r[3];
+    for (int k = 0; k < 3; ++k) r[k] = pu_inverse(clampf(nan_to_zero(v[k]), 0.0f, FLT_MAX_) * rcp_norm) * inv;
+    size_t at = (size_t)(y + t.out_dst[0]) * image_w + x + t.out_dst[1];
+    out[at] = make_float4(r[0], r[1], r[2], color[at].w);
+}
+
+void launch_input(hipStream_t s, const float4* color, const float4* albedo, const float4* normal, const float* scale,
+                  uint32_t image_w, const TileDev& t, uint32_t tile_w, uint32_t tile_h, float norm, float* tensor) {
+    uint32_t n = tile_w * tile_h;
+    hipLaunchKernelGGL(k_input, dim3((n + 255) / 256), dim3(256), 0, s, color, albedo, normal, scale, image_w, t,
+                       tile_w, tile_h, norm, tensor);
+}
+
+void launch_output(hipStream_t s, const float* tensor, uint32_t tile_w, const float* scale, const float4* color,
+                   uint32_t image_w, const TileDev& t, float rcp_norm, float4* out) {
+    uint32_t n = (uint32_t)t.out_size[0] * (uint32_t)t.out_size[1];
+    if (!n) return;
+    hipLaunchKernelGGL(k_output, dim3((n + 255) / 256), dim3(256), 0, s, tensor, tile_w, scale, color, image_w, t,
+                       rcp_norm, out);
+}
+
+}  // namespace pfxd

@@ -17,6 +17,10 @@
 
 #include "../../include/pfxhip.h"
 #include "kernels.h"
+
+pfxh_status pfxh_denoise_device(pfxh_denoiser* d, uint32_t width, uint32_t height, const void* color,
+                                const void* albedo, const void* normal, float* out, hipStream_t after, char* err,
+                                size_t err_len);
 #include "layout.h"
 #include "vec3.h"
 
@@ -641,6 +645,20 @@ pfxh_status pfxh_trace_restore(pfxh_trace* t, const char* key, const uint8_t* by
         t->any_frozen = w < 0.0f;
     }
     return PFXH_OK;
+}
+
+pfxh_status pfxh_trace_denoise(pfxh_trace* t, pfxh_denoiser* d, float* out, char* err, size_t err_len) {
+    if (!t || !d || !out) return PFXH_INVALID;
+    size_t pixels = (size_t)t->width * t->height;
+    pfxh_status st = guarded(err, err_len, [&] {
+        if (!pixels) return;
+        float4* c = t->images;
+        launch_develop(t->stream, t->A, (uint32_t)pixels, c, c + pixels, c + 2 * pixels);
+        HIP_TRY(hipGetLastError());
+    });
+    if (st != PFXH_OK || !pixels) return st;
+    float4* c = t->images;
+    return pfxh_denoise_device(d, t->width, t->height, c, c + pixels, c + 2 * pixels, out, t->stream, err, err_len);
 }
 
 uint32_t pfxh_trace_samples(const pfxh_trace* t) { return t ? t->samples : 0; }

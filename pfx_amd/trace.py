@@ -184,6 +184,14 @@ class Trace:
         buf = C.create_string_buffer(data, len(data))
         self._check(N.lib().pfxh_trace_restore(self._handle(), key.encode(), buf, len(data), err, len(err)), err)
 
+    def denoise(self, denoiser) -> np.ndarray:
+        """The repo denoiser on this trace's colour, albedo and normal, on the
+        device (Denoiser::denoise_textures): the denoised (H, W, 4) image."""
+        out = np.empty((self.height, self.width, 4), np.float32)
+        err = N.err_buffer()
+        self._check(N.lib().pfxh_trace_denoise(self._handle(), denoiser._h, out.ctypes.data, err, len(err)), err)
+        return out
+
     def samples(self) -> int:
         return int(N.lib().pfxh_trace_samples(self._handle()))
 
