@@ -15,6 +15,10 @@ Prints one JSON line. Extra fields:
                 against the 8 TB/s HBM peak
   cpu_baseline  the CPU oracle (a restatement of the reference tracer, OpenMP
                 over the host cores) on a bounded strip of the same frame
+  quality       relMSE against a converged 4096-spp window of the frame at 4
+                and 64 spp, raw and through the repo denoiser, with the times
+                (the metric's "ms to relMSE <= 0.01" half; null when a
+                method does not reach it)
 """
 from __future__ import annotations
 
@@ -31,6 +35,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 from pfx_amd import Adaptive, Bounces, Detail, Trace, scenes  # noqa: E402
+from pfx_amd import denoise as denoise_mod  # noqa: E402
+from pfx_amd.scene import ACCUM_FP64  # noqa: E402
 
 WIDTH, HEIGHT = 3840, 2160
 ADAPTIVE = Adaptive(threshold=0.01, min_samples=16, max_samples=64, growth=1.5)
@@ -101,12 +107,59 @@ def cpu_baseline(scene, seconds_target: float = 15.0) -> dict:
     }
 
 
+def rel_mse(x: np.ndarray, r: np.ndarray) -> float:
+    """mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2)"""
+    x, r = x[..., :3].astype(np.float64), r[..., :3].astype(np.float64)
+    return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
+
+
+def quality_leg(scene) -> dict:
+    """Image error against a converged render, for the time-to-quality half
+    of the metric: a 960x540 window at the frame centre (windows are
+    byte-identical to the same pixels of the full frame) rendered at 4096
+    spp with FP64 sums is the reference; the full 4K frame is traced at 4 spp
+    and at 64 spp, with and without the repo denoiser (OIDN's RT filter)."""
+    ww, wh = 960, 540
+    x0, y0 = (WIDTH - ww) // 2, (HEIGHT - wh) // 2
+    win = (slice(y0, y0 + wh), slice(x0, x0 + ww))
+    t = time.perf_counter()
+    with Trace.new_detailed(scene, Detail(bounces=BOUNCES, accumulate=ACCUM_FP64), ww, wh) as ref_trace:
+        ref_trace.set_window((WIDTH, HEIGHT), (x0, y0))
+        for _ in range(64):
+            ref_trace.sample(64, 7777)
+        ref = ref_trace.readback().color
+    log(f"quality reference: 4096 spp window in {time.perf_counter() - t:.1f} s")
+    out = {"window": [x0, y0, ww, wh], "reference": "4096 spp, FP64 sums, seed 7777", "ms": {}, "relmse": {}}
+    with denoise_mod.Denoiser() as d, Trace.new_detailed(scene, Detail(bounces=BOUNCES), WIDTH, HEIGHT) as t4k:
+        for spp in (4, 64):
+            t4k.set_camera(scene.camera)
+            t = time.perf_counter()
+            t4k.sample(spp, 1)
+            img = t4k.readback().color
+            traced = time.perf_counter() - t
+            t = time.perf_counter()
+            den = t4k.denoise(d)
+            denoised = time.perf_counter() - t
+            out["ms"][f"spp{spp}_trace"] = round(traced * 1e3, 1)
+            out["ms"]["repo_denoiser"] = round(denoised * 1e3, 1)
+            out["relmse"][f"spp{spp}_raw"] = round(rel_mse(img[win], ref), 5)
+            out["relmse"][f"spp{spp}_repo_denoiser"] = round(rel_mse(den[win], ref), 5)
+    best = [(out["ms"][f"spp{s}_trace"] + out["ms"]["repo_denoiser"], s) for s in (4, 64)
+            if out["relmse"][f"spp{s}_repo_denoiser"] <= 0.01]
+    out["ms_to_relmse_0.01"] = {
+        "plain_tracer_plus_repo_denoiser": round(min(best)[0], 1) if best else None,
+        "spp4_plus_learned_denoiser": None,  # the learned denoiser is not built
+    }
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-quality", action="store_true", help="skip the image-quality leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); across GPUs the still would be "
@@ -192,6 +245,8 @@ def main() -> int:
         "cpu_baseline": None,
     }
     trace.close()
+    if not args.no_quality:
+        result["quality"] = quality_leg(scene)
     if not args.no_cpu:
         log("cpu baseline (oracle on host cores)...")
         result["cpu_baseline"] = cpu_baseline(scene)

@@ -86,3 +86,27 @@ def plan(height: int, width: int, max_tile_pixels: int = 2160 * 2160):
     N.lib().pfxh_denoise_plan(height, width, max_tile_pixels, C.byref(th), C.byref(tw), tiles, count.value,
                               C.byref(count))
     return th.value, tw.value, [{k: tuple(getattr(t, k)) for k, _ in N.DenoiseTile._fields_} for t in tiles[:count.value]]
+
+
+def _main() -> None:
+    """`python -m pfx_amd.denoise [WIDTH HEIGHT [RUNS]]`: denoise a rendered
+    terrain frame (4 spp) RUNS times and print the time of each run."""
+    import sys
+    import time
+
+    from . import Bounces, Detail, Trace, scenes
+
+    w, h = (int(sys.argv[1]), int(sys.argv[2])) if len(sys.argv) > 2 else (3840, 2160)
+    runs = int(sys.argv[3]) if len(sys.argv) > 3 else 3
+    scene = scenes.terrain(width=w, height=h)
+    with Trace.new_detailed(scene, Detail(bounces=Bounces.deep(6)), w, h) as t, Denoiser() as d:
+        t.sample(4, 1)
+        for k in range(runs):
+            s = time.perf_counter()
+            img = t.denoise(d)
+            print(f"denoise {w}x{h} run {k}: {(time.perf_counter() - s) * 1e3:.1f} ms, finite {bool(np.isfinite(img).all())}",
+                  flush=True)
+
+
+if __name__ == "__main__":
+    _main()
