@@ -388,6 +388,77 @@ pfxh_status pfxh_denoise_plan(uint32_t height, uint32_t width, uint64_t max_tile
 /* Drop the denoiser and its device memory. NULL is allowed. */
 void pfxh_denoiser_free(pfxh_denoiser* denoiser);
 
+/* ---- the learned denoiser --------------------------------------------------
+ * No counterpart in the reference, which runs only the fixed-weight filter
+ * above (crates/denoise/src/lib.rs:15-17). A 5-level encoder-decoder of 16
+ * 3x3 convolutions with skip connections (the U-Net shape of net.rs:202-232,
+ * 32->128 channels, 1,238,787 parameters), BF16 operands on matrix cores
+ * with FP32 sums, trained on the GPU (forward and backward kernels, L1 loss
+ * on log radiance, Adam) from pairs the tracer renders. Inputs are the
+ * still's colour, albedo, normal and depth (10 channels); the output is
+ * colour. Parameters cross the boundary in a canonical order: per layer,
+ * weights [out][in][3][3] (the inputs being the first source's channels,
+ * then the skip source's) and then the bias [out]. */
+
+#define PFXH_LEARN_LAYERS 16
+#define PFXH_LEARN_FP32 1u /* pfxh_learner_new: FP32 operands instead of BF16 */
+
+typedef struct pfxh_learner pfxh_learner;
+
+typedef struct pfxh_learn_layer_info {
+    char name[16];
+    int32_t in0;  /* tensor read (0: the encoded input, k + 1: layer k's output) */
+    int32_t in1;  /* skip tensor concatenated after in0, or -1 */
+    uint32_t in_channels, out_channels;
+    uint32_t level;    /* resolution: full >> level */
+    uint32_t upsample; /* in0 comes from level + 1, nearest 2x */
+    uint32_t pool;     /* 2x2 max pool after the activation */
+    uint32_t relu;
+} pfxh_learn_layer_info;
+
+/* The network's layers (host only). */
+pfxh_status pfxh_learn_layer(uint32_t index, pfxh_learn_layer_info* out);
+uint64_t pfxh_learn_param_count(void);
+/* A network with zero parameters; `flags` PFXH_LEARN_FP32 or 0. */
+pfxh_status pfxh_learner_new(uint32_t flags, pfxh_learner** out, char* err, size_t err_len);
+pfxh_status pfxh_learner_set_params(pfxh_learner* learner, const float* params, uint64_t count,
+                                    char* err, size_t err_len);
+pfxh_status pfxh_learner_params(pfxh_learner* learner, float* out, uint64_t count, char* err,
+                                size_t err_len);
+/* Gradient of the loss at the last pfxh_learner_gradient or training step. */
+pfxh_status pfxh_learner_grads(pfxh_learner* learner, float* out, uint64_t count, char* err,
+                               size_t err_len);
+/* Clear Adam's moments and step count. */
+void pfxh_learner_reset_optimizer(pfxh_learner* learner);
+/* Make room on the GPU for `capacity` training crops of side x side pixels
+ * (a multiple of 16), dropping crops added before. */
+pfxh_status pfxh_learner_reserve(pfxh_learner* learner, uint32_t side, uint32_t capacity, char* err,
+                                 size_t err_len);
+/* Add one crop: RGBA32F colour, albedo, normal and target (side*side*4
+ * floats each) and depth (side*side floats, or NULL). */
+pfxh_status pfxh_learner_add(pfxh_learner* learner, const float* color, const float* albedo,
+                             const float* normal, const float* depth, const float* target, char* err,
+                             size_t err_len);
+uint32_t pfxh_learner_crops(const pfxh_learner* learner);
+/* Loss and gradient on `batch` crops, no update: picks[b] = crop << 3 |
+ * symmetry (bit 0 mirrors x, bit 1 mirrors y, bit 2 transposes). */
+pfxh_status pfxh_learner_gradient(pfxh_learner* learner, const uint32_t* picks, uint32_t batch,
+                                  float* loss, char* err, size_t err_len);
+/* `steps` Adam steps (beta 0.9, 0.999, eps 1e-8) on batches drawn from the
+ * training set with `seed`; losses[k] gets step k's mean loss. */
+pfxh_status pfxh_learner_train(pfxh_learner* learner, uint32_t steps, uint32_t batch, float lr,
+                               uint64_t seed, float* losses, char* err, size_t err_len);
+/* Denoise host images (RGBA32F colour, albedo, normal; depth NULL or
+ * width*height floats); `out` gets RGBA32F, alpha from the colour. */
+pfxh_status pfxh_learned_denoise(pfxh_learner* learner, uint32_t width, uint32_t height,
+                                 const float* color, const float* albedo, const float* normal,
+                                 const float* depth, float* out, char* err, size_t err_len);
+/* The same on a trace's current colour, albedo and normal and its probe's
+ * depth, without a host round trip for the inputs. */
+pfxh_status pfxh_trace_learned_denoise(pfxh_trace* trace, pfxh_learner* learner, float* out, char* err,
+                                       size_t err_len);
+void pfxh_learner_free(pfxh_learner* learner);
+
 /* Library version string. */
 const char* pfxh_version(void);
 

@@ -36,6 +36,10 @@ def lib() -> C.CDLL:
             "or_tza_count": (C.c_int, [C.c_char_p, C.c_size_t]),
             "or_denoise_plan": (C.c_int, [_U, _U, C.c_uint64, C.POINTER(_U), C.POINTER(_U), _P, _U]),
             "or_denoise": (C.c_int, [C.c_char_p, C.c_size_t, _U, _U, _P, _P, _P, _P, C.c_uint64]),
+            "or_learn_param_count": (C.c_uint64, []),
+            "or_learn_layer": (None, [_U, _P]),
+            "or_learn_grad": (C.c_double, [_P, _U, _U, _U, _P, _P, _P, _P, _P, _P]),
+            "or_learn_forward": (None, [_P, _U, _U, _P, _P, _P, _P, _P]),
             "or_scene_free": (None, [_P]),
             "or_set_camera": (None, [_P, _P]),
             "or_set_window": (C.c_int, [_P, _U, _U, _U, _U]),
@@ -164,3 +168,45 @@ class Oracle:
             self.close()
         except Exception:
             pass
+
+
+# ---- the learned denoiser (FP64) ---------------------------------------------------------
+
+def learn_param_count() -> int:
+    return int(lib().or_learn_param_count())
+
+
+def learn_layers() -> list[tuple[int, ...]]:
+    """(in0, in1, in_channels, out_channels, level, upsample, pool, relu) per layer."""
+    out = []
+    for k in range(16):
+        v = np.zeros(8, np.int32)
+        lib().or_learn_layer(k, v.ctypes.data)
+        out.append(tuple(int(x) for x in v))
+    return out
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, np.float32)
+
+
+def learn_grad(params, color, albedo, normal, depth, target, want_grad: bool = True):
+    """Loss and gradient for crops shaped (batch, h, w, 4) (depth (batch, h, w))."""
+    p = np.ascontiguousarray(params, np.float64)
+    color, albedo, normal, depth, target = (_f32(a) for a in (color, albedo, normal, depth, target))
+    b, h, w = color.shape[:3]
+    grad = np.zeros(len(p), np.float64) if want_grad else None
+    loss = lib().or_learn_grad(p.ctypes.data, b, h, w, color.ctypes.data, albedo.ctypes.data, normal.ctypes.data,
+                               _ptr(depth), target.ctypes.data, _ptr(grad))
+    return float(loss), grad
+
+
+def learn_forward(params, color, albedo, normal, depth):
+    """Denoise one (h, w, 4) image; returns (h, w, 4) float32."""
+    p = np.ascontiguousarray(params, np.float64)
+    color, albedo, normal, depth = (_f32(a) for a in (color, albedo, normal, depth))
+    h, w = color.shape[:2]
+    out = np.zeros((h, w, 4), np.float32)
+    lib().or_learn_forward(p.ctypes.data, h, w, color.ctypes.data, albedo.ctypes.data, normal.ctypes.data,
+                           _ptr(depth), out.ctypes.data)
+    return out

@@ -190,6 +190,23 @@ class CheckpointC(C.Structure):
 
 NO_ROUNDS = 0xFFFFFFFF
 
+LEARN_LAYERS = 16
+LEARN_FP32 = 1
+
+
+class LearnLayerC(C.Structure):
+    _fields_ = [
+        ("name", C.c_char * 16),
+        ("in0", C.c_int32),
+        ("in1", C.c_int32),
+        ("in_channels", C.c_uint32),
+        ("out_channels", C.c_uint32),
+        ("level", C.c_uint32),
+        ("upsample", C.c_uint32),
+        ("pool", C.c_uint32),
+        ("relu", C.c_uint32),
+    ]
+
 _P = C.c_void_p
 _U = C.c_uint32
 _S = C.c_char_p
@@ -214,6 +231,21 @@ SIGNATURES = {
     "pfxh_denoiser_layers": (C.c_int, [_S, _Z, C.POINTER(_U), C.POINTER(C.c_uint64), _S, _Z]),
     "pfxh_denoise_plan": (C.c_int, [_U, _U, C.c_uint64, C.POINTER(_U), C.POINTER(_U), _P, _U, C.POINTER(_U)]),
     "pfxh_denoiser_free": (None, [_P]),
+    "pfxh_learn_layer": (C.c_int, [_U, _P]),
+    "pfxh_learn_param_count": (C.c_uint64, []),
+    "pfxh_learner_new": (C.c_int, [_U, C.POINTER(_P), _S, _Z]),
+    "pfxh_learner_set_params": (C.c_int, [_P, _P, C.c_uint64, _S, _Z]),
+    "pfxh_learner_params": (C.c_int, [_P, _P, C.c_uint64, _S, _Z]),
+    "pfxh_learner_grads": (C.c_int, [_P, _P, C.c_uint64, _S, _Z]),
+    "pfxh_learner_reset_optimizer": (None, [_P]),
+    "pfxh_learner_reserve": (C.c_int, [_P, _U, _U, _S, _Z]),
+    "pfxh_learner_add": (C.c_int, [_P, _P, _P, _P, _P, _P, _S, _Z]),
+    "pfxh_learner_crops": (_U, [_P]),
+    "pfxh_learner_gradient": (C.c_int, [_P, _P, _U, C.POINTER(C.c_float), _S, _Z]),
+    "pfxh_learner_train": (C.c_int, [_P, _U, _U, C.c_float, C.c_uint64, _P, _S, _Z]),
+    "pfxh_learned_denoise": (C.c_int, [_P, _U, _U, _P, _P, _P, _P, _P, _S, _Z]),
+    "pfxh_trace_learned_denoise": (C.c_int, [_P, _P, _P, _S, _Z]),
+    "pfxh_learner_free": (None, [_P]),
     "pfxh_trace_restore": (C.c_int, [_P, _S, _P, _Z, _S, _Z]),
     "pfxh_trace_samples": (_U, [_P]),
     "pfxh_trace_gpu_bytes": (C.c_uint64, [_P]),

@@ -18,6 +18,9 @@ struct ConvDev {
     uint32_t n_pad;      // padded output channels (16 * n-tiles)
     uint32_t h, w_;      // conv resolution (before pooling)
     uint32_t upsample, pool;
+    uint32_t relu;       // 1: max(0, x) after the bias
+    uint32_t bf16;       // 1: operands rounded to bfloat16, FP32 sums
+    uint32_t batch;      // images, each tensor holding them one after another
 };
 
 // One tile of the image (tile.rs Tile, [y, x] pairs).

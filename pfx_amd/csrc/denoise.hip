@@ -20,10 +20,19 @@
 namespace pfxd {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v4s __attribute__((ext_vector_type(4)));
+
+// float -> bfloat16 bits, round to nearest even (finite inputs)
+__device__ __forceinline__ short bf16_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+__device__ __forceinline__ v4s bf16x4(float4 v) { return v4s{bf16_bits(v.x), bf16_bits(v.y), bf16_bits(v.z), bf16_bits(v.w)}; }
 
 constexpr int TW = 16, TH = 4, PW = TW + 2, PH = TH + 2;
 
-template <int NT>
+template <int NT, bool BF16>
 __global__ __launch_bounds__(256) void k_conv(ConvDev c) {
     __shared__ float4 patch[PH * PW * 4];    // [row][col][16 channels]
     __shared__ float pooled[2][NT][64][2];   // odd rows' x-pooled values
@@ -32,6 +41,14 @@ __global__ __launch_bounds__(256) void k_conv(ConvDev c) {
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
     const int h = (int)c.h, w = (int)c.w_;
     const int sw = c.upsample ? w / 2 : w;
+    {
+        // image blockIdx.z of the batch
+        const size_t z = blockIdx.z, px = (size_t)h * w;
+        const size_t spx = c.upsample ? (size_t)(h / 2) * sw : px;
+        c.src0 += z * spx * c.c0;
+        if (c.src1) c.src1 += z * px * c.c1;
+        c.dst += z * (c.pool ? px / 4 : px) * c.n_pad;
+    }
     v4f acc[NT];
     for (int t = 0; t < NT; ++t) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     const uint32_t chunks = (c.c0 + c.c1) / 16u;
@@ -59,13 +76,22 @@ __global__ __launch_bounds__(256) void k_conv(ConvDev c) {
             const int ky = tap / 3, kx = tap - ky * 3;
             const float4 a = patch[((wave + ky) * PW + i + kx) * 4 + kk];
             const float* wt = wb + (size_t)tap * c.n_pad * 16u;
+            if (BF16) {
+                const v4s ab = bf16x4(a);
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const float4 b = *reinterpret_cast<const float4*>(wt + (size_t)(16 * t + i) * 16u + 4 * kk);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b.z, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, acc[t], 0, 0, 0);
+                for (int t = 0; t < NT; ++t) {
+                    const float4 b = *reinterpret_cast<const float4*>(wt + (size_t)(16 * t + i) * 16u + 4 * kk);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ab, bf16x4(b), acc[t], 0, 0, 0);
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const float4 b = *reinterpret_cast<const float4*>(wt + (size_t)(16 * t + i) * 16u + 4 * kk);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b.z, acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, acc[t], 0, 0, 0);
+                }
             }
         }
     }
@@ -75,11 +101,12 @@ __global__ __launch_bounds__(256) void k_conv(ConvDev c) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int n = 16 * t + i;
-            const float bias = c.bias[n];
+            const float bias = c.bias ? c.bias[n] : 0.0f;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int x = x0 + 4 * kk + v;
-                if (y < h && x < w) c.dst[((size_t)y * w + x) * c.n_pad + n] = fmaxf(acc[t][v] + bias, 0.0f);
+                const float r = acc[t][v] + bias;
+                if (y < h && x < w) c.dst[((size_t)y * w + x) * c.n_pad + n] = c.relu ? fmaxf(r, 0.0f) : r;
             }
         }
         return;
@@ -88,7 +115,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvDev c) {
     float m[NT][2];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const float bias = c.bias[16 * t + i];
+        const float bias = c.bias ? c.bias[16 * t + i] : 0.0f;
 #pragma unroll
         for (int p = 0; p < 2; ++p)
             m[t][p] = fmaxf(fmaxf(acc[t][2 * p] + bias, acc[t][2 * p + 1] + bias), 0.0f);
@@ -109,18 +136,25 @@ __global__ __launch_bounds__(256) void k_conv(ConvDev c) {
         }
 }
 
-void launch_conv(hipStream_t s, const ConvDev& c) {
-    dim3 grid((c.w_ + TW - 1) / TW, (c.h + TH - 1) / TH);
+template <bool BF16>
+static void launch_conv_t(hipStream_t s, const ConvDev& c, dim3 grid) {
     switch (c.n_pad / 16u) {
-        case 1: hipLaunchKernelGGL(k_conv<1>, grid, dim3(256), 0, s, c); break;
-        case 2: hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), 0, s, c); break;
-        case 3: hipLaunchKernelGGL(k_conv<3>, grid, dim3(256), 0, s, c); break;
-        case 4: hipLaunchKernelGGL(k_conv<4>, grid, dim3(256), 0, s, c); break;
-        case 5: hipLaunchKernelGGL(k_conv<5>, grid, dim3(256), 0, s, c); break;
-        case 6: hipLaunchKernelGGL(k_conv<6>, grid, dim3(256), 0, s, c); break;
-        case 7: hipLaunchKernelGGL(k_conv<7>, grid, dim3(256), 0, s, c); break;
+#define PFXD_CONV(n) \
+    case n: hipLaunchKernelGGL((k_conv<n, BF16>), grid, dim3(256), 0, s, c); break;
+        PFXD_CONV(1) PFXD_CONV(2) PFXD_CONV(3) PFXD_CONV(4) PFXD_CONV(5) PFXD_CONV(6)
+        PFXD_CONV(7) PFXD_CONV(8) PFXD_CONV(9) PFXD_CONV(10) PFXD_CONV(11) PFXD_CONV(12)
+        PFXD_CONV(13) PFXD_CONV(14)
+#undef PFXD_CONV
         default: break;  // the host refuses other widths
     }
+}
+
+void launch_conv(hipStream_t s, const ConvDev& c) {
+    dim3 grid((c.w_ + TW - 1) / TW, (c.h + TH - 1) / TH, c.batch ? c.batch : 1u);
+    if (c.bf16)
+        launch_conv_t<true>(s, c, grid);
+    else
+        launch_conv_t<false>(s, c, grid);
 }
 
 // ---- autoexposure (process.wgsl:104-157) --------------------------------------

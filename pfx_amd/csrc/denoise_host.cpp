@@ -337,6 +337,9 @@ struct pfxh_denoiser {
                 c.w_ = tile_w >> l.level;
                 c.upsample = l.upsample ? 1u : 0u;
                 c.pool = l.pool ? 1u : 0u;
+                c.relu = 1u;  // OIDN's RT filter applies ReLU after every layer
+                c.bf16 = 0u;
+                c.batch = 1u;
                 pfxd::launch_conv(stream, c);
             }
             pfxd::launch_output(stream, tensors.back(), tile_w, scale, color, width, t, rcp_norm, out);

@@ -21,6 +21,10 @@
 pfxh_status pfxh_denoise_device(pfxh_denoiser* d, uint32_t width, uint32_t height, const void* color,
                                 const void* albedo, const void* normal, float* out, hipStream_t after, char* err,
                                 size_t err_len);
+pfxh_status pfxh_learned_denoise_device(pfxh_learner* l, uint32_t width, uint32_t height, const void* color,
+                                        const void* albedo, const void* normal, const float* depth,
+                                        uint32_t depth_stride, float* out, hipStream_t after, char* err,
+                                        size_t err_len);
 #include "layout.h"
 #include "vec3.h"
 
@@ -659,6 +663,24 @@ pfxh_status pfxh_trace_denoise(pfxh_trace* t, pfxh_denoiser* d, float* out, char
     if (st != PFXH_OK || !pixels) return st;
     float4* c = t->images;
     return pfxh_denoise_device(d, t->width, t->height, c, c + pixels, c + 2 * pixels, out, t->stream, err, err_len);
+}
+
+pfxh_status pfxh_trace_learned_denoise(pfxh_trace* t, pfxh_learner* l, float* out, char* err, size_t err_len) {
+    if (!t || !l || !out) return PFXH_INVALID;
+    size_t pixels = (size_t)t->width * t->height;
+    pfxh_status st = guarded(err, err_len, [&] {
+        if (!pixels) return;
+        float4* c = t->images;
+        launch_develop(t->stream, t->A, (uint32_t)pixels, c, c + pixels, c + 2 * pixels);
+        if (!t->probe) t->probe = (pfxh_probe_hit*)t->alloc(pixels * sizeof(pfxh_probe_hit), "probe");
+        launch_probe(t->stream, t->S, t->W, t->probe);
+        HIP_TRY(hipGetLastError());
+    });
+    if (st != PFXH_OK || !pixels) return st;
+    float4* c = t->images;
+    static_assert(offsetof(pfxh_probe_hit, depth) == 0 && sizeof(pfxh_probe_hit) % 4 == 0, "probe layout");
+    return pfxh_learned_denoise_device(l, t->width, t->height, c, c + pixels, c + 2 * pixels, (const float*)t->probe,
+                                       (uint32_t)(sizeof(pfxh_probe_hit) / 4), out, t->stream, err, err_len);
 }
 
 uint32_t pfxh_trace_samples(const pfxh_trace* t) { return t ? t->samples : 0; }

@@ -192,6 +192,15 @@ class Trace:
         self._check(N.lib().pfxh_trace_denoise(self._handle(), denoiser._h, out.ctypes.data, err, len(err)), err)
         return out
 
+    def learned_denoise(self, learner) -> np.ndarray:
+        """The learned denoiser (`pfx_amd.learn.Learner`) on this trace's colour,
+        albedo, normal and probe depth, on the device: the (H, W, 4) image."""
+        out = np.empty((self.height, self.width, 4), np.float32)
+        err = N.err_buffer()
+        self._check(N.lib().pfxh_trace_learned_denoise(self._handle(), learner._h, out.ctypes.data, err, len(err)),
+                    err)
+        return out
+
     def samples(self) -> int:
         return int(N.lib().pfxh_trace_samples(self._handle()))
 

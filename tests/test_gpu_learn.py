@@ -1,0 +1,162 @@
+"""The learned denoiser's HIP kernels against the FP64 oracle: loss and
+gradient of every layer (FP32 operands: 1e-3 of each layer's gradient norm;
+BF16 operands, the training mode: 0.15 per layer and 0.05 over the whole
+gradient, the deep layers of a 32-pixel crop seeing only 16 to 64 pixels,
+where single ReLU flips from rounding show), the batch gather under all eight
+symmetries of the square crop, inference on a frame that is not a multiple of
+16, the first Adam step in closed form (p - lr * g / (|g| + eps)), and
+training that lowers the loss."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pfx_amd import learn as Lm
+
+pytestmark = pytest.mark.gpu
+
+
+def crops(rng, b, h, w):
+    color = rng.exponential(0.5, (b, h, w, 4)).astype(np.float32)
+    albedo = rng.uniform(0, 1, (b, h, w, 4)).astype(np.float32)
+    normal = rng.uniform(-1, 1, (b, h, w, 4)).astype(np.float32)
+    depth = rng.uniform(0, 20, (b, h, w)).astype(np.float32)
+    target = rng.exponential(0.5, (b, h, w, 4)).astype(np.float32)
+    return color, albedo, normal, depth, target
+
+
+def symmetry(a, sym):
+    """The crop as the gather kernel reads it under `sym` (bit 0 mirrors x,
+    bit 1 mirrors y, bit 2 transposes)."""
+    side = a.shape[0]
+    oy, ox = np.indices((side, side))
+    sx = side - 1 - ox if sym & 1 else ox
+    sy = side - 1 - oy if sym & 2 else oy
+    if sym & 4:
+        sx, sy = sy, sx
+    return a[sy, sx]
+
+
+def params_for_test(seed):
+    rng = np.random.default_rng(seed)
+    p = Lm.init_params(seed)
+    p += rng.normal(0, 0.01, len(p)).astype(np.float32) * (np.abs(p) == 0)  # nonzero biases
+    return p
+
+
+def layer_slices():
+    out, off = [], 0
+    for l in O.learn_layers():
+        n = l[3] * l[2] * 9 + l[3]
+        out.append((off, off + n))
+        off += n
+    return out
+
+
+def filled(learner, data, side):
+    learner.reserve(side, len(data[0]))
+    for i in range(len(data[0])):
+        learner.add(*(a[i] for a in data))
+    return learner
+
+
+@pytest.mark.parametrize("fp32,tol,total", [(True, 1e-3, 1e-3), (False, 0.15, 0.05)])
+def test_gradient_matches_fp64_oracle(fp32, tol, total):
+    print(f"learn gradient fp32={fp32}", flush=True)
+    rng = np.random.default_rng(21)
+    data = crops(rng, 2, 32, 32)
+    p = params_for_test(21)
+    picks = [0 << 3 | 0, 1 << 3 | 5]
+    with filled(Lm.Learner(fp32=fp32, params=p), data, 32) as L:
+        loss = L.gradient(picks)
+        g = L.grads()
+    ref_data = [np.stack([a[0], symmetry(a[1], 5)]) for a in data]
+    ref_loss, ref_g = O.learn_grad(p.astype(np.float64), *ref_data)
+    assert loss == pytest.approx(ref_loss, rel=tol * 1e-2 if fp32 else 1e-2)
+    for k, (lo, hi) in enumerate(layer_slices()):
+        gap = np.linalg.norm(g[lo:hi] - ref_g[lo:hi])
+        size = np.linalg.norm(ref_g[lo:hi])
+        assert gap <= tol * size, (k, gap, size)
+    assert np.linalg.norm(g - ref_g) <= total * np.linalg.norm(ref_g)
+
+
+def test_gather_symmetries_match_the_oracle():
+    print("learn gather symmetries", flush=True)
+    rng = np.random.default_rng(22)
+    data = crops(rng, 1, 32, 32)
+    p = params_for_test(22)
+    with filled(Lm.Learner(fp32=True, params=p), data, 32) as L:
+        for sym in range(8):
+            loss = L.gradient([sym])
+            ref = [symmetry(a[0], sym)[None] for a in data]
+            ref_loss, _ = O.learn_grad(p.astype(np.float64), *ref, want_grad=False)
+            assert loss == pytest.approx(ref_loss, rel=1e-5), sym
+
+
+@pytest.mark.parametrize("fp32,tol", [(True, 1e-4), (False, 5e-2)])
+def test_inference_matches_fp64_oracle(fp32, tol):
+    print(f"learn inference fp32={fp32}", flush=True)
+    rng = np.random.default_rng(23)
+    h, w = 24, 40
+    color, albedo, normal, depth, _ = (a[0] for a in crops(rng, 1, h, w))
+    p = params_for_test(23)
+    with Lm.Learner(fp32=fp32, params=p) as L:
+        out = L.denoise(color, albedo, normal, depth)
+    ref = O.learn_forward(p.astype(np.float64), color, albedo, normal, depth)
+    gap = np.abs(np.log1p(out[..., :3]) - np.log1p(ref[..., :3]))
+    assert float(gap.max()) <= tol * max(1.0, float(np.log1p(ref[..., :3]).max()))
+    np.testing.assert_array_equal(out[..., 3], color[..., 3])
+
+
+def splitmix_picks(seed, count, crops_n):
+    mask = (1 << 64) - 1
+    x, out = seed, []
+    for _ in range(count):
+        x = (x + 0x9E3779B97F4A7C15) & mask
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        z ^= z >> 31
+        out.append(((z >> 3) % crops_n) << 3 | (z & 7))
+    return out
+
+
+def test_first_adam_step_is_the_closed_form():
+    print("learn adam step", flush=True)
+    rng = np.random.default_rng(24)
+    data = crops(rng, 3, 32, 32)
+    p0 = params_for_test(24)
+    lr, seed, batch = 1e-3, 99, 2
+    with filled(Lm.Learner(fp32=True, params=p0), data, 32) as L:
+        L.gradient(splitmix_picks(seed, batch, 3))
+        g = L.grads()
+        losses = L.train(1, batch=batch, lr=lr, seed=seed)
+        p1 = L.params()
+    assert np.isfinite(losses).all()
+    want = p0 - lr * g / (np.abs(g) + 1e-8)
+    live = np.abs(g) > 1e-5
+    assert live.sum() > 1000
+    np.testing.assert_allclose(p1[live], want[live], rtol=0, atol=2e-3 * lr)
+    np.testing.assert_array_equal(p1[g == 0], p0[g == 0])
+
+
+def test_training_lowers_the_loss():
+    print("learn training", flush=True)
+    rng = np.random.default_rng(25)
+    side, n = 32, 8
+    yy, xx = np.indices((side, side)) / side
+    targets, noisy = [], []
+    for i in range(n):
+        base = np.stack([np.sin(3 * xx + i) + 1.2, np.cos(2 * yy + i) + 1.2, xx * yy + 0.2, np.ones_like(xx)], -1)
+        targets.append(base.astype(np.float32))
+        noisy.append((base * rng.exponential(1.0, base.shape)).astype(np.float32))
+    albedo = np.full((side, side, 4), 0.5, np.float32)
+    normal = np.zeros((side, side, 4), np.float32)
+    with Lm.Learner(params=Lm.init_params(25)) as L:
+        L.reserve(side, n)
+        for i in range(n):
+            L.add(noisy[i], albedo, normal, None, targets[i])
+        losses = L.train(150, batch=4, lr=1e-3, seed=7)
+    assert np.isfinite(losses).all()
+    first, last = float(losses[:10].mean()), float(losses[-10:].mean())
+    print(f"loss {first:.4f} -> {last:.4f}", flush=True)
+    assert last < 0.5 * first
