@@ -12,14 +12,16 @@ struct ConvDev {
     const float* src0;   // level `level`, or level + 1 when upsampled
     const float* src1;   // skip connection at level `level` (concat), or null
     float* dst;          // level `level`, or level + 1 when pooled
-    const float* w;      // packed: [chunk][tap][n][16 channels of the chunk]
+    const void* w;       // packed: [chunk][tap][n][16 channels of the chunk];
+                         // float, or bfloat16 bits when `bf16`
     const float* bias;   // n_pad floats
     uint32_t c0, c1;     // padded channels of src0 and src1
     uint32_t n_pad;      // padded output channels (16 * n-tiles)
     uint32_t h, w_;      // conv resolution (before pooling)
     uint32_t upsample, pool;
     uint32_t relu;       // 1: max(0, x) after the bias
-    uint32_t bf16;       // 1: operands rounded to bfloat16, FP32 sums
+    uint32_t bf16;       // 1: bfloat16 operands (inputs rounded to nearest even
+                         // as they are staged), FP32 sums
     uint32_t batch;      // images, each tensor holding them one after another
 };
 

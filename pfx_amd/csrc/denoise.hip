@@ -15,6 +15,8 @@
 //                      (process.wgsl:33-102)
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "denoise.h"
 
 namespace pfxd {
@@ -32,9 +34,21 @@ __device__ __forceinline__ v4s bf16x4(float4 v) { return v4s{bf16_bits(v.x), bf1
 
 constexpr int TW = 16, TH = 4, PW = TW + 2, PH = TH + 2;
 
+__device__ __forceinline__ uint2 bf16x4_bits(float4 v) {
+    return make_uint2((uint32_t)(uint16_t)bf16_bits(v.x) | (uint32_t)(uint16_t)bf16_bits(v.y) << 16,
+                      (uint32_t)(uint16_t)bf16_bits(v.z) | (uint32_t)(uint16_t)bf16_bits(v.w) << 16);
+}
+__device__ __forceinline__ v4s as_v4s(uint2 u) {
+    return v4s{(short)(u.x & 0xffffu), (short)(u.x >> 16), (short)(u.y & 0xffffu), (short)(u.y >> 16)};
+}
+
+// BF16: the input chunk is rounded to bfloat16 as it is staged (uint2 = four
+// channels) and the packed weights are bfloat16 already, so the K loop feeds
+// v_mfma_f32_16x16x16_bf16 straight from LDS and memory.
 template <int NT, bool BF16>
 __global__ __launch_bounds__(256) void k_conv(ConvDev c) {
-    __shared__ float4 patch[PH * PW * 4];    // [row][col][16 channels]
+    typedef typename std::conditional<BF16, uint2, float4>::type Staged;
+    __shared__ Staged patch[PH * PW * 4];    // [row][col][16 channels]
     __shared__ float pooled[2][NT][64][2];   // odd rows' x-pooled values
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i = lane & 15, kk = lane >> 4;
@@ -68,22 +82,25 @@ __global__ __launch_bounds__(256) void k_conv(ConvDev c) {
                     v = *reinterpret_cast<const float4*>(c.src1 + ((size_t)y * w + x) * c.c1 + (cc - c.c0));
                 }
             }
-            patch[e] = v;
+            if constexpr (BF16)
+                patch[e] = bf16x4_bits(v);
+            else
+                patch[e] = v;
         }
         __syncthreads();
-        const float* wb = c.w + (size_t)ch * 9u * c.n_pad * 16u;
         for (int tap = 0; tap < 9; ++tap) {
             const int ky = tap / 3, kx = tap - ky * 3;
-            const float4 a = patch[((wave + ky) * PW + i + kx) * 4 + kk];
-            const float* wt = wb + (size_t)tap * c.n_pad * 16u;
-            if (BF16) {
-                const v4s ab = bf16x4(a);
+            const Staged a = patch[((wave + ky) * PW + i + kx) * 4 + kk];
+            if constexpr (BF16) {
+                const uint16_t* wt = static_cast<const uint16_t*>(c.w) + ((size_t)ch * 9u + tap) * c.n_pad * 16u;
+                const v4s ab = as_v4s(a);
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
-                    const float4 b = *reinterpret_cast<const float4*>(wt + (size_t)(16 * t + i) * 16u + 4 * kk);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ab, bf16x4(b), acc[t], 0, 0, 0);
+                    const uint2 b = *reinterpret_cast<const uint2*>(wt + (size_t)(16 * t + i) * 16u + 4 * kk);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ab, as_v4s(b), acc[t], 0, 0, 0);
                 }
             } else {
+                const float* wt = static_cast<const float*>(c.w) + ((size_t)ch * 9u + tap) * c.n_pad * 16u;
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
                     const float4 b = *reinterpret_cast<const float4*>(wt + (size_t)(16 * t + i) * 16u + 4 * kk);

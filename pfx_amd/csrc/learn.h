@@ -14,17 +14,25 @@ struct DwArgs {
     const float* src0;   // [batch][h or h/2][w or w/2][c0]
     const float* src1;   // [batch][h][w][c1], or null when c1 == 0
     float* part;         // chunks * 9 * np * (c0 + c1) partial sums
+    float* part_b;       // chunks * np partial bias sums
     uint32_t c0, c1, np, h, w, batch, up;
+    uint32_t ct;         // 16-channel input tiles per block (set from the plan)
 };
+// Launch shape of the weight gradient for one layer and frame.
+struct DwPlan {
+    uint32_t ct, groups, chunks;
+    size_t lds;
+};
+DwPlan dw_plan(uint32_t np, uint32_t cin, uint32_t batch, uint32_t h, uint32_t w);
 
-void pack(hipStream_t s, const float* master, uint32_t np, uint32_t cin, float* fwd, float* transposed);
+void pack(hipStream_t s, const float* master, uint32_t np, uint32_t cin, void* fwd, void* transposed, bool bf16);
 void pool_fwd(hipStream_t s, const float* r, uint32_t b, uint32_t h, uint32_t w, uint32_t c, float* p);
 void pool_bwd(hipStream_t s, const float* dp, const float* r, uint32_t b, uint32_t h, uint32_t w, uint32_t c,
               float* dr);
 void relu_bwd(hipStream_t s, const float* dy, const float* y, size_t n, float* out);
 void scatter(hipStream_t s, const float* dx, uint32_t b, uint32_t h, uint32_t w, uint32_t c0, uint32_t c1, bool up,
              float* d0, float* d1);
-void weight_grads(hipStream_t s, const DwArgs& a, uint32_t chunks, float* dw, float* db);
+void weight_grads(hipStream_t s, const DwArgs& a, const DwPlan& p, bool bf16, float* dw, float* db);
 void learn_in(hipStream_t s, const float4* color, const float4* albedo, const float4* normal, const float* depth,
               uint32_t depth_stride, uint32_t w, uint32_t h, uint32_t w16, uint32_t h16, float* x);
 uint32_t loss(hipStream_t s, const float* y, const float4* target, size_t n, float* dy, float* sums);

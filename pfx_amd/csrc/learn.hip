@@ -10,14 +10,16 @@
 //   k_relu_bwd              gradient through ReLU (the output is kept)
 //   k_scatter               the stacked input gradient back to the upsampled
 //                           coarser tensor (2x2 sums) and the skip tensor
-//   k_dw                    weight gradient: for each tap a GEMM over pixels
-//                           on v_mfma_f32_16x16x4f32, partial sums per pixel
-//                           chunk, then k_dw_sum
-//   k_db                    bias gradient
+//   k_dw                    weight and bias gradients: for each tap a GEMM
+//                           over pixels on the matrix cores (BF16 or FP32
+//                           operands), partial sums per pixel chunk, then
+//                           k_dw_sum / k_db_sum
 //   k_learn_in / k_learn_out / k_loss   input encoding, decoding, L1 loss on
 //                           log radiance and its gradient
 //   k_adam                  Adam (Kingma & Ba 2015) with bias correction
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "learn.h"
 
@@ -28,8 +30,15 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 static inline uint32_t blocks(size_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
 // ---- weight layouts ---------------------------------------------------------------
-// master M[o][pc][tap], o < np (padded outputs), pc < cin (padded stacked inputs)
-__global__ void k_pack_fwd(const float* M, uint32_t np, uint32_t cin, float* W) {
+// master M[o][pc][tap], o < np (padded outputs), pc < cin (padded stacked
+// inputs), into k_conv's [chunk][tap][n][16] as float or bfloat16 bits
+__device__ __forceinline__ uint16_t bf16_round(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+template <bool BF16>
+__global__ void k_pack_fwd(const float* M, uint32_t np, uint32_t cin, void* W) {
     size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
     size_t total = (size_t)np * cin * 9;
     if (e >= total) return;
@@ -37,10 +46,15 @@ __global__ void k_pack_fwd(const float* M, uint32_t np, uint32_t cin, float* W) 
     uint32_t n = rest % np, rest2 = rest / np;
     uint32_t tap = rest2 % 9, chunk = rest2 / 9;
     uint32_t pc = chunk * 16 + q;
-    W[e] = M[((size_t)n * cin + pc) * 9 + tap];
+    float v = M[((size_t)n * cin + pc) * 9 + tap];
+    if (BF16)
+        static_cast<uint16_t*>(W)[e] = bf16_round(v);
+    else
+        static_cast<float*>(W)[e] = v;
 }
 // transposed: inputs are the np output channels, outputs the cin channels
-__global__ void k_pack_t(const float* M, uint32_t np, uint32_t cin, float* W) {
+template <bool BF16>
+__global__ void k_pack_t(const float* M, uint32_t np, uint32_t cin, void* W) {
     size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
     size_t total = (size_t)np * cin * 9;
     if (e >= total) return;
@@ -48,12 +62,21 @@ __global__ void k_pack_t(const float* M, uint32_t np, uint32_t cin, float* W) {
     uint32_t pc = rest % cin, rest2 = rest / cin;
     uint32_t tap = rest2 % 9, chunk = rest2 / 9;
     uint32_t o = chunk * 16 + q;
-    W[e] = M[((size_t)o * cin + pc) * 9 + (8 - tap)];
+    float v = M[((size_t)o * cin + pc) * 9 + (8 - tap)];
+    if (BF16)
+        static_cast<uint16_t*>(W)[e] = bf16_round(v);
+    else
+        static_cast<float*>(W)[e] = v;
 }
-void pack(hipStream_t s, const float* M, uint32_t np, uint32_t cin, float* fwd, float* t) {
+void pack(hipStream_t s, const float* M, uint32_t np, uint32_t cin, void* fwd, void* t, bool bf16) {
     size_t n = (size_t)np * cin * 9;
-    hipLaunchKernelGGL(k_pack_fwd, dim3(blocks(n, 256)), dim3(256), 0, s, M, np, cin, fwd);
-    hipLaunchKernelGGL(k_pack_t, dim3(blocks(n, 256)), dim3(256), 0, s, M, np, cin, t);
+    if (bf16) {
+        hipLaunchKernelGGL(k_pack_fwd<true>, dim3(blocks(n, 256)), dim3(256), 0, s, M, np, cin, fwd);
+        hipLaunchKernelGGL(k_pack_t<true>, dim3(blocks(n, 256)), dim3(256), 0, s, M, np, cin, t);
+    } else {
+        hipLaunchKernelGGL(k_pack_fwd<false>, dim3(blocks(n, 256)), dim3(256), 0, s, M, np, cin, fwd);
+        hipLaunchKernelGGL(k_pack_t<false>, dim3(blocks(n, 256)), dim3(256), 0, s, M, np, cin, t);
+    }
 }
 
 // ---- pooling and activations ------------------------------------------------------
@@ -142,101 +165,171 @@ void scatter(hipStream_t s, const float* dX, uint32_t b, uint32_t h, uint32_t w,
 }
 
 // ---- weight and bias gradients ------------------------------------------------------
-// One wave per (16 outputs, 16 inputs, pixel chunk). A segment is 16 pixels
-// of one row: dPre [16 px][16 o] and the input rows y-1..y+1 over 18 px [16 c]
-// go to LDS, then for each tap four MFMAs add the segment's 16 pixels:
-// A[o][k] = dPre[4g + k][o], B[k][c] = X[ky][4g + k + kx][c].
-__global__ __launch_bounds__(64) void k_dw(DwArgs a) {
-    __shared__ float D[16][17];
-    __shared__ float X[3][18][17];
-    const int lane = threadIdx.x, i = lane & 15, kk = lane >> 4;
-    const uint32_t otiles = a.np / 16;
-    const uint32_t ot = blockIdx.x % otiles, ct = blockIdx.x / otiles;
-    const uint32_t o0 = ot * 16, cc0 = ct * 16;
-    const bool from1 = cc0 >= a.c0;
-    const uint32_t h = a.h, w = a.w, segw = (w + 15) / 16;
+// dW_tap[o][c] = sum over pixels p of dPre[p][o] * X[p + offset(tap)][c]: per
+// tap a GEMM with M = outputs, N = stacked inputs and K = pixels, split
+// over K into chunks. A block (4 waves) takes a segment of 64 pixels of one
+// row at a time: dPre [64 px][np] and the input rows y-1..y+1 over 66 px
+// for the block's `ct` 16-channel input tiles are staged in LDS with float4
+// loads, then each wave adds the segment to its (16 outputs, 16 inputs)
+// tiles for the nine taps -- v_mfma_f32_16x16x16_bf16 with K = 16 pixels, or
+// four v_mfma_f32_16x16x4_f32 (A[o][k] = dPre[k][o], B[k][c] = X[k][c]).
+// Blocks of the first input group also sum dPre over the segment for the
+// bias gradient. Partial sums per chunk; k_dw_sum and k_db_sum add them.
+constexpr int SEG = 64;
+typedef short v4s __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ short bf16_of(float f) { return (short)bf16_round(f); }
+
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_dw(DwArgs a) {
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 15, kk = lane >> 4;
+    const uint32_t np = a.np, cin = a.c0 + a.c1, ot_n = np / 16, ct_all = cin / 16;
+    const uint32_t ct0 = blockIdx.x * a.ct, ctn = min(a.ct, ct_all - ct0), pairs = ot_n * ctn;
+    const uint32_t dstride = np + 4, xw = a.ct * 16 + 4;
+    float* D = lds;
+    float* X = lds + SEG * dstride;
+    const uint32_t h = a.h, w = a.w, segw = (w + SEG - 1) / SEG;
     const size_t segs = (size_t)a.batch * h * segw;
     const size_t s0 = segs * blockIdx.y / gridDim.y, s1 = segs * (blockIdx.y + 1) / gridDim.y;
-    v4f acc[9];
-    for (int t = 0; t < 9; ++t) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     const uint32_t sh = a.up ? h / 2 : h, sw = a.up ? w / 2 : w;
+    const bool bias_block = blockIdx.x == 0 && (uint32_t)tid < np;
+    float bsum = 0.0f;
+    v4f acc[2][9];
+    for (int p = 0; p < 2; ++p)
+        for (int t = 0; t < 9; ++t) acc[p][t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    const uint32_t q4 = ctn * 4;  // float4s per staged input pixel
     for (size_t sg = s0; sg < s1; ++sg) {
-        const uint32_t xs = (uint32_t)(sg % segw) * 16;
+        const uint32_t xs = (uint32_t)(sg % segw) * SEG;
         const uint32_t y = (uint32_t)((sg / segw) % h);
         const size_t b = sg / ((size_t)segw * h);
         __syncthreads();
-        for (int e = lane; e < 256; e += 64) {
-            int px = e >> 4, o = e & 15;
-            uint32_t x = xs + (uint32_t)px;
-            D[px][o] = x < w ? a.dpre[((b * h + y) * w + x) * a.np + o0 + o] : 0.0f;
+        for (uint32_t e = tid; e < SEG * np / 4; e += 256) {
+            uint32_t px = e / (np / 4), q = e % (np / 4), x = xs + px;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (x < w) v = *reinterpret_cast<const float4*>(a.dpre + ((b * h + y) * w + x) * np + 4 * q);
+            float* d = D + px * dstride + 4 * q;
+            d[0] = v.x, d[1] = v.y, d[2] = v.z, d[3] = v.w;
         }
-        for (int e = lane; e < 3 * 18 * 16; e += 64) {
-            int cc = e & 15, rest = e >> 4, px = rest % 18, r = rest / 18;
-            int yy = (int)y + r - 1, xx = (int)xs + px - 1;
-            float v = 0.0f;
+        for (uint32_t e = tid; e < 3 * (SEG + 2) * q4; e += 256) {
+            uint32_t q = e % q4, rest = e / q4, px = rest % (SEG + 2), r = rest / (SEG + 2);
+            int yy = (int)y + (int)r - 1, xx = (int)xs + (int)px - 1;
+            uint32_t cc = ct0 * 16 + 4 * q;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (yy >= 0 && yy < (int)h && xx >= 0 && xx < (int)w) {
-                if (!from1) {
+                if (cc < a.c0) {
                     int sy = a.up ? yy >> 1 : yy, sx = a.up ? xx >> 1 : xx;
-                    v = a.src0[((b * sh + (uint32_t)sy) * sw + (uint32_t)sx) * a.c0 + cc0 + cc];
+                    v = *reinterpret_cast<const float4*>(a.src0 + ((b * sh + (uint32_t)sy) * sw + (uint32_t)sx) * a.c0 + cc);
                 } else {
-                    v = a.src1[((b * h + (uint32_t)yy) * w + (uint32_t)xx) * a.c1 + (cc0 - a.c0) + cc];
+                    v = *reinterpret_cast<const float4*>(a.src1 + ((b * h + (uint32_t)yy) * w + (uint32_t)xx) * a.c1 +
+                                                         (cc - a.c0));
                 }
             }
-            X[r][px][cc] = v;
+            float* d = X + (r * (SEG + 2) + px) * xw + 4 * q;
+            d[0] = v.x, d[1] = v.y, d[2] = v.z, d[3] = v.w;
         }
         __syncthreads();
+        if (bias_block)
+            for (int px = 0; px < SEG; ++px) bsum += D[px * dstride + tid];
+        for (int s = 0; s < SEG / 16; ++s) {
+            if (xs + 16u * s >= w) break;
 #pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
-            const int ky = tap / 3, kx = tap - ky * 3;
+            for (int pi = 0; pi < 2; ++pi) {
+                const uint32_t p = (uint32_t)wave + 4u * pi;
+                if (p >= pairs) continue;
+                const uint32_t ot = p % ot_n, ctl = p / ot_n;
+                const float* dcol = D + ot * 16 + i;
+                const float* xcol = X + ctl * 16 + i;
+                if (BF16) {
+                    const int k0 = 16 * s + 4 * kk;
+                    const v4s av = v4s{bf16_of(dcol[(k0 + 0) * dstride]), bf16_of(dcol[(k0 + 1) * dstride]),
+                                       bf16_of(dcol[(k0 + 2) * dstride]), bf16_of(dcol[(k0 + 3) * dstride])};
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const float av = D[4 * g + kk][i];
-                const float bv = X[ky][4 * g + kk + kx][i];
-                acc[tap] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[tap], 0, 0, 0);
+                    for (int tap = 0; tap < 9; ++tap) {
+                        const int ky = tap / 3, kx = tap - ky * 3;
+                        const float* xr = xcol + (ky * (SEG + 2) + k0 + kx) * xw;
+                        const v4s bv = v4s{bf16_of(xr[0]), bf16_of(xr[xw]), bf16_of(xr[2 * xw]), bf16_of(xr[3 * xw])};
+                        acc[pi][tap] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[pi][tap], 0, 0, 0);
+                    }
+                } else {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const int k = 16 * s + 4 * g + kk;
+                        const float av = dcol[k * dstride];
+#pragma unroll
+                        for (int tap = 0; tap < 9; ++tap) {
+                            const int ky = tap / 3, kx = tap - ky * 3;
+                            const float bv = xcol[(ky * (SEG + 2) + k + kx) * xw];
+                            acc[pi][tap] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[pi][tap], 0, 0, 0);
+                        }
+                    }
+                }
             }
         }
     }
-    // lane: column c = i, rows o = 4 kk + v
-    const uint32_t cin = a.c0 + a.c1;
-    for (int tap = 0; tap < 9; ++tap)
-        for (int v = 0; v < 4; ++v)
-            a.part[(((size_t)blockIdx.y * 9 + tap) * a.np + o0 + 4 * kk + v) * cin + cc0 + i] = acc[tap][v];
+    // lane: input column c = i, output rows 4 kk + v
+#pragma unroll
+    for (int pi = 0; pi < 2; ++pi) {
+        const uint32_t p = (uint32_t)wave + 4u * pi;
+        if (p >= pairs) continue;
+        const uint32_t ot = p % ot_n, ctl = p / ot_n;
+        for (int tap = 0; tap < 9; ++tap)
+            for (int v = 0; v < 4; ++v)
+                a.part[(((size_t)blockIdx.y * 9 + tap) * np + ot * 16 + 4 * kk + v) * cin + (ct0 + ctl) * 16 + i] =
+                    acc[pi][tap][v];
+    }
+    if (bias_block) a.part_b[(size_t)blockIdx.y * np + tid] = bsum;
 }
-// dW[o][pc][tap] = sum over chunks
+// dW[o][pc][tap] = sum over chunks (threads walk the partials' own order,
+// so the chunk reads are coalesced)
 __global__ void k_dw_sum(const float* part, uint32_t chunks, uint32_t np, uint32_t cin, float* dW) {
     size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
     size_t n = (size_t)np * cin * 9;
     if (e >= n) return;
-    uint32_t tap = e % 9, rest = (uint32_t)(e / 9);
-    uint32_t pc = rest % cin, o = rest / cin;
+    uint32_t pc = e % cin, rest = (uint32_t)(e / cin);
+    uint32_t o = rest % np, tap = rest / np;
     float s = 0.0f;
-    for (uint32_t g = 0; g < chunks; ++g) s += part[(((size_t)g * 9 + tap) * np + o) * cin + pc];
-    dW[e] = s;
+    for (uint32_t g = 0; g < chunks; ++g) s += part[(size_t)g * n + e];
+    dW[((size_t)o * cin + pc) * 9 + tap] = s;
 }
-// bias: each block sums a pixel chunk for every output channel
-__global__ void k_db(const float* dpre, size_t pixels, uint32_t np, uint32_t chunks, float* part) {
-    uint32_t o = threadIdx.x;
-    if (o >= np) return;
-    size_t p0 = pixels * blockIdx.x / chunks, p1 = pixels * (blockIdx.x + 1) / chunks;
-    float s = 0.0f;
-    for (size_t p = p0; p < p1; ++p) s += dpre[p * np + o];
-    part[(size_t)blockIdx.x * np + o] = s;
-}
+// one block per output channel
 __global__ void k_db_sum(const float* part, uint32_t chunks, uint32_t np, float* db) {
-    uint32_t o = threadIdx.x;
-    if (o >= np) return;
+    __shared__ float red[256];
+    const uint32_t o = blockIdx.x;
     float s = 0.0f;
-    for (uint32_t g = 0; g < chunks; ++g) s += part[(size_t)g * np + o];
-    db[o] = s;
+    for (uint32_t g = threadIdx.x; g < chunks; g += 256) s += part[(size_t)g * np + o];
+    red[threadIdx.x] = s;
+    for (int k = 128; k > 0; k >>= 1) {
+        __syncthreads();
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    }
+    if (threadIdx.x == 0) db[o] = red[0];
 }
-void weight_grads(hipStream_t s, const DwArgs& a, uint32_t chunks, float* dW, float* db) {
+DwPlan dw_plan(uint32_t np, uint32_t cin, uint32_t batch, uint32_t h, uint32_t w) {
+    DwPlan p;
+    uint32_t ot_n = np / 16, ct_all = cin / 16;
+    p.ct = std::max(1u, std::min(ct_all, 4u / std::max(ot_n, 1u)));
+    p.groups = (ct_all + p.ct - 1) / p.ct;
+    size_t segs = (size_t)batch * h * ((w + SEG - 1) / SEG);
+    p.chunks = (uint32_t)std::max<size_t>(1, std::min<size_t>(segs, (1024 + p.groups - 1) / p.groups));
+    p.lds = (size_t)(SEG * (np + 4) + 3 * (SEG + 2) * (p.ct * 16 + 4)) * 4;
+    // each of the 4 waves holds at most two (output, input) tile pairs
+    if (ot_n * p.ct > 8 || p.lds > 64 * 1024) p.chunks = 0;
+    return p;
+}
+void weight_grads(hipStream_t s, const DwArgs& a, const DwPlan& p, bool bf16, float* dW, float* db) {
     uint32_t cin = a.c0 + a.c1;
-    hipLaunchKernelGGL(k_dw, dim3((a.np / 16) * (cin / 16), chunks), dim3(64), 0, s, a);
+    if (p.chunks == 0) return;  // the host refuses such layers
+    DwArgs b = a;
+    b.ct = p.ct;
+    dim3 grid(p.groups, p.chunks);
+    if (bf16)
+        hipLaunchKernelGGL(k_dw<true>, grid, dim3(256), p.lds, s, b);
+    else
+        hipLaunchKernelGGL(k_dw<false>, grid, dim3(256), p.lds, s, b);
     size_t n = (size_t)a.np * cin * 9;
-    hipLaunchKernelGGL(k_dw_sum, dim3(blocks(n, 256)), dim3(256), 0, s, a.part, chunks, a.np, cin, dW);
-    size_t pixels = (size_t)a.batch * a.h * a.w;
-    hipLaunchKernelGGL(k_db, dim3(chunks), dim3(128), 0, s, a.dpre, pixels, a.np, chunks, a.part);
-    hipLaunchKernelGGL(k_db_sum, dim3(1), dim3(128), 0, s, a.part, chunks, a.np, db);
+    hipLaunchKernelGGL(k_dw_sum, dim3(blocks(n, 256)), dim3(256), 0, s, a.part, p.chunks, a.np, cin, dW);
+    hipLaunchKernelGGL(k_db_sum, dim3(a.np), dim3(256), 0, s, a.part_b, p.chunks, a.np, db);
 }
 
 // ---- encoding, loss, decoding, batches ---------------------------------------------

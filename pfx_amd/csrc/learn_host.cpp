@@ -82,7 +82,6 @@ constexpr Def NET[PFXH_LEARN_LAYERS] = {
     {"dec_conv1b", 14, -1, 32, 0, false, false, true},  {"dec_conv0", 15, -1, 3, 0, false, false, false},
 };
 constexpr uint32_t TENSORS = PFXH_LEARN_LAYERS + 1;
-constexpr uint32_t DEPTH = 4;  // pooled levels: frames are padded to multiples of 16
 
 uint32_t channels(int t) { return t == 0 ? INPUT_CHANNELS : NET[t - 1].out; }
 uint32_t tensor_level(int t) { return t == 0 ? 0 : NET[t - 1].level + (NET[t - 1].pool ? 1u : 0u); }
@@ -146,14 +145,14 @@ struct pfxh_learner {
         float* t[TENSORS] = {};
         float* r[PFXH_LEARN_LAYERS] = {};  // pre-pool outputs (training)
         float* dt[TENSORS] = {};
-        float *dpre = nullptr, *dx = nullptr, *part = nullptr, *sums = nullptr;
+        float *dpre = nullptr, *dx = nullptr, *part = nullptr, *part_b = nullptr, *sums = nullptr;
         float4* target = nullptr;
         float4* images = nullptr;  // inference: colour, albedo, normal, output
         float* depth = nullptr;
         uint32_t* picks = nullptr;
         float* losses = nullptr;
         uint32_t pick_cap = 0, loss_cap = 0;
-        std::vector<uint32_t> chunks;
+        std::vector<pfxl::DwPlan> plans;
         std::vector<void*> mem;
     };
     Frame train_frame, infer_frame;
@@ -199,7 +198,7 @@ struct pfxh_learner {
     void repack() {
         for (uint32_t k = 0; k < PFXH_LEARN_LAYERS; ++k) {
             Shape s = shape(k);
-            pfxl::pack(stream, master + w_off[k], s.np, s.cin, wf[k], wt[k]);
+            pfxl::pack(stream, master + w_off[k], s.np, s.cin, wf[k], wt[k], bf16);
         }
     }
 
@@ -242,24 +241,24 @@ struct pfxh_learner {
             f.t[t] = (float*)dalloc(f.mem, n * 4, "activations");
             if (train && t > 0) f.dt[t] = (float*)dalloc(f.mem, n * 4, "activation gradients");
         }
-        size_t scratch_pre = 0, scratch_x = 0, part = 0;
+        size_t scratch_pre = 0, scratch_x = 0, part = 0, part_b = 0;
         for (uint32_t k = 0; k < PFXH_LEARN_LAYERS; ++k) {
             Shape s = shape(k);
             size_t lp = px >> (2 * NET[k].level);
             if (train && NET[k].pool) f.r[k] = (float*)dalloc(f.mem, lp * s.np * 4, "activations");
             scratch_pre = std::max(scratch_pre, lp * s.np);
             scratch_x = std::max(scratch_x, lp * s.cin);
-            uint32_t lh = h >> NET[k].level, lw = w >> NET[k].level;
-            size_t segments = (size_t)batch * lh * ((lw + 15) / 16);
-            uint32_t tiles = (s.np / 16) * (s.cin / 16);
-            uint32_t ch = (uint32_t)std::max<size_t>(1, std::min<size_t>(segments, (2048 + tiles - 1) / tiles));
-            f.chunks.push_back(ch);
-            part = std::max(part, (size_t)ch * 9 * s.np * s.cin);
+            pfxl::DwPlan pl = pfxl::dw_plan(s.np, s.cin, batch, h >> NET[k].level, w >> NET[k].level);
+            if (pl.chunks == 0) throw Failure{PFXH_SETUP, std::string(NET[k].name) + ": no weight-gradient launch fits"};
+            f.plans.push_back(pl);
+            part = std::max(part, (size_t)pl.chunks * 9 * s.np * s.cin);
+            part_b = std::max(part_b, (size_t)pl.chunks * s.np);
         }
         if (train) {
             f.dpre = (float*)dalloc(f.mem, scratch_pre * 4, "gradient scratch");
             f.dx = (float*)dalloc(f.mem, scratch_x * 4, "gradient scratch");
             f.part = (float*)dalloc(f.mem, part * 4, "gradient partial sums");
+            f.part_b = (float*)dalloc(f.mem, part_b * 4, "gradient partial sums");
             f.sums = (float*)dalloc(f.mem, ((px + 255) / 256) * 4, "loss sums");
             f.target = (float4*)dalloc(f.mem, px * 16, "targets");
         } else {
@@ -325,6 +324,7 @@ struct pfxh_learner {
             a.src0 = f.t[d.in0];
             a.src1 = d.in1 >= 0 ? f.t[d.in1] : nullptr;
             a.part = f.part;
+            a.part_b = f.part_b;
             a.c0 = s.c0;
             a.c1 = s.c1;
             a.np = s.np;
@@ -332,7 +332,8 @@ struct pfxh_learner {
             a.w = w;
             a.batch = f.batch;
             a.up = d.up ? 1u : 0u;
-            pfxl::weight_grads(stream, a, f.chunks[(size_t)k], grad + w_off[(size_t)k], grad + b_off[(size_t)k]);
+            a.ct = 0;
+            pfxl::weight_grads(stream, a, f.plans[(size_t)k], bf16, grad + w_off[(size_t)k], grad + b_off[(size_t)k]);
             float* d0 = d.in0 > 0 ? f.dt[d.in0] : nullptr;
             float* d1 = d.in1 > 0 ? f.dt[d.in1] : nullptr;
             if (!d0 && !d1) continue;
