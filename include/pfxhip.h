@@ -388,6 +388,58 @@ pfxh_status pfxh_denoise_plan(uint32_t height, uint32_t width, uint64_t max_tile
 /* Drop the denoiser and its device memory. NULL is allowed. */
 void pfxh_denoiser_free(pfxh_denoiser* denoiser);
 
+/* ---- batches of stills -------------------------------------------------------
+ * No counterpart in the reference, which renders one still per Trace; the
+ * nearest is the bake's probe batch, many camera faces of one scene in one
+ * dispatch (crates/bake/src/batch.rs:10-14,79-100). A batch renders many
+ * stills, each with its own scene, size, sample count and seed, with one
+ * launch per stage for all of them; the bounce loop of a wave is captured
+ * in a HIP graph. Every still's result equals that of its own pfxh_trace at
+ * the same seed, byte for byte. Calls on one batch, and on the traces
+ * pfxh_batch_trace lends, must be serialised by the caller. */
+
+typedef struct pfxh_batch pfxh_batch;
+
+typedef struct pfxh_batch_still {
+    const pfxh_scene* scene; /* read during pfxh_batch_new only */
+    uint32_t width, height;
+    uint32_t spp;  /* samples pfxh_batch_render adds */
+    uint32_t seed; /* the seed pfxh_trace_sample would be given */
+} pfxh_batch_still;
+
+/* Launch shape; pfxh_batch_tune picks it by timing trial renders. */
+typedef struct pfxh_batch_tuning {
+    uint32_t wave_paths;          /* paths per wave (>= the largest still's pixels) */
+    uint32_t refill;              /* idle lanes that make a traversal wave fetch rays, 1..64 */
+    uint32_t trace_blocks_per_cu; /* persistent traversal blocks per CU, 1..16 */
+    uint32_t shade_blocks_per_cu; /* shading blocks per CU, 1..64 */
+    uint32_t graph;               /* 1: replay the captured bounce loop, 0: launch directly */
+} pfxh_batch_tuning;
+
+/* Build every still's scene (one `detail` for all) and the shared wave. */
+pfxh_status pfxh_batch_new(const pfxh_batch_still* stills, uint32_t count, const pfxh_detail* detail,
+                           pfxh_batch** out, char* err, size_t err_len);
+/* Add each still's `spp` samples, as pfxh_trace_sample(trace, spp, seed)
+ * would on each still's own trace. */
+pfxh_status pfxh_batch_render(pfxh_batch* batch, char* err, size_t err_len);
+/* The same with counts[i] samples for still i. */
+pfxh_status pfxh_batch_render_samples(pfxh_batch* batch, const uint32_t* counts, char* err, size_t err_len);
+/* Autotune: time trial renders (an eighth of every still's samples) over
+ * wave sizes, refill thresholds and traversal blocks per CU, keep the
+ * fastest; the stills are left with zero samples. `out` may be NULL. */
+pfxh_status pfxh_batch_tune(pfxh_batch* batch, pfxh_batch_tuning* out, char* err, size_t err_len);
+void pfxh_batch_get_tuning(const pfxh_batch* batch, pfxh_batch_tuning* out);
+pfxh_status pfxh_batch_set_tuning(pfxh_batch* batch, const pfxh_batch_tuning* tuning, char* err,
+                                  size_t err_len);
+uint32_t pfxh_batch_count(const pfxh_batch* batch);
+/* Still `index` as a trace (readback, denoise, probe, sample), owned by the
+ * batch: do not free it. NULL when out of range. */
+pfxh_trace* pfxh_batch_trace(pfxh_batch* batch, uint32_t index);
+/* Rays and paths traced by pfxh_batch_render since the last reset. */
+pfxh_status pfxh_batch_stats(const pfxh_batch* batch, pfxh_stats* out);
+void pfxh_batch_reset_stats(pfxh_batch* batch);
+void pfxh_batch_free(pfxh_batch* batch);
+
 /* ---- the learned denoiser --------------------------------------------------
  * No counterpart in the reference, which runs only the fixed-weight filter
  * above (crates/denoise/src/lib.rs:15-17). A 5-level encoder-decoder of 16

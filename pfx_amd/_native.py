@@ -190,6 +190,16 @@ class CheckpointC(C.Structure):
 
 NO_ROUNDS = 0xFFFFFFFF
 
+class BatchStillC(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("seed", C.c_uint32)]
+
+
+class BatchTuningC(C.Structure):
+    _fields_ = [("wave_paths", C.c_uint32), ("refill", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
+                ("shade_blocks_per_cu", C.c_uint32), ("graph", C.c_uint32)]
+
+
 LEARN_LAYERS = 16
 LEARN_FP32 = 1
 
@@ -231,6 +241,17 @@ SIGNATURES = {
     "pfxh_denoiser_layers": (C.c_int, [_S, _Z, C.POINTER(_U), C.POINTER(C.c_uint64), _S, _Z]),
     "pfxh_denoise_plan": (C.c_int, [_U, _U, C.c_uint64, C.POINTER(_U), C.POINTER(_U), _P, _U, C.POINTER(_U)]),
     "pfxh_denoiser_free": (None, [_P]),
+    "pfxh_batch_new": (C.c_int, [_P, _U, _P, C.POINTER(_P), _S, _Z]),
+    "pfxh_batch_render": (C.c_int, [_P, _S, _Z]),
+    "pfxh_batch_render_samples": (C.c_int, [_P, _P, _S, _Z]),
+    "pfxh_batch_tune": (C.c_int, [_P, _P, _S, _Z]),
+    "pfxh_batch_get_tuning": (None, [_P, _P]),
+    "pfxh_batch_set_tuning": (C.c_int, [_P, _P, _S, _Z]),
+    "pfxh_batch_count": (_U, [_P]),
+    "pfxh_batch_trace": (_P, [_P, _U]),
+    "pfxh_batch_stats": (C.c_int, [_P, _P]),
+    "pfxh_batch_reset_stats": (None, [_P]),
+    "pfxh_batch_free": (None, [_P]),
     "pfxh_learn_layer": (C.c_int, [_U, _P]),
     "pfxh_learn_param_count": (C.c_uint64, []),
     "pfxh_learner_new": (C.c_int, [_U, C.POINTER(_P), _S, _Z]),

@@ -18,7 +18,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "_build"
 LIB = HERE / "libpfxhip.so"
-SOURCES = ["kernels.hip", "layout.cpp", "trace.cpp", "checkpoint.cpp", "denoise.hip", "denoise_host.cpp",
+SOURCES = ["kernels.hip", "layout.cpp", "trace.cpp", "checkpoint.cpp", "denoise.hip", "denoise_host.cpp", "batch.cpp",
            "learn.hip", "learn_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

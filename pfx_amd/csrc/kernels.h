@@ -55,6 +55,7 @@ struct WaveDev {
     uint32_t* q_sh;      // paths with NEE entries: index | entries << 28
     uint32_t* q_sray;    // shadow rays to test: path index | entry << 28
     uint32_t* counters;  // [0] next, [1] shadow paths, [2] shadow rays, [3..] stats
+    uint32_t* sid;       // batch: the still of each path
     uint2* spill;        // traversal stack overflow
     uint32_t capacity;   // paths per wave (entry arrays are strided by it)
     uint32_t spill_threads;
@@ -86,7 +87,36 @@ enum Counter : uint32_t {
     C_SPRIMS = 5 * SLOT + 6,
     C_FETCH = 8 * SLOT,                // PARTS work counters of k_extend
     C_SFETCH = C_FETCH + PARTS * SLOT, // PARTS work counters of k_shadow
-    C_COUNT = C_SFETCH + PARTS * SLOT
+    C_N = C_SFETCH + PARTS * SLOT,     // batch: rays, shadow rays, shadow paths of the bounce
+    C_STATS = C_N + SLOT,              // batch: 64-bit closest-hit and any-hit ray counts
+    C_COUNT = C_STATS + SLOT
+};
+
+// A batch renders many stills (own scene, size, sample count and seed each)
+// with one launch per stage. A wave of paths is a list of units, each a run
+// of samples of one still laid out as a single trace lays out its wave
+// (sample-major over the still's pixels), so every pixel's samples are
+// folded in sample order and each still's sums are those of its own trace.
+struct Unit {
+    uint32_t sid;     // still
+    uint32_t first;   // first path of the unit in the wave
+    uint32_t pfirst;  // first pixel slot of the unit in the wave (k_finish_batch)
+    uint32_t npix;    // pixels of the still
+    uint32_t ns;      // samples of each pixel in this unit
+    uint32_t acc;     // sample index of the first of them
+    uint32_t seed;
+    uint32_t pad;
+};
+constexpr uint32_t MAX_UNITS = 64;
+struct WavePlan {
+    uint32_t units, paths, pixels, pad;
+    Unit u[MAX_UNITS];
+};
+struct BatchDev {
+    const SceneDev* scenes;  // per still
+    const AccumDev* accums;  // per still
+    const WavePlan* waves;
+    uint32_t* wave;          // the wave being traced (advanced by k_next_wave)
 };
 
 constexpr int TRACE_BLOCK = 128;
@@ -112,5 +142,15 @@ void launch_measure(hipStream_t s, const AccumDev& A, float4* snaps, uint32_t wi
 void launch_mark(hipStream_t s, const AccumDev& A, const float4* snaps, uint32_t width, uint32_t height,
                  uint32_t min_samples, float threshold, uint32_t* running);
 void launch_zero(hipStream_t s, uint32_t* p, uint32_t n);
+
+void launch_camera_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid);
+void launch_roll(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t phase, uint32_t bounce);
+void launch_extend_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t grid);
+void launch_shade_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t bounce,
+                        uint32_t grid);
+void launch_shadow_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid);
+void launch_fold_batch(hipStream_t s, const WaveDev& W, uint32_t grid);
+void launch_finish_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid);
+void launch_next_wave(hipStream_t s, const BatchDev& B);
 
 }  // namespace pfxh

@@ -224,7 +224,8 @@ struct Closest {
 // Test the four children of a Node4, then continue with the first hit
 // child and push the other hit children right to left, so children pop in
 // the reference's left-first order. Returns false when no child was hit.
-DEV bool visit4(const SceneDev& S, uint32_t index, const Ray& R, float t, Stack& st, uint32_t& cur) {
+template <class G>
+DEV bool visit4(const G& S, uint32_t index, const Ray& R, float t, Stack& st, uint32_t& cur) {
     const float4* np = reinterpret_cast<const float4*>(S.nodes + index);
     float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
     uint4 rf = reinterpret_cast<const uint4*>(np)[6];
@@ -266,7 +267,8 @@ struct Walk {
     int inst_top;
 
     // Begin a ray; false when it misses the scene's root box.
-    DEV bool start(const SceneDev& S, v3 o, v3 d, float lim, Stack& st) {
+    template <class G>
+    DEV bool start(const G& S, v3 o, v3 d, float lim, Stack& st) {
         limit = lim;
         h.t = lim;
         h.u = 0.0f;
@@ -311,8 +313,8 @@ struct Walk {
     }
     // Test the current leaf's triangles in order; with ANY, true on the
     // first blocking hit.
-    template <bool COUNT>
-    DEV bool leaf(const SceneDev& S, uint32_t& prims_ctr) {
+    template <bool COUNT, class G>
+    DEV bool leaf(const G& S, uint32_t& prims_ctr) {
         bool skip = ANY ? (word & 2u) != 0u : (word & 1u) != 0u;  // casts-no-shadow / shadow-only
         if (skip) return false;
         uint32_t first = base + (cur & REF_FIRST_MASK);
@@ -352,7 +354,8 @@ struct Walk {
     }
     // Enter the current instance: move the ray into its object space and
     // continue at its bottom-level root; false when its box is missed.
-    DEV bool enter(const SceneDev& S, Stack& st) {
+    template <class G>
+    DEV bool enter(const G& S, Stack& st) {
         const Group& g = S.groups[cur & REF_INDEX_MASK];
         base = g.base;
         rec = g.record;
@@ -371,13 +374,13 @@ struct Walk {
     // One step: an inner node, a leaf or an instance, then the next deferred
     // child if needed. Returns false when the walk is over; `blocked` is
     // set when an any-hit walk found an occluder.
-    template <bool COUNT>
-    DEV bool step(const SceneDev& S, Stack& st, bool& blocked, uint32_t& nodes_ctr, uint32_t& prims_ctr) {
+    template <bool COUNT, class G>
+    DEV bool step(const G& S, Stack& st, bool& blocked, uint32_t& nodes_ctr, uint32_t& prims_ctr) {
         if (inner()) {
             if (COUNT) nodes_ctr++;
             if (visit4(S, cur, R, h.t, st, cur)) return true;
         } else if (leafy()) {
-            if (leaf<COUNT>(S, prims_ctr)) {
+            if (leaf<COUNT, G>(S, prims_ctr)) {
                 blocked = true;
                 return false;
             }
@@ -389,14 +392,34 @@ struct Walk {
 };
 
 // Whole walks for one ray (used where lanes do not refill).
-DEV Closest trace_closest(const SceneDev& S, v3 o, v3 d, float tmax, Stack& st) {
+template <class G>
+DEV Closest trace_closest(const G& S, v3 o, v3 d, float tmax, Stack& st) {
     Walk<false> w;
     uint32_t nodes = 0, prims = 0;
     bool blocked = false;
     if (w.start(S, o, d, tmax, st))
-        while (w.step<false>(S, st, blocked, nodes, prims)) {
+        while (w.template step<false, G>(S, st, blocked, nodes, prims)) {
         }
     return w.h;
+}
+
+// What a walk reads of a scene. In a batch each lane walks its own still's
+// scene, so a lane keeps this much of it in registers; a single trace walks
+// with the uniform SceneDev itself.
+struct Geo {
+    const Node4* nodes;
+    const TriPos* tris;
+    const Group* groups;
+    RootEntry root;
+    uint32_t bounds_tame;
+};
+DEV Geo geo_of(const SceneDev& S) { return Geo{S.nodes, S.tris, S.groups, S.root, S.bounds_tame}; }
+template <bool BATCH>
+DEV decltype(auto) walk_scene(const SceneDev& S, const Geo& g) {
+    if constexpr (BATCH)
+        return (g);
+    else
+        return (S);
 }
 
 // Persistent waves refill finished lanes. The queue [0, n) is split into
@@ -946,16 +969,12 @@ DEV void project(const SceneDev& S, float ndcx, float ndcy, v3 fwd, v3 right, v3
     }
 }
 
-__global__ __launch_bounds__(256) void k_camera(SceneDev S, WaveDev W, const uint32_t* active, uint32_t npix,
-                                                uint32_t nsamples, uint32_t acc, uint32_t seed) {
-    uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= npix * nsamples) return;
-    uint32_t s = i / npix, p = i - s * npix;
-    uint32_t pix = active ? active[p] : p;
+// Start path i at pixel `pix` of the still with sample index `sample`.
+DEV void camera_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t pix, uint32_t sample, uint32_t seed) {
     uint32_t y = pix / S.width, x = pix - y * S.width;
     uint32_t px = x + S.window[2], py = y + S.window[3];
     uint32_t stream = py * S.window[0] + px;
-    uint32_t state = pfx::pcg(stream * 1973u + pfx::pcg(seed * 9277u + (acc + s) * 7919u + 26699u));
+    uint32_t state = pfx::pcg(stream * 1973u + pfx::pcg(seed * 9277u + sample * 7919u + 26699u));
     float jx = pfx::rnd(&state);
     float jy = pfx::rnd(&state);
     float ndcx = ((float)px + jx) / (float)S.window[0] * 2.0f - 1.0f;
@@ -981,6 +1000,42 @@ __global__ __launch_bounds__(256) void k_camera(SceneDev S, WaveDev W, const uin
     W.nrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+__global__ __launch_bounds__(256) void k_camera(SceneDev S, WaveDev W, const uint32_t* active, uint32_t npix,
+                                                uint32_t nsamples, uint32_t acc, uint32_t seed) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= npix * nsamples) return;
+    uint32_t s = i / npix, p = i - s * npix;
+    uint32_t pix = active ? active[p] : p;
+    camera_path(S, W, i, pix, acc + s, seed);
+}
+
+// The unit of a batch wave that holds path (or pixel slot) k: units are in
+// increasing order of both.
+DEV uint32_t unit_of(const WavePlan& P, uint32_t k, bool by_pixel) {
+    uint32_t lo = 0, hi = P.units;  // the last unit whose first is <= k
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        uint32_t first = by_pixel ? P.u[mid].pfirst : P.u[mid].first;
+        if (first <= k)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Batch: every path of the current wave, each unit a run of samples of one
+// still (sample-major over its pixels, as a single trace lays out a wave).
+__global__ __launch_bounds__(256) void k_camera_batch(BatchDev B, WaveDev W) {
+    const WavePlan& P = B.waves[*B.wave];
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < P.paths; i += gridDim.x * 256u) {
+        const Unit& u = P.u[unit_of(P, i, false)];
+        uint32_t local = i - u.first, s = local / u.npix, pix = local - s * u.npix;
+        W.sid[i] = u.sid;
+        camera_path(B.scenes[u.sid], W, i, pix, u.acc + s, u.seed);
+    }
+}
+
 DEV void store_closest(const WaveDev& W, uint32_t i, const Closest& h) {
     W.hit[i] = make_float4(h.t, h.u, h.v, ubits(h.slot));
     W.hit_rec[i] = h.rec;
@@ -993,13 +1048,18 @@ DEV void store_closest(const WaveDev& W, uint32_t i, const Closest& h) {
 // Six waves per SIMD: the walk is bound by dependent node fetches, and the
 // extra resident waves hide more of that latency than the few registers
 // spilled at this limit cost.
-template <bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(SceneDev S, WaveDev W, const uint32_t* queue, uint32_t n) {
+// In a batch (BATCH) each path belongs to a still of its own (W.sid) whose
+// scene is scenes[sid], and the queue length is the one k_roll left in the
+// counters.
+template <bool COUNT, bool BATCH>
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(SceneDev S, const SceneDev* scenes, WaveDev W, const uint32_t* queue, uint32_t n) {
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
     uint32_t nodes = 0, prims = 0;
     Walk<false> w;
+    Geo g{};
+    if (BATCH) n = W.counters[C_N];
     uint32_t ray = 0;
     bool live = false;
     Fetch f = fetch_start();
@@ -1008,7 +1068,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         if (k != 0xffffffffu) {
             ray = queue ? queue[k] : k;
             float4 o4 = W.o[ray], d4 = W.d[ray];
-            live = w.start(S, f3(o4), f3(d4), 1e20f, st);
+            if (BATCH) g = geo_of(scenes[W.sid[ray]]);
+            live = w.start(walk_scene<BATCH>(S, g), f3(o4), f3(d4), 1e20f, st);
             if (!live) store_closest(W, ray, w.h);
         }
         if (!__any(live)) {
@@ -1022,7 +1083,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         if (inner_round(in, out)) {
             if (in) {
                 if (COUNT) nodes++;
-                if (!visit4(S, w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) {
+                if (!visit4(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) {
                     store_closest(W, ray, w.h);
                     live = false;
                 }
@@ -1030,9 +1091,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         } else if (out) {
             bool go = true;
             if (w.leafy())
-                w.template leaf<COUNT>(S, prims);
+                w.template leaf<COUNT>(walk_scene<BATCH>(S, g), prims);
             else
-                go = !w.enter(S, st);
+                go = !w.enter(walk_scene<BATCH>(S, g), st);
             if (go && !w.next(st)) {
                 store_closest(W, ray, w.h);
                 live = false;
@@ -1046,183 +1107,192 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
 }
 
 // One path vertex (trace.wgsl:2169-2501 loop body).
-__global__ __launch_bounds__(256) void k_shade(SceneDev S, WaveDev W, const uint32_t* queue, uint32_t n,
-                                               uint32_t bounce) {
-    uint32_t k = blockIdx.x * 256u + threadIdx.x;
+DEV void shade_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t bounce, bool& cont, uint32_t& entries,
+                    uint32_t& tested) {
+    float4 o4 = W.o[i], d4 = W.d[i], t4 = W.thr[i], r4 = W.res[i], hv = W.hit[i];
+    v3 o = f3(o4), d = f3(d4), throughput = f3(t4), result = f3(r4);
+    float previous_pdf = o4.w;
+    uint32_t lobes = fbits(d4.w);
+    uint32_t state = fbits(t4.w);
+    float t = hv.x;
+    bool done = false;
+    if (t >= 1e19f) {
+        if (!S.env_black) {
+            float pdf = sky_pdf(S, d);
+            float weight = (bounce > 0u && previous_pdf > 0.0f) ? previous_pdf / (previous_pdf + pdf) : 1.0f;
+            result = result + throughput * env_radiance(S, d) * weight;
+        }
+        done = true;
+    }
+    if (!done) {
+        uint32_t slot = fbits(hv.w), rec = W.hit_rec[i];
+        Surf sf = surface_at(S, slot, rec, hv.y, hv.z);
+        v3 p = o + d * t;
+        v3 origin_point = pfx::spawn_point(o, -d, o, 0.0f);
+        v3 n = dot(sf.n, d) > 0.0f ? -sf.n : sf.n;
+        Mat s = load_mat(S, sf.material);
+        v3 geometric = dot(sf.geometric, d) > 0.0f ? -sf.geometric : sf.geometric;
+        if (dot(n, geometric) < 0.05f) n = normalize(n + geometric * 0.1f);
+        s.roughness = fmaxf(s.roughness, 0.05f);
+        s.clearcoat_roughness = fmaxf(s.clearcoat_roughness, 0.05f);
+        if (bounce == 0u) {
+            W.alb[i] = f4(s.base, 1.0f);
+            W.nrm[i] = f4(n, 1.0f);
+        }
+        // emitter_weight (lights.wgsl:38-44)
+        if (S.lit) {
+            float weight = 1.0f;
+            bool emissive = !(s.emission.x <= 0.0f && s.emission.y <= 0.0f && s.emission.z <= 0.0f);
+            if (previous_pdf > 0.0f && emissive) {
+                uint32_t index = S.attrs[slot].emitter;
+                if (index != 0u) {
+                    float4 ee = row(S.emitters, index, 3), ec = row(S.emitters, index, 2);
+                    v3 to = p - origin_point;
+                    float distance2 = dot(to, to);
+                    float cosine = fabsf(dot(sf.geometric, to)) / sqrtf(fmaxf(distance2, 1e-30f));
+                    float epdf = ee.w / fmaxf(ec.w, 1e-30f) * distance2 / fmaxf(cosine, 1e-7f);
+                    weight = mis_power(previous_pdf, epdf);
+                }
+            }
+            s.emission = s.emission * weight;
+        }
+        result = result + throughput * s.emission;
+        if (lobes & LAST_BIT) {
+            done = true;
+        } else {
+            m3 T = basis(n);
+            v3 wo = pfx::tmul(T, -d);
+            v3 origin = pfx::spawn_point(p, n, o, t);
+            uint32_t slots = 2u + S.light_slots;
+            for (uint32_t kk = 0; kk < slots; ++kk) {
+                v3 local = mk(0, 0, 1), direction = mk(0, 0, 1), strength = mk(0, 0, 0);
+                float limit = 1e19f, pdf = 0.0f;
+                Mat lit = s;
+                if (kk == 0u) {
+                    if (S.sun_intensity <= 0.0f) continue;
+                    direction = normalize(mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]));
+                    if (S.sun_w < 1.0f) {
+                        float height = 1.0f - pfx::rnd(&state) * (1.0f - S.sun_w);
+                        float radius = sqrtf(fmaxf(0.0f, 1.0f - height * height));
+                        float phi = 2.0f * PI * pfx::rnd(&state);
+                        m3 orient = basis(direction);
+                        direction =
+                            normalize(pfx::mul(orient, mk(radius * fm_cos(phi), radius * fm_sin(phi), height)));
+                    }
+                    local = pfx::tmul(T, direction);
+                    if (local.z <= 0.0f) continue;
+                    strength = mk(S.sun_color[0], S.sun_color[1], S.sun_color[2]) * (S.sun_intensity * 1.0f);
+                } else if (kk == 1u) {
+                    float rx = pfx::rnd(&state);
+                    float ry = pfx::rnd(&state);
+                    if (S.env_black) continue;  // strength 0: the term adds nothing
+                    float spdf;
+                    sky_sample(S, rx, ry, direction, spdf);
+                    local = pfx::tmul(T, direction);
+                    if (local.z <= 0.0f) continue;
+                    strength = env_radiance(S, direction) * 1.0f;
+                    pdf = spdf;
+                    if (!(pdf > 0.0f)) continue;
+                } else {
+                    Cand c = direct_light(S, kk - 2u, T, p, origin, geometric, &state);
+                    if (c.limit < 0.0f) continue;
+                    local = c.wi;
+                    direction = c.direction;
+                    strength = c.scale;
+                    pdf = c.pdf;
+                    limit = c.limit;
+                    lit.roughness = fmaxf(s.roughness, c.roughness);
+                }
+                if (pdf > 0.0f) {
+                    float bsdf = bsdf_pdf(s, wo, local);
+                    if (kk == 1u) {
+                        strength = strength / (pdf + bsdf);
+                    } else {
+                        float ratio = bsdf / pdf;
+                        strength = strength / (pdf * (1.0f + ratio * ratio));
+                    }
+                }
+                v3 add = throughput * eval(lit, wo, local) * 1.0f * local.z * strength;
+                if (add.x == 0.0f && add.y == 0.0f && add.z == 0.0f) continue;
+                if (entries < S.entries) {
+                    size_t e = (size_t)entries * W.capacity + i;
+                    W.e_o[e] = f4(origin, limit > 0.0f ? limit : 0.0f);
+                    W.e_d[e] = f4(direction, 0.0f);
+                    W.e_c[e] = f4(add, 0.0f);
+                    if (limit > 0.0f) tested |= 1u << entries;
+                    entries++;
+                }
+            }
+            // sample_bsdf (trace.wgsl:1716-1773), diffuse + GGX + coat
+            (void)pfx::rnd(&state);  // transmission pick: never below zero transmission
+            float cw = coat_weight(s, wo);
+            float sw = (1.0f - cw) * fm_clamp(s.metalness + s.specular, 0.0f, 0.9f);
+            float pick = pfx::rnd(&state);
+            v3 wi;
+            uint32_t lobe;
+            if (pick < cw + sw) {
+                bool coat = pick < cw;
+                float alpha = alpha_rough(coat ? s.clearcoat_roughness : s.roughness);
+                float u1 = pfx::rnd(&state);
+                float u2 = pfx::rnd(&state);
+                float phi = 2.0f * PI * u1;
+                float cos_theta = sqrtf((1.0f - u2) / (1.0f + (alpha * alpha - 1.0f) * u2));
+                float stt = sqrtf(fmaxf(0.0f, 1.0f - cos_theta * cos_theta));
+                v3 h = mk(fm_cos(phi) * stt, fm_sin(phi) * stt, cos_theta);
+                wi = reflect(-wo, h);
+                lobe = 1u;
+            } else {
+                float u1 = pfx::rnd(&state);
+                float u2 = pfx::rnd(&state);
+                float phi = 2.0f * PI * u1;
+                float r = sqrtf(u2);
+                wi = mk(fm_cos(phi) * r, fm_sin(phi) * r, sqrtf(1.0f - u2));
+                lobe = 0u;
+            }
+            float spdf = bsdf_pdf(s, wo, wi);
+            uint32_t dif = (lobes & 0xffu) + (lobe == 0u ? 1u : 0u);
+            uint32_t glo = ((lobes >> 8) & 0xffu) + (lobe == 1u ? 1u : 0u);
+            uint32_t tra = (lobes >> 16) & 0xffu;
+            bool last = bounce >= S.total || dif > S.diffuse || glo > S.glossy || tra > S.transmission;
+            lobes = (dif & 0xffu) | ((glo & 0xffu) << 8) | ((tra & 0xffu) << 16) | (last ? LAST_BIT : 0u);
+            if (wi.z <= 0.0f || spdf <= 0.0f) {
+                done = true;
+            } else {
+                throughput = throughput * (eval(s, wo, wi) * 1.0f * wi.z / spdf * 1.0f);
+                d = normalize(pfx::mul(T, wi));
+                previous_pdf = spdf;
+                o = pfx::spawn_point(p, d, o, t);
+                if (bounce >= 3u) {
+                    float keep = fm_clamp(fmaxf(throughput.x, fmaxf(throughput.y, throughput.z)), 0.05f, 0.95f);
+                    if (pfx::rnd(&state) > keep)
+                        done = true;
+                    else
+                        throughput = throughput / keep;
+                }
+            }
+        }
+    }
+    cont = !done;
+    W.o[i] = f4(o, previous_pdf);
+    W.d[i] = f4(d, ubits(lobes));
+    W.thr[i] = f4(throughput, ubits(state));
+    W.res[i] = f4(result, 0.0f);
+}
+
+// Queue position k of the bounce (every thread of the block calls it).
+template <bool BATCH>
+DEV void shade_step(const SceneDev& S, const SceneDev* scenes, const WaveDev& W, const uint32_t* queue, uint32_t n,
+                    uint32_t bounce, uint32_t k) {
     bool valid = k < n;
     bool cont = false;
     uint32_t entries = 0, tested = 0;
     uint32_t i = 0;
     if (valid) {
         i = queue ? queue[k] : k;
-        float4 o4 = W.o[i], d4 = W.d[i], t4 = W.thr[i], r4 = W.res[i], hv = W.hit[i];
-        v3 o = f3(o4), d = f3(d4), throughput = f3(t4), result = f3(r4);
-        float previous_pdf = o4.w;
-        uint32_t lobes = fbits(d4.w);
-        uint32_t state = fbits(t4.w);
-        float t = hv.x;
-        bool done = false;
-        if (t >= 1e19f) {
-            if (!S.env_black) {
-                float pdf = sky_pdf(S, d);
-                float weight = (bounce > 0u && previous_pdf > 0.0f) ? previous_pdf / (previous_pdf + pdf) : 1.0f;
-                result = result + throughput * env_radiance(S, d) * weight;
-            }
-            done = true;
-        }
-        if (!done) {
-            uint32_t slot = fbits(hv.w), rec = W.hit_rec[i];
-            Surf sf = surface_at(S, slot, rec, hv.y, hv.z);
-            v3 p = o + d * t;
-            v3 origin_point = pfx::spawn_point(o, -d, o, 0.0f);
-            v3 n = dot(sf.n, d) > 0.0f ? -sf.n : sf.n;
-            Mat s = load_mat(S, sf.material);
-            v3 geometric = dot(sf.geometric, d) > 0.0f ? -sf.geometric : sf.geometric;
-            if (dot(n, geometric) < 0.05f) n = normalize(n + geometric * 0.1f);
-            s.roughness = fmaxf(s.roughness, 0.05f);
-            s.clearcoat_roughness = fmaxf(s.clearcoat_roughness, 0.05f);
-            if (bounce == 0u) {
-                W.alb[i] = f4(s.base, 1.0f);
-                W.nrm[i] = f4(n, 1.0f);
-            }
-            // emitter_weight (lights.wgsl:38-44)
-            if (S.lit) {
-                float weight = 1.0f;
-                bool emissive = !(s.emission.x <= 0.0f && s.emission.y <= 0.0f && s.emission.z <= 0.0f);
-                if (previous_pdf > 0.0f && emissive) {
-                    uint32_t index = S.attrs[slot].emitter;
-                    if (index != 0u) {
-                        float4 ee = row(S.emitters, index, 3), ec = row(S.emitters, index, 2);
-                        v3 to = p - origin_point;
-                        float distance2 = dot(to, to);
-                        float cosine = fabsf(dot(sf.geometric, to)) / sqrtf(fmaxf(distance2, 1e-30f));
-                        float epdf = ee.w / fmaxf(ec.w, 1e-30f) * distance2 / fmaxf(cosine, 1e-7f);
-                        weight = mis_power(previous_pdf, epdf);
-                    }
-                }
-                s.emission = s.emission * weight;
-            }
-            result = result + throughput * s.emission;
-            if (lobes & LAST_BIT) {
-                done = true;
-            } else {
-                m3 T = basis(n);
-                v3 wo = pfx::tmul(T, -d);
-                v3 origin = pfx::spawn_point(p, n, o, t);
-                uint32_t slots = 2u + S.light_slots;
-                for (uint32_t kk = 0; kk < slots; ++kk) {
-                    v3 local = mk(0, 0, 1), direction = mk(0, 0, 1), strength = mk(0, 0, 0);
-                    float limit = 1e19f, pdf = 0.0f;
-                    Mat lit = s;
-                    if (kk == 0u) {
-                        if (S.sun_intensity <= 0.0f) continue;
-                        direction = normalize(mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]));
-                        if (S.sun_w < 1.0f) {
-                            float height = 1.0f - pfx::rnd(&state) * (1.0f - S.sun_w);
-                            float radius = sqrtf(fmaxf(0.0f, 1.0f - height * height));
-                            float phi = 2.0f * PI * pfx::rnd(&state);
-                            m3 orient = basis(direction);
-                            direction =
-                                normalize(pfx::mul(orient, mk(radius * fm_cos(phi), radius * fm_sin(phi), height)));
-                        }
-                        local = pfx::tmul(T, direction);
-                        if (local.z <= 0.0f) continue;
-                        strength = mk(S.sun_color[0], S.sun_color[1], S.sun_color[2]) * (S.sun_intensity * 1.0f);
-                    } else if (kk == 1u) {
-                        float rx = pfx::rnd(&state);
-                        float ry = pfx::rnd(&state);
-                        if (S.env_black) continue;  // strength 0: the term adds nothing
-                        float spdf;
-                        sky_sample(S, rx, ry, direction, spdf);
-                        local = pfx::tmul(T, direction);
-                        if (local.z <= 0.0f) continue;
-                        strength = env_radiance(S, direction) * 1.0f;
-                        pdf = spdf;
-                        if (!(pdf > 0.0f)) continue;
-                    } else {
-                        Cand c = direct_light(S, kk - 2u, T, p, origin, geometric, &state);
-                        if (c.limit < 0.0f) continue;
-                        local = c.wi;
-                        direction = c.direction;
-                        strength = c.scale;
-                        pdf = c.pdf;
-                        limit = c.limit;
-                        lit.roughness = fmaxf(s.roughness, c.roughness);
-                    }
-                    if (pdf > 0.0f) {
-                        float bsdf = bsdf_pdf(s, wo, local);
-                        if (kk == 1u) {
-                            strength = strength / (pdf + bsdf);
-                        } else {
-                            float ratio = bsdf / pdf;
-                            strength = strength / (pdf * (1.0f + ratio * ratio));
-                        }
-                    }
-                    v3 add = throughput * eval(lit, wo, local) * 1.0f * local.z * strength;
-                    if (add.x == 0.0f && add.y == 0.0f && add.z == 0.0f) continue;
-                    if (entries < S.entries) {
-                        size_t e = (size_t)entries * W.capacity + i;
-                        W.e_o[e] = f4(origin, limit > 0.0f ? limit : 0.0f);
-                        W.e_d[e] = f4(direction, 0.0f);
-                        W.e_c[e] = f4(add, 0.0f);
-                        if (limit > 0.0f) tested |= 1u << entries;
-                        entries++;
-                    }
-                }
-                // sample_bsdf (trace.wgsl:1716-1773), diffuse + GGX + coat
-                (void)pfx::rnd(&state);  // transmission pick: never below zero transmission
-                float cw = coat_weight(s, wo);
-                float sw = (1.0f - cw) * fm_clamp(s.metalness + s.specular, 0.0f, 0.9f);
-                float pick = pfx::rnd(&state);
-                v3 wi;
-                uint32_t lobe;
-                if (pick < cw + sw) {
-                    bool coat = pick < cw;
-                    float alpha = alpha_rough(coat ? s.clearcoat_roughness : s.roughness);
-                    float u1 = pfx::rnd(&state);
-                    float u2 = pfx::rnd(&state);
-                    float phi = 2.0f * PI * u1;
-                    float cos_theta = sqrtf((1.0f - u2) / (1.0f + (alpha * alpha - 1.0f) * u2));
-                    float stt = sqrtf(fmaxf(0.0f, 1.0f - cos_theta * cos_theta));
-                    v3 h = mk(fm_cos(phi) * stt, fm_sin(phi) * stt, cos_theta);
-                    wi = reflect(-wo, h);
-                    lobe = 1u;
-                } else {
-                    float u1 = pfx::rnd(&state);
-                    float u2 = pfx::rnd(&state);
-                    float phi = 2.0f * PI * u1;
-                    float r = sqrtf(u2);
-                    wi = mk(fm_cos(phi) * r, fm_sin(phi) * r, sqrtf(1.0f - u2));
-                    lobe = 0u;
-                }
-                float spdf = bsdf_pdf(s, wo, wi);
-                uint32_t dif = (lobes & 0xffu) + (lobe == 0u ? 1u : 0u);
-                uint32_t glo = ((lobes >> 8) & 0xffu) + (lobe == 1u ? 1u : 0u);
-                uint32_t tra = (lobes >> 16) & 0xffu;
-                bool last = bounce >= S.total || dif > S.diffuse || glo > S.glossy || tra > S.transmission;
-                lobes = (dif & 0xffu) | ((glo & 0xffu) << 8) | ((tra & 0xffu) << 16) | (last ? LAST_BIT : 0u);
-                if (wi.z <= 0.0f || spdf <= 0.0f) {
-                    done = true;
-                } else {
-                    throughput = throughput * (eval(s, wo, wi) * 1.0f * wi.z / spdf * 1.0f);
-                    d = normalize(pfx::mul(T, wi));
-                    previous_pdf = spdf;
-                    o = pfx::spawn_point(p, d, o, t);
-                    if (bounce >= 3u) {
-                        float keep = fm_clamp(fmaxf(throughput.x, fmaxf(throughput.y, throughput.z)), 0.05f, 0.95f);
-                        if (pfx::rnd(&state) > keep)
-                            done = true;
-                        else
-                            throughput = throughput / keep;
-                    }
-                }
-            }
-        }
-        cont = !done;
-        W.o[i] = f4(o, previous_pdf);
-        W.d[i] = f4(d, ubits(lobes));
-        W.thr[i] = f4(throughput, ubits(state));
-        W.res[i] = f4(result, 0.0f);
+        if (BATCH)
+            shade_path(scenes[W.sid[i]], W, i, bounce, cont, entries, tested);
+        else
+            shade_path(S, W, i, bounce, cont, entries, tested);
     }
     bool sh = valid && entries > 0u;
     uint3 at = block_append3(W.counters, cont ? 1u : 0u, sh ? 1u : 0u, (uint32_t)__popc(tested));
@@ -1232,15 +1302,31 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, WaveDev W, const uint
         W.q_sray[r] = ((uint32_t)(__ffs(bits) - 1) << 28) | i;
 }
 
+// In a batch (BATCH) the path's scene is scenes[W.sid[i]] and the queue
+// length comes from the counters; blocks then stride over the queue.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_shade(SceneDev S, const SceneDev* scenes, WaveDev W, const uint32_t* queue,
+                                               uint32_t n, uint32_t bounce) {
+    if (!BATCH) {
+        shade_step<false>(S, scenes, W, queue, n, bounce, blockIdx.x * 256u + threadIdx.x);
+        return;
+    }
+    n = W.counters[C_N];
+    for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u)
+        shade_step<true>(S, scenes, W, queue, n, bounce, base + threadIdx.x);
+}
+
 // Any-hit test of every queued shadow ray (one NEE entry each). An occluded
 // entry is marked in its contribution's w; k_fold adds the others.
-template <bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(SceneDev S, WaveDev W, uint32_t n) {
+template <bool COUNT, bool BATCH>
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(SceneDev S, const SceneDev* scenes, WaveDev W, uint32_t n) {
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
     uint32_t nodes = 0, prims = 0;
     Walk<true> w;
+    Geo g{};
+    if (BATCH) n = W.counters[C_N + 1];
     size_t at = 0;
     bool live = false;
     Fetch f = fetch_start();
@@ -1250,7 +1336,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
             uint32_t q = W.q_sray[k];
             at = (size_t)(q >> 28) * W.capacity + (q & 0x0fffffffu);
             float4 eo = W.e_o[at], ed = W.e_d[at];
-            live = w.start(S, f3(eo), f3(ed), eo.w, st);
+            if (BATCH) g = geo_of(scenes[W.sid[q & 0x0fffffffu]]);
+            live = w.start(walk_scene<BATCH>(S, g), f3(eo), f3(ed), eo.w, st);
         }
         if (!__any(live)) {
             if (fetch_drained(f)) break;
@@ -1261,18 +1348,18 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         if (inner_round(in, out)) {
             if (in) {
                 if (COUNT) nodes++;
-                if (!visit4(S, w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) live = false;
+                if (!visit4(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) live = false;
             }
         } else if (out) {
             bool go = true;
             if (w.leafy()) {
-                if (w.template leaf<COUNT>(S, prims)) {
+                if (w.template leaf<COUNT>(walk_scene<BATCH>(S, g), prims)) {
                     reinterpret_cast<float*>(W.e_c + at)[3] = 1.0f;
                     live = false;
                     go = false;
                 }
             } else {
-                go = !w.enter(S, st);
+                go = !w.enter(walk_scene<BATCH>(S, g), st);
             }
             if (go && !w.next(st)) live = false;
         }
@@ -1285,17 +1372,19 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
 
 // Add each queued path's unoccluded light contributions in slot order
 // (trace.wgsl:2465-2470).
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_fold(WaveDev W, uint32_t n) {
-    uint32_t k = blockIdx.x * 256u + threadIdx.x;
-    if (k >= n) return;
-    uint32_t q = W.q_sh[k];
-    uint32_t i = q & 0x0fffffffu, m = q >> 28;
-    v3 result = f3(W.res[i]);
-    for (uint32_t e = 0; e < m; ++e) {
-        float4 c = W.e_c[(size_t)e * W.capacity + i];
-        if (c.w == 0.0f) result = result + f3(c);
+    if (BATCH) n = W.counters[C_N + 2];
+    for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < n; k += gridDim.x * 256u) {
+        uint32_t q = W.q_sh[k];
+        uint32_t i = q & 0x0fffffffu, m = q >> 28;
+        v3 result = f3(W.res[i]);
+        for (uint32_t e = 0; e < m; ++e) {
+            float4 c = W.e_c[(size_t)e * W.capacity + i];
+            if (c.w == 0.0f) result = result + f3(c);
+        }
+        W.res[i] = f4(result, 0.0f);
     }
-    W.res[i] = f4(result, 0.0f);
 }
 
 DEV void neumaier(float& s, float& c, float x) {
@@ -1309,11 +1398,11 @@ DEV void neumaier(float& s, float& c, float x) {
 
 // Fold a wave's samples into each pixel's sums, in sample order
 // (trace.wgsl:2562-2571).
-__global__ __launch_bounds__(256) void k_finish(WaveDev W, AccumDev A, const uint32_t* active, uint32_t npix,
-                                                uint32_t nsamples, float count_after) {
-    uint32_t p = blockIdx.x * 256u + threadIdx.x;
-    if (p >= npix) return;
-    uint32_t pix = active ? active[p] : p;
+// Pixel `pix` of a still whose samples in the wave are paths
+// first + s * npix + p, s < nsamples.
+DEV void finish_pixel(const WaveDev& W, const AccumDev& A, uint32_t pix, uint32_t first, uint32_t npix, uint32_t p,
+                      uint32_t nsamples, float count_after) {
+    p += first;
     if (A.mode == PFXH_ACCUM_PLAIN) {
         float4 c = A.sum[pix * 3u], a = A.sum[pix * 3u + 1u], nn = A.sum[pix * 3u + 2u];
         for (uint32_t s = 0; s < nsamples; ++s) {
@@ -1377,6 +1466,63 @@ __global__ __launch_bounds__(256) void k_finish(WaveDev W, AccumDev A, const uin
         A.comp[pix * 3u + 1u] = C1;
         A.comp[pix * 3u + 2u] = C2;
     }
+}
+
+__global__ __launch_bounds__(256) void k_finish(WaveDev W, AccumDev A, const uint32_t* active, uint32_t npix,
+                                                uint32_t nsamples, float count_after) {
+    uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p >= npix) return;
+    finish_pixel(W, A, active ? active[p] : p, 0u, npix, p, nsamples, count_after);
+}
+
+// Batch: every pixel of every unit of the current wave, into its still's sums.
+__global__ __launch_bounds__(256) void k_finish_batch(BatchDev B, WaveDev W) {
+    const WavePlan& P = B.waves[*B.wave];
+    for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < P.pixels; k += gridDim.x * 256u) {
+        const Unit& u = P.u[unit_of(P, k, true)];
+        uint32_t p = k - u.pfirst;
+        finish_pixel(W, B.accums[u.sid], p, u.first, u.npix, p, u.ns, (float)(u.acc + u.ns));
+    }
+}
+
+// Queue lengths of a batch bounce, kept on the device so the bounce loop
+// needs no host round trip (and can be captured in a graph):
+//   phase 0, before k_extend: the queue length (all paths of the wave at
+//            bounce 0, else what the last k_shade appended) to C_N, the
+//            traversal work counters cleared, rays counted
+//   phase 1, before k_shade: the append counters cleared
+//   phase 2, before k_shadow: shadow rays and shadow paths to C_N + 1 and
+//            C_N + 2, the shadow work counters cleared, rays counted
+__global__ void k_roll(BatchDev B, uint32_t* counters, uint32_t phase, uint32_t bounce) {
+    uint32_t t = threadIdx.x;
+    if (phase == 0u) {
+        uint32_t n = bounce == 0u ? B.waves[*B.wave].paths : counters[C_NEXT];
+        __syncthreads();
+        if (t < PARTS) counters[C_FETCH + t * SLOT] = 0u;
+        if (t == 0) {
+            counters[C_N] = n;
+            *reinterpret_cast<unsigned long long*>(counters + C_STATS) += n;
+        }
+    } else if (phase == 1u) {
+        if (t == 0) {
+            counters[C_NEXT] = 0u;
+            counters[C_SHADOW] = 0u;
+            counters[C_SRAYS] = 0u;
+        }
+    } else {
+        uint32_t rays = counters[C_SRAYS], paths = counters[C_SHADOW];
+        __syncthreads();
+        if (t < PARTS) counters[C_SFETCH + t * SLOT] = 0u;
+        if (t == 0) {
+            counters[C_N + 1] = rays;
+            counters[C_N + 2] = paths;
+            *reinterpret_cast<unsigned long long*>(counters + C_STATS + 2) += rays;
+        }
+    }
+}
+
+__global__ void k_next_wave(BatchDev B) {
+    if (threadIdx.x == 0) *B.wave += 1u;
 }
 
 DEV float total_of(const AccumDev& A, uint32_t pix, int k) {
@@ -1546,27 +1692,30 @@ void launch_extend(hipStream_t s, const SceneDev& S, const WaveDev& W, const uin
                    uint32_t grid) {
     if (!n) return;
     uint32_t blocks = std::min(blocks_for(n, TRACE_BLOCK), grid);
+    const SceneDev* none = nullptr;
     if (count)
-        hipLaunchKernelGGL(k_extend<true>, dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, W, queue, n);
+        hipLaunchKernelGGL((k_extend<true, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, queue, n);
     else
-        hipLaunchKernelGGL(k_extend<false>, dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, W, queue, n);
+        hipLaunchKernelGGL((k_extend<false, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, queue, n);
 }
 void launch_shade(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* queue, uint32_t n,
                   uint32_t bounce) {
     if (!n) return;
-    hipLaunchKernelGGL(k_shade, dim3(blocks_for(n, 256)), dim3(256), 0, s, S, W, queue, n, bounce);
+    const SceneDev* none = nullptr;
+    hipLaunchKernelGGL(k_shade<false>, dim3(blocks_for(n, 256)), dim3(256), 0, s, S, none, W, queue, n, bounce);
 }
 void launch_shadow(hipStream_t s, const SceneDev& S, const WaveDev& W, uint32_t n, bool count, uint32_t grid) {
     if (!n) return;
     uint32_t blocks = std::min(blocks_for(n, TRACE_BLOCK), grid);
+    const SceneDev* none = nullptr;
     if (count)
-        hipLaunchKernelGGL(k_shadow<true>, dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, W, n);
+        hipLaunchKernelGGL((k_shadow<true, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, n);
     else
-        hipLaunchKernelGGL(k_shadow<false>, dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, W, n);
+        hipLaunchKernelGGL((k_shadow<false, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, n);
 }
 void launch_fold(hipStream_t s, const WaveDev& W, uint32_t n) {
     if (!n) return;
-    hipLaunchKernelGGL(k_fold, dim3(blocks_for(n, 256)), dim3(256), 0, s, W, n);
+    hipLaunchKernelGGL(k_fold<false>, dim3(blocks_for(n, 256)), dim3(256), 0, s, W, n);
 }
 void launch_finish(hipStream_t s, const WaveDev& W, const AccumDev& A, const uint32_t* active, uint32_t npix,
                    uint32_t nsamples, float count_after) {
@@ -1607,6 +1756,36 @@ void launch_mark(hipStream_t s, const AccumDev& A, const float4* snaps, uint32_t
 void launch_zero(hipStream_t s, uint32_t* p, uint32_t n) {
     if (!n) return;
     hipLaunchKernelGGL(k_zero, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n);
+}
+
+// ---- batch launchers: fixed grids, lengths from the device ------------------------
+void launch_camera_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid) {
+    hipLaunchKernelGGL(k_camera_batch, dim3(grid), dim3(256), 0, s, B, W);
+}
+void launch_roll(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t phase, uint32_t bounce) {
+    hipLaunchKernelGGL(k_roll, dim3(1), dim3(64), 0, s, B, W.counters, phase, bounce);
+}
+void launch_extend_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t grid) {
+    SceneDev none{};
+    hipLaunchKernelGGL((k_extend<false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W, queue, 0u);
+}
+void launch_shade_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t bounce,
+                        uint32_t grid) {
+    SceneDev none{};
+    hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, s, none, B.scenes, W, queue, 0u, bounce);
+}
+void launch_shadow_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid) {
+    SceneDev none{};
+    hipLaunchKernelGGL((k_shadow<false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W, 0u);
+}
+void launch_fold_batch(hipStream_t s, const WaveDev& W, uint32_t grid) {
+    hipLaunchKernelGGL(k_fold<true>, dim3(grid), dim3(256), 0, s, W, 0u);
+}
+void launch_finish_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid) {
+    hipLaunchKernelGGL(k_finish_batch, dim3(grid), dim3(256), 0, s, B, W);
+}
+void launch_next_wave(hipStream_t s, const BatchDev& B) {
+    hipLaunchKernelGGL(k_next_wave, dim3(1), dim3(64), 0, s, B);
 }
 
 }  // namespace pfxh

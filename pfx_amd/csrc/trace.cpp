@@ -15,8 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/pfxhip.h"
-#include "kernels.h"
+#include "trace_internal.h"
 
 pfxh_status pfxh_denoise_device(pfxh_denoiser* d, uint32_t width, uint32_t height, const void* color,
                                 const void* albedo, const void* normal, float* out, hipStream_t after, char* err,
@@ -25,85 +24,7 @@ pfxh_status pfxh_learned_denoise_device(pfxh_learner* l, uint32_t width, uint32_
                                         const void* albedo, const void* normal, const float* depth,
                                         uint32_t depth_stride, float* out, hipStream_t after, char* err,
                                         size_t err_len);
-#include "layout.h"
 #include "vec3.h"
-
-using namespace pfxh;
-
-namespace {
-
-void put_err(char* err, size_t n, const std::string& s) {
-    if (!err || n == 0) return;
-    size_t k = std::min(n - 1, s.size());
-    memcpy(err, s.data(), k);
-    err[k] = 0;
-}
-
-struct DeviceError {
-    pfxh_status status;
-    std::string what;
-};
-
-#define HIP_TRY(expr)                                                                          \
-    do {                                                                                       \
-        hipError_t e_ = (expr);                                                                \
-        if (e_ != hipSuccess) throw DeviceError{PFXH_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
-    } while (0)
-
-}  // namespace
-
-struct pfxh_trace {
-    HostScene host;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[6] = {};
-    std::vector<void*> allocations;
-    uint64_t gpu_bytes = 0;
-    SceneDev S{};
-    WaveDev W{};
-    AccumDev A{};
-    uint32_t* qa = nullptr;
-    uint32_t* qb = nullptr;
-    uint32_t* active = nullptr;
-    uint32_t* host_counters = nullptr;  // pinned
-    float4* snaps = nullptr;
-    float4* images = nullptr;           // color, albedo, normal for readback
-    pfxh_probe_hit* probe = nullptr;
-    uint32_t width = 0, height = 0, samples = 0;
-    uint32_t capacity = 0;
-    bool any_frozen = false;
-    bool counting = false;
-    pfxh_stats stats{};
-
-    void* alloc(size_t bytes, const char* what) {
-        void* p = nullptr;
-        if (bytes == 0) bytes = 16;
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            throw DeviceError{PFXH_REFUSED, std::string("the tracer's ") + what + " need " + std::to_string(bytes) +
-                                                " bytes on the GPU: " + hipGetErrorString(e)};
-        }
-        allocations.push_back(p);
-        gpu_bytes += bytes;
-        return p;
-    }
-    template <class T>
-    T* upload(const std::vector<T>& v, const char* what) {
-        T* p = (T*)alloc(v.size() * sizeof(T), what);
-        if (!v.empty()) HIP_TRY(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-        return p;
-    }
-    ~pfxh_trace() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (void* p : allocations) (void)hipFree(p);
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (host_counters) (void)hipHostFree(host_counters);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
-
-namespace {
 
 // Idle lanes out of 64 at which a traversal wave fetches new rays; a tuning
 // knob (PFXH_REFILL), clamped to 1..64.
@@ -125,6 +46,8 @@ uint32_t spill_threads_for() {
     return (uint32_t)cus * 16u * (uint32_t)TRACE_BLOCK;
 }
 
+namespace {
+
 // Projection::valid and Projection::fits (gpu.rs:191-208)
 bool projection_valid(uint32_t kind, float w, float h) {
     if (kind == PFXH_PROJ_ORTHOGRAPHIC) return std::isfinite(w) && std::isfinite(h) && w > 0.0f && h > 0.0f;
@@ -143,7 +66,10 @@ void place_projection(SceneDev& S, uint32_t kind, float w, float h) {
     S.half_view[1] = kind == PFXH_PROJ_ORTHOGRAPHIC ? h * 0.5f : 0.0f;
 }
 
-void setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& detail, uint32_t width, uint32_t height) {
+}  // namespace
+
+void trace_setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& detail, uint32_t width, uint32_t height,
+                 bool with_wave) {
     std::string e = build_scene(scene, t->host, detail.no_emitter_sampling == 0u);
     if (!e.empty()) throw DeviceError{PFXH_SETUP, e};
     const HostScene& h = t->host;
@@ -209,6 +135,19 @@ void setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& detail, ui
     t->width = width;
     t->height = height;
     size_t pixels = (size_t)width * height;
+    AccumDev& A = t->A;
+    A.mode = detail.accumulate;
+    if (A.mode > 2u) throw DeviceError{PFXH_SETUP, "unknown accumulation mode"};
+    A.sum = (float4*)t->alloc(pixels * 48, "accumulation");
+    A.comp = A.mode == PFXH_ACCUM_COMPENSATED ? (float4*)t->alloc(pixels * 48, "accumulation compensation") : nullptr;
+    A.sum64 = A.mode == 2u ? (double*)t->alloc(pixels * 96, "fp64 accumulation") : nullptr;
+    t->images = (float4*)t->alloc(pixels * 48, "output images");
+    launch_clear(t->stream, A, (uint32_t)pixels);
+    if (!with_wave) {
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(t->stream));
+        return;
+    }
     // a wave holds every active pixel at least once
     // several samples of every pixel per wave keep late bounces wide enough
     // to fill the GPU; HBM is plentiful (about 180 bytes per path)
@@ -241,18 +180,12 @@ void setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& detail, ui
     W.refill = refill_lanes();
     W.spill = (uint2*)t->alloc((size_t)W.spill_threads * (STACK_DEPTH - STACK_SHORT) * 8, "traversal stacks");
     t->active = (uint32_t*)t->alloc(pixels * 4, "active pixels");
-    AccumDev& A = t->A;
-    A.mode = detail.accumulate;
-    if (A.mode > 2u) throw DeviceError{PFXH_SETUP, "unknown accumulation mode"};
-    A.sum = (float4*)t->alloc(pixels * 48, "accumulation");
-    A.comp = A.mode == PFXH_ACCUM_COMPENSATED ? (float4*)t->alloc(pixels * 48, "accumulation compensation") : nullptr;
-    A.sum64 = A.mode == 2u ? (double*)t->alloc(pixels * 96, "fp64 accumulation") : nullptr;
-    t->images = (float4*)t->alloc(pixels * 48, "output images");
     launch_zero(t->stream, W.counters, C_COUNT);
-    launch_clear(t->stream, A, (uint32_t)pixels);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(t->stream));
 }
+
+namespace {
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0.0f;
@@ -341,36 +274,16 @@ void sample(pfxh_trace* t, uint32_t count, uint32_t seed) {
     t->samples += count;
 }
 
-template <class F>
-pfxh_status guarded(char* err, size_t err_len, F&& f) {
-    try {
-        f();
-        return PFXH_OK;
-    } catch (const DeviceError& e) {
-        put_err(err, err_len, e.what);
-        return e.status;
-    } catch (const std::bad_alloc&) {
-        put_err(err, err_len, "host memory is exhausted");
-        return PFXH_REFUSED;
-    }
-}
-
 bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
-
 
 }  // namespace
 
-extern "C" {
-
-const char* pfxh_version(void) { return "pfxhip 0.1.0 (gfx950 wavefront tracer)"; }
-
-pfxh_status pfxh_trace_new(const pfxh_scene* scene, const pfxh_detail* detail, uint32_t width, uint32_t height,
-                           pfxh_trace** out, char* err, size_t err_len) {
-    if (!scene || !detail || !out) {
+pfxh_status trace_check(const pfxh_scene* scene, const pfxh_detail* detail, uint32_t width, uint32_t height,
+                        char* err, size_t err_len) {
+    if (!scene || !detail) {
         put_err(err, err_len, "null argument");
         return PFXH_INVALID;
     }
-    *out = nullptr;
     if (!finite3(scene->camera.origin) || !finite3(scene->camera.forward) || !finite3(scene->camera.right) ||
         !finite3(scene->camera.up)) {
         put_err(err, err_len, "camera contains invalid parameters");
@@ -388,8 +301,24 @@ pfxh_status pfxh_trace_new(const pfxh_scene* scene, const pfxh_detail* detail, u
         put_err(err, err_len, "an orthographic or equirect camera needs forward, right and up");
         return PFXH_SETUP;
     }
+    return PFXH_OK;
+}
+
+extern "C" {
+
+const char* pfxh_version(void) { return "pfxhip 0.1.0 (gfx950 wavefront tracer)"; }
+
+pfxh_status pfxh_trace_new(const pfxh_scene* scene, const pfxh_detail* detail, uint32_t width, uint32_t height,
+                           pfxh_trace** out, char* err, size_t err_len) {
+    if (!out) {
+        put_err(err, err_len, "null argument");
+        return PFXH_INVALID;
+    }
+    *out = nullptr;
+    pfxh_status chk = trace_check(scene, detail, width, height, err, err_len);
+    if (chk != PFXH_OK) return chk;
     pfxh_trace* t = new pfxh_trace();
-    pfxh_status st = guarded(err, err_len, [&] { setup(t, *scene, *detail, width, height); });
+    pfxh_status st = guarded(err, err_len, [&] { trace_setup(t, *scene, *detail, width, height, true); });
     if (st != PFXH_OK) {
         delete t;
         return st;

@@ -270,3 +270,82 @@ def terrain(seed: int = 2, grid=(1024, 512), size=(128.0, 64.0), amplitude: floa
     scene.camera = Camera.look((0.0, top + 0.3 * sz * 0.4 + 6.0, -0.72 * sz), (0.0, 0.0, 0.1 * sz), 55.0,
                                width / height)
     return scene
+
+
+def sphere_terrain(seed: int, tris: int = 100_000, lights: int = 8, width: int = 512, height: int = 512,
+                   size: float = 64.0) -> Scene:
+    """A seeded sphere-and-terrain mix of about `tris` triangles: an fBm
+    heightfield holding about 60% of them (world triangles), the rest as
+    instances of a subdivision-3 icosphere (1,280 triangles each), diffuse
+    and GGX materials, `lights` point and spot lights (log-uniform
+    intensities over [1, 1000]), a camera looking down across the terrain."""
+    rng = np.random.default_rng(seed)
+    ground_tris = max(2, int(tris * 0.6))
+    n = max(1, int(round((ground_tris / 2) ** 0.5)))
+    h = fbm_heights(seed * 7919 + 3, n, n, size, size, size * 0.1)
+    ground = heightfield_triangles(h, size, size)
+    mats = []
+    for k in range(6):
+        r = float(rng.uniform(0.05, 1.0))
+        if k % 2 == 0:
+            mats.append(material(tuple(rng.uniform(0.2, 0.8, 3)), roughness=r, specular=0.04))
+        else:
+            mats.append(material(tuple(rng.uniform(0.5, 0.95, 3)), roughness=r, metalness=1.0, specular=0.04))
+    qz, qx = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    tri_mat = np.repeat(((qx * 3 // n) + 3 * ((qz * 2) // n)) % 4, 2)
+    scene = Scene(triangles=_triangles(ground, tri_mat), materials=np.concatenate(mats))
+    unit = icosphere(3)
+    rows = np.zeros(len(unit), dtype=N.TRIANGLE_DETAIL)
+    rows["vertices"] = unit.reshape(len(unit), 9).astype(np.float32)
+    rows["normals"] = unit.reshape(len(unit), 9).astype(np.float32)  # unit sphere: normal = position
+    rows["material"] = 4
+    scene.mesh_triangles = rows
+    scene.meshes = np.array([(0, len(unit))], dtype=N.MESH)
+    spheres = max(0, (tris - len(ground)) // len(unit))
+    inst = np.zeros(spheres, dtype=N.INSTANCE)
+    top = float(h.max())
+    for i in range(spheres):
+        x, z = rng.uniform(-0.45, 0.45, 2) * size
+        ix = int(np.clip((x / size + 0.5) * n, 0, n))
+        iz = int(np.clip((z / size + 0.5) * n, 0, n))
+        radius = float(rng.uniform(0.3, 2.0))
+        inst[i]["mesh"] = 0
+        inst[i]["word"] = 0
+        inst[i]["transform"] = transform((x, float(h[iz, ix]) + radius, z), 0.0, radius)
+    scene.instances = inst
+    light_rows = []
+    for k in range(lights):
+        x, z = rng.uniform(-0.5, 0.5, 2) * size
+        y = top + float(rng.uniform(0.5, 8.0))
+        intensity = float(math.exp(rng.uniform(0.0, math.log(1000.0))))
+        colour = tuple(rng.uniform(0.6, 1.0, 3))
+        if k % 2 == 0:
+            light_rows.append(point_light((x, y, z), colour, intensity, size * 0.6, radius=0.1))
+        else:
+            d = np.array([rng.uniform(-0.5, 0.5), -1.0, rng.uniform(-0.5, 0.5)])
+            outer = float(rng.uniform(15.0, 60.0))
+            light_rows.append(spot_light((x, y, z), colour, intensity, size * 0.6, tuple(d / np.linalg.norm(d)),
+                                         0.8 * outer, outer, radius=0.1))
+    scene.lights = np.concatenate(light_rows) if light_rows else scene.lights
+    yaw = float(rng.uniform(0, 2 * math.pi))
+    eye = (0.55 * size * math.cos(yaw), top + size * 0.15, 0.55 * size * math.sin(yaw))
+    scene.camera = Camera.look(eye, (0.0, 0.0, 0.0), 55.0, width / height)
+    return scene
+
+
+def batch_mix(seed: int = 3, stills: int = 32, sizes=(256, 512, 1024, 2048), tris=(10_000, 500_000),
+              lights=(1, 64), spps=(4, 16, 64)) -> list[tuple[Scene, int, int, int, int]]:
+    """Uneven stills for a batch: per still a square size drawn from `sizes`,
+    a sphere-and-terrain scene of log-uniform triangle count in `tris` with
+    a light count drawn from `lights`, spp drawn from `spps` and a seed.
+    Returns (scene, width, height, spp, seed) tuples."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(stills):
+        side = int(rng.choice(sizes))
+        t = int(math.exp(rng.uniform(math.log(tris[0]), math.log(tris[1]))))
+        nl = int(rng.integers(lights[0], lights[1] + 1))
+        spp = int(rng.choice(spps))
+        scene = sphere_terrain(seed * 1000 + k, tris=t, lights=nl, width=side, height=side)
+        out.append((scene, side, side, spp, int(rng.integers(1, 2**31))))
+    return out
