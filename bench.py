@@ -15,11 +15,14 @@ Prints one JSON line. Extra fields:
                 against the 8 TB/s HBM peak
   cpu_baseline  the CPU oracle (a restatement of the reference tracer, OpenMP
                 over the host cores) on a bounded strip of the same frame
+  batch         32 uneven stills rendered as one batch (autotuned, graph-
+                replayed bounce loop) against the same stills traced one
+                after another, plus a mix of small stills
   quality       relMSE against two independent converged 4096-spp windows
                 of the frame (as stated, and an unbiased estimate of the
                 error against the noise-free image) for the plain tracer +
                 repo denoiser at 4/16/64 spp and the learned denoiser at
-                4/8/16/32 spp, with the times (the metric's "ms to relMSE
+                4/8/16/32/40/48 spp, with the times (the metric's "ms to relMSE
                 <= 0.01" half; null when a method does not reach it)
 """
 from __future__ import annotations
@@ -163,7 +166,8 @@ def quality_leg(scene) -> dict:
         log(f"no learned weights at {learn_mod.WEIGHTS}: the learned leg is skipped")
     settings = [(4, "repo"), (16, "repo"), (64, "repo")]
     if learner:
-        settings += [(4, "learned"), (8, "learned"), (16, "learned"), (32, "learned")]
+        settings += [(4, "learned"), (8, "learned"), (16, "learned"), (32, "learned"), (40, "learned"),
+                     (48, "learned")]
     with denoise_mod.Denoiser() as d, Trace.new_detailed(scene, Detail(bounces=BOUNCES), WIDTH, HEIGHT) as t4k:
         for spp, how in settings:
             t4k.set_camera(scene.camera)
@@ -194,6 +198,53 @@ def quality_leg(scene) -> dict:
     return out
 
 
+def batch_leg() -> dict:
+    """Uneven stills in one batch (pfx_amd.batch): 32 stills of seeded
+    sphere-and-terrain scenes (10k-500k triangles, 1-64 lights), sizes drawn
+    from 256^2..2048^2 and spp from {4, 16, 64}; the batch is autotuned, then
+    rendered once (timed). The same stills traced one after another, each by
+    its own Trace, are the comparison. A second mix of small stills (256^2 and
+    512^2 at 4 spp), where one still alone cannot fill the GPU, shows what
+    batching buys."""
+    from pfx_amd.batch import Batch, Still
+
+    detail = Detail(bounces=BOUNCES)
+
+    def run(mix, label):
+        b = Batch([Still(*m) for m in mix], detail)
+        t = time.perf_counter()
+        tuning = b.tune()
+        tune_s = time.perf_counter() - t
+        b.reset_stats()
+        t = time.perf_counter()
+        b.render()
+        batch_s = time.perf_counter() - t
+        st = b.stats()
+        b.close()
+        rays = st["closest_rays"] + st["shadow_rays"]
+        seq_s, seq_rays = 0.0, 0
+        for sc, w, h, spp, seed in mix:
+            with Trace.new_detailed(sc, detail, w, h) as tr:
+                tr.reset_stats()
+                t = time.perf_counter()
+                tr.sample(spp, seed)
+                seq_s += time.perf_counter() - t
+                ss = tr.stats()
+                seq_rays += ss["closest_rays"] + ss["shadow_rays"]
+        out = {"stills": len(mix), "pixels": sum(m[1] * m[2] for m in mix), "paths": st["paths"],
+               "batch_ms": round(batch_s * 1e3, 1), "batch_mrays_per_s": round(rays / batch_s / 1e6, 1),
+               "sequential_ms": round(seq_s * 1e3, 1), "sequential_mrays_per_s": round(seq_rays / seq_s / 1e6, 1),
+               "speedup": round(seq_s / batch_s, 3), "tune_s": round(tune_s, 2), "tuning": tuning.__dict__}
+        log(f"batch {label}: {out}")
+        return out
+
+    out = {"workload": "32 stills, sizes {256,512,1024,2048}^2, sphere-and-terrain scenes 10k-500k tris, "
+                       "1-64 lights, spp {4,16,64}, 6 bounces"}
+    out.update(run(scenes.batch_mix(seed=3), "uneven"))
+    out["small_stills"] = run(scenes.batch_mix(seed=5, sizes=(256, 512), spps=(4,)), "small")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +252,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-quality", action="store_true", help="skip the image-quality leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-stills leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); across GPUs the still would be "
@@ -288,6 +340,8 @@ def main() -> int:
     trace.close()
     if not args.no_quality:
         result["quality"] = quality_leg(scene)
+    if not args.no_batch:
+        result["batch"] = batch_leg()
     if not args.no_cpu:
         log("cpu baseline (oracle on host cores)...")
         result["cpu_baseline"] = cpu_baseline(scene)

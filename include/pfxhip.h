@@ -343,6 +343,17 @@ pfxh_status pfxh_trace_save(pfxh_trace* trace, const char* key, uint8_t* out, si
 pfxh_status pfxh_trace_restore(pfxh_trace* trace, const char* key, const uint8_t* bytes, size_t len,
                                char* err, size_t err_len);
 
+/* ---- the tracer's own denoiser (crates/trace/src/denoise.rs) -------------- */
+
+/* denoise::filter (denoise.rs:1-62): three edge-aware a-trous passes on
+ * RGBA32F colour guided by albedo and normal (width*height*4 floats each),
+ * for an image of `samples` spp; `out` gets RGBA32F (alpha from the colour).
+ * PFXH_INVALID ("invalid denoise inputs") for zero samples. */
+pfxh_status pfxh_atrous(uint32_t width, uint32_t height, const float* color, const float* albedo,
+                        const float* normal, uint32_t samples, float* out, char* err, size_t err_len);
+/* The same on a trace's current images and sample count, on the device. */
+pfxh_status pfxh_trace_atrous(pfxh_trace* trace, float* out, char* err, size_t err_len);
+
 /* ---- the repo denoiser (crates/denoise: OIDN 2.5.1's RT filter) ------------ */
 
 typedef struct pfxh_denoiser pfxh_denoiser;
@@ -509,6 +520,15 @@ pfxh_status pfxh_learned_denoise(pfxh_learner* learner, uint32_t width, uint32_t
  * depth, without a host round trip for the inputs. */
 pfxh_status pfxh_trace_learned_denoise(pfxh_trace* trace, pfxh_learner* learner, float* out, char* err,
                                        size_t err_len);
+/* The same with the repo denoiser as fallback: per `tile` x `tile` tile the
+ * variance the network removed, mean (log(1 + noisy) - log(1 + learned))^2
+ * over its pixels and colour channels, is checked against `threshold`;
+ * tiles above it (or not finite) take the repo denoiser's output, which is
+ * computed only when some tile needs it. *fallback_tiles gets their count. */
+pfxh_status pfxh_trace_learned_denoise_checked(pfxh_trace* trace, pfxh_learner* learner,
+                                               pfxh_denoiser* denoiser, float threshold, uint32_t tile,
+                                               float* out, uint32_t* fallback_tiles, char* err,
+                                               size_t err_len);
 void pfxh_learner_free(pfxh_learner* learner);
 
 /* Library version string. */

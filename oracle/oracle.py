@@ -36,6 +36,7 @@ def lib() -> C.CDLL:
             "or_tza_count": (C.c_int, [C.c_char_p, C.c_size_t]),
             "or_denoise_plan": (C.c_int, [_U, _U, C.c_uint64, C.POINTER(_U), C.POINTER(_U), _P, _U]),
             "or_denoise": (C.c_int, [C.c_char_p, C.c_size_t, _U, _U, _P, _P, _P, _P, C.c_uint64]),
+            "or_atrous": (C.c_int, [_U, _U, _P, _P, _P, _U, _P]),
             "or_learn_param_count": (C.c_uint64, []),
             "or_learn_layer": (None, [_U, _P]),
             "or_learn_grad": (C.c_double, [_P, _U, _U, _U, _P, _P, _P, _P, _P, _P]),
@@ -209,4 +210,14 @@ def learn_forward(params, color, albedo, normal, depth):
     out = np.zeros((h, w, 4), np.float32)
     lib().or_learn_forward(p.ctypes.data, h, w, color.ctypes.data, albedo.ctypes.data, normal.ctypes.data,
                            _ptr(depth), out.ctypes.data)
+    return out
+
+
+def atrous(color, albedo, normal, samples: int):
+    """denoise::filter on (h, w, 4) float32 images; returns (h, w, 4)."""
+    color, albedo, normal = (np.ascontiguousarray(a, np.float32) for a in (color, albedo, normal))
+    h, w = color.shape[:2]
+    out = np.zeros_like(color)
+    if lib().or_atrous(w, h, color.ctypes.data, albedo.ctypes.data, normal.ctypes.data, samples, out.ctypes.data):
+        raise ValueError("invalid denoise inputs")
     return out

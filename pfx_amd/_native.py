@@ -234,6 +234,8 @@ SIGNATURES = {
     "pfxh_checkpoint_write": (C.c_int, [_P, _S, _P, _Z, C.POINTER(C.c_size_t), _S, _Z]),
     "pfxh_checkpoint_read": (C.c_int, [_P, _Z, _S, _P, _P, _P, _U, _S, _Z]),
     "pfxh_trace_save": (C.c_int, [_P, _S, _P, _Z, C.POINTER(C.c_size_t), _S, _Z]),
+    "pfxh_atrous": (C.c_int, [_U, _U, _P, _P, _P, _U, _P, _S, _Z]),
+    "pfxh_trace_atrous": (C.c_int, [_P, _P, _S, _Z]),
     "pfxh_denoiser_new": (C.c_int, [_S, _Z, C.POINTER(_P), _S, _Z]),
     "pfxh_denoise": (C.c_int, [_P, _U, _U, _P, _P, _P, _P, _S, _Z]),
     "pfxh_trace_denoise": (C.c_int, [_P, _P, _P, _S, _Z]),
@@ -266,6 +268,7 @@ SIGNATURES = {
     "pfxh_learner_train": (C.c_int, [_P, _U, _U, C.c_float, C.c_uint64, _P, _S, _Z]),
     "pfxh_learned_denoise": (C.c_int, [_P, _U, _U, _P, _P, _P, _P, _P, _S, _Z]),
     "pfxh_trace_learned_denoise": (C.c_int, [_P, _P, _P, _S, _Z]),
+    "pfxh_trace_learned_denoise_checked": (C.c_int, [_P, _P, _P, C.c_float, _U, _P, C.POINTER(_U), _S, _Z]),
     "pfxh_learner_free": (None, [_P]),
     "pfxh_trace_restore": (C.c_int, [_P, _S, _P, _Z, _S, _Z]),
     "pfxh_trace_samples": (_U, [_P]),

@@ -138,3 +138,28 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def _main() -> None:
+    """`python -m pfx_amd.batch [SEED]`: build the 32-still uneven mix
+    (scenes.batch_mix), render it once as a batch and print the time."""
+    import sys
+    import time
+
+    from . import Bounces, scenes
+
+    seed = int(sys.argv[1]) if len(sys.argv) > 1 else 3
+    mix = scenes.batch_mix(seed=seed)
+    with Batch([Still(*m) for m in mix], Detail(bounces=Bounces.deep(6))) as b:
+        for k in range(2):
+            b.reset_stats()
+            t = time.perf_counter()
+            b.render()
+            dt = time.perf_counter() - t
+            s = b.stats()
+            print(f"batch render {k}: {dt * 1e3:.0f} ms, {(s['closest_rays'] + s['shadow_rays']) / dt / 1e6:.0f} "
+                  f"Mrays/s", flush=True)
+
+
+if __name__ == "__main__":
+    _main()

@@ -1,7 +1,8 @@
 """Build libpfxhip.so for gfx950 in-tree (hipcc, no JIT cache).
 
 `python -m pfx_amd.build` compiles every source under pfx_amd/csrc into
-pfx_amd/_build/*.o and links pfx_amd/libpfxhip.so. All device and host code
+pfx_amd/_build/*.o, links pfx_amd/libpfxhip.so and builds the C++
+command-line caller pfx_amd/pfxh-render against it. All device and host code
 is built with -ffp-contract=off (no fused multiply-add), IEEE division and
 square root (hipcc's default for HIP), and with packed FP32 instructions
 disabled.
@@ -18,7 +19,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "_build"
 LIB = HERE / "libpfxhip.so"
-SOURCES = ["kernels.hip", "layout.cpp", "trace.cpp", "checkpoint.cpp", "denoise.hip", "denoise_host.cpp", "batch.cpp",
+SOURCES = ["kernels.hip", "layout.cpp", "trace.cpp", "checkpoint.cpp", "denoise.hip", "denoise_host.cpp", "batch.cpp", "atrous.hip",
            "learn.hip", "learn_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -60,6 +61,13 @@ def build(verbose: bool = False) -> Path:
     if proc.returncode != 0:
         raise RuntimeError("link failed:\n" + proc.stdout + proc.stderr)
     os.replace(tmp, LIB)
+    # the C++ command-line caller, linked against the library beside it
+    cli = HERE / "pfxh-render"
+    cmd = ["g++", "-O2", "-std=c++17", str(HERE / "cli" / "pfxh_render.cpp"), "-o", str(cli), "-L", str(HERE),
+           "-lpfxhip", "-Wl,-rpath,$ORIGIN"]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError("pfxh-render failed to build:\n" + proc.stdout + proc.stderr)
     return LIB
 
 

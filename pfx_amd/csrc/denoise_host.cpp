@@ -371,18 +371,21 @@ pfxh_status guard(char* err, size_t err_len, F&& f) {
 }  // namespace
 
 // Internal: denoise device images (width x height RGBA32F) written on
-// `after`, and copy the result to the host buffer `out`.
+// `after`, and copy the result to the host buffer `out`; with `out` null the
+// result stays on the device at *device_out (valid until the next call).
 pfxh_status pfxh_denoise_device(pfxh_denoiser* d, uint32_t width, uint32_t height, const void* color,
                                 const void* albedo, const void* normal, float* out, hipStream_t after, char* err,
-                                size_t err_len) {
+                                size_t err_len, const void** device_out) {
     return guard(err, err_len, [&] {
         if (width == 0 || height == 0) throw Failure{PFXH_INVALID, "image dimensions must be nonzero"};
         d->prepare(width, height);
         if (after) hip_check(hipStreamSynchronize(after), "denoise input");
         float4* o = d->images + 3 * (size_t)width * height;
         d->run((const float4*)color, (const float4*)albedo, (const float4*)normal, o);
-        hip_check(hipMemcpyAsync(out, o, (size_t)width * height * 16, hipMemcpyDeviceToHost, d->stream),
-                  "denoise readback");
+        if (device_out) *device_out = o;
+        if (out)
+            hip_check(hipMemcpyAsync(out, o, (size_t)width * height * 16, hipMemcpyDeviceToHost, d->stream),
+                      "denoise readback");
         hip_check(hipStreamSynchronize(d->stream), "denoise");
     });
 }

@@ -40,6 +40,10 @@ void loss_total(hipStream_t s, const float* sums, uint32_t nb, size_t n, float* 
 void learn_out(hipStream_t s, const float* y, uint32_t w16, const float4* color, uint32_t w, uint32_t h, float4* out);
 void gather(hipStream_t s, const float* inputs, const float4* targets, size_t crop_px, uint32_t side,
             const uint32_t* picks, uint32_t batch, float* x, float4* t);
+void tile_residual(hipStream_t s, const float4* noisy, const float4* den, uint32_t w, uint32_t h, uint32_t tile,
+                   float* r);
+void tile_merge(hipStream_t s, float4* out, const float4* repo, const uint8_t* mask, uint32_t w, uint32_t h,
+                uint32_t tile);
 void adam(hipStream_t s, float* p, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2,
           float eps, int t);
 

@@ -450,6 +450,56 @@ void loss_total(hipStream_t s, const float* sums, uint32_t nb, size_t n, float* 
     hipLaunchKernelGGL(k_loss_total, dim3(1), dim3(256), 0, s, sums, nb, 1.0f / (3.0f * (float)n), out);
 }
 
+// ---- the per-tile check that hands tiles to the repo denoiser --------------------
+// r[tile] = mean over the tile's pixels and colour channels of
+// (log(1 + noisy) - log(1 + denoised))^2: the variance the network removed,
+// an estimate of the tile's noise.
+__global__ __launch_bounds__(256) void k_tile_residual(const float4* noisy, const float4* den, uint32_t w, uint32_t h,
+                                                       uint32_t tile, uint32_t tiles_x, float* r) {
+    __shared__ float red[256];
+    const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const uint32_t x0 = tx * tile, y0 = ty * tile;
+    float acc = 0.0f;
+    for (uint32_t k = threadIdx.x; k < tile * tile; k += 256) {
+        uint32_t x = x0 + k % tile, y = y0 + k / tile;
+        if (x >= w || y >= h) continue;
+        size_t p = (size_t)y * w + x;
+        float4 a = noisy[p], b = den[p];
+        float d0 = log1pf(fmaxf(finite_or0(a.x), 0.0f)) - log1pf(fmaxf(finite_or0(b.x), 0.0f));
+        float d1 = log1pf(fmaxf(finite_or0(a.y), 0.0f)) - log1pf(fmaxf(finite_or0(b.y), 0.0f));
+        float d2 = log1pf(fmaxf(finite_or0(a.z), 0.0f)) - log1pf(fmaxf(finite_or0(b.z), 0.0f));
+        acc += d0 * d0 + d1 * d1 + d2 * d2;
+    }
+    red[threadIdx.x] = acc;
+    for (int k = 128; k > 0; k >>= 1) {
+        __syncthreads();
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    }
+    if (threadIdx.x == 0) {
+        uint32_t nx = min(tile, w - x0), ny = min(tile, h - y0);
+        r[blockIdx.x] = red[0] / (3.0f * (float)(nx * ny));
+    }
+}
+// out = repo where the pixel's tile is marked
+__global__ void k_tile_merge(float4* out, const float4* repo, const uint8_t* mask, uint32_t w, uint32_t h,
+                             uint32_t tile, uint32_t tiles_x) {
+    size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (size_t)w * h) return;
+    uint32_t x = p % w, y = (uint32_t)(p / w);
+    if (mask[(y / tile) * tiles_x + x / tile]) out[p] = repo[p];
+}
+void tile_residual(hipStream_t s, const float4* noisy, const float4* den, uint32_t w, uint32_t h, uint32_t tile,
+                   float* r) {
+    uint32_t tx = (w + tile - 1) / tile, ty = (h + tile - 1) / tile;
+    hipLaunchKernelGGL(k_tile_residual, dim3(tx * ty), dim3(256), 0, s, noisy, den, w, h, tile, tx, r);
+}
+void tile_merge(hipStream_t s, float4* out, const float4* repo, const uint8_t* mask, uint32_t w, uint32_t h,
+                uint32_t tile) {
+    uint32_t tx = (w + tile - 1) / tile;
+    hipLaunchKernelGGL(k_tile_merge, dim3(blocks((size_t)w * h, 256)), dim3(256), 0, s, out, repo, mask, w, h, tile,
+                       tx);
+}
+
 // ---- Adam ---------------------------------------------------------------------------------
 __global__ void k_adam(float* p, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2,
                        float eps, float c1, float c2) {

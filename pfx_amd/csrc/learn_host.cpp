@@ -401,11 +401,12 @@ struct pfxh_learner {
 
 // Internal: the learned denoiser on device images of `width` x `height`
 // (depth with a stride of `depth_stride` floats), written on `after`; the
-// result is copied to the host buffer `out`.
+// result is copied to the host buffer `out`, or with `out` null left on the
+// device at *device_out (valid until the next call).
 pfxh_status pfxh_learned_denoise_device(pfxh_learner* l, uint32_t width, uint32_t height, const void* color,
                                         const void* albedo, const void* normal, const float* depth,
                                         uint32_t depth_stride, float* out, hipStream_t after, char* err,
-                                        size_t err_len) {
+                                        size_t err_len, void** device_out) {
     return guard(err, err_len, [&] {
         if (width == 0 || height == 0) throw Failure{PFXH_INVALID, "image dimensions must be nonzero"};
         if (after) hip_check(hipStreamSynchronize(after), "learned denoise input");
@@ -414,8 +415,10 @@ pfxh_status pfxh_learned_denoise_device(pfxh_learner* l, uint32_t width, uint32_
         float4* o = l->infer_frame.images + 3 * (size_t)w16 * h16;
         l->infer(width, height, (const float4*)color, (const float4*)albedo, (const float4*)normal, depth,
                  depth_stride, o);
-        hip_check(hipMemcpyAsync(out, o, (size_t)width * height * 16, hipMemcpyDeviceToHost, l->stream),
-                  "learned denoise readback");
+        if (device_out) *device_out = o;
+        if (out)
+            hip_check(hipMemcpyAsync(out, o, (size_t)width * height * 16, hipMemcpyDeviceToHost, l->stream),
+                      "learned denoise readback");
         hip_check(hipStreamSynchronize(l->stream), "learned denoise");
     });
 }

@@ -19,11 +19,14 @@
 
 pfxh_status pfxh_denoise_device(pfxh_denoiser* d, uint32_t width, uint32_t height, const void* color,
                                 const void* albedo, const void* normal, float* out, hipStream_t after, char* err,
-                                size_t err_len);
+                                size_t err_len, const void** device_out = nullptr);
 pfxh_status pfxh_learned_denoise_device(pfxh_learner* l, uint32_t width, uint32_t height, const void* color,
                                         const void* albedo, const void* normal, const float* depth,
                                         uint32_t depth_stride, float* out, hipStream_t after, char* err,
-                                        size_t err_len);
+                                        size_t err_len, void** device_out = nullptr);
+#include "learn.h"
+hipError_t pfxh_atrous_device(hipStream_t s, float4* a, float4* b, const float4* albedo, const float4* normal,
+                              uint32_t w, uint32_t h, uint32_t samples);
 #include "vec3.h"
 
 // Idle lanes out of 64 at which a traversal wave fetches new rays; a tuning
@@ -610,6 +613,88 @@ pfxh_status pfxh_trace_learned_denoise(pfxh_trace* t, pfxh_learner* l, float* ou
     static_assert(offsetof(pfxh_probe_hit, depth) == 0 && sizeof(pfxh_probe_hit) % 4 == 0, "probe layout");
     return pfxh_learned_denoise_device(l, t->width, t->height, c, c + pixels, c + 2 * pixels, (const float*)t->probe,
                                        (uint32_t)(sizeof(pfxh_probe_hit) / 4), out, t->stream, err, err_len);
+}
+
+pfxh_status pfxh_trace_learned_denoise_checked(pfxh_trace* t, pfxh_learner* l, pfxh_denoiser* d, float threshold,
+                                               uint32_t tile, float* out, uint32_t* fallback_tiles, char* err,
+                                               size_t err_len) {
+    if (!t || !l || !d || !out || tile == 0) return PFXH_INVALID;
+    size_t pixels = (size_t)t->width * t->height;
+    if (fallback_tiles) *fallback_tiles = 0;
+    if (!pixels) return PFXH_OK;
+    pfxh_status st = guarded(err, err_len, [&] {
+        float4* c = t->images;
+        launch_develop(t->stream, t->A, (uint32_t)pixels, c, c + pixels, c + 2 * pixels);
+        if (!t->probe) t->probe = (pfxh_probe_hit*)t->alloc(pixels * sizeof(pfxh_probe_hit), "probe");
+        launch_probe(t->stream, t->S, t->W, t->probe);
+        HIP_TRY(hipGetLastError());
+    });
+    if (st != PFXH_OK) return st;
+    float4* c = t->images;
+    void* learned = nullptr;
+    st = pfxh_learned_denoise_device(l, t->width, t->height, c, c + pixels, c + 2 * pixels, (const float*)t->probe,
+                                     (uint32_t)(sizeof(pfxh_probe_hit) / 4), nullptr, t->stream, err, err_len,
+                                     &learned);
+    if (st != PFXH_OK) return st;
+    uint32_t tiles_x = (t->width + tile - 1) / tile, tiles = tiles_x * ((t->height + tile - 1) / tile);
+    std::vector<float> r(tiles);
+    std::vector<uint8_t> mask(tiles, 0);
+    uint32_t marked = 0;
+    st = guarded(err, err_len, [&] {
+        // the learned output was written on the learner's stream, which
+        // pfxh_learned_denoise_device has drained
+        if (!t->tile_buf || t->tile_cap < tiles) {
+            t->tile_buf = (float*)t->alloc((size_t)tiles * 8, "tile check");
+            t->tile_cap = tiles;
+        }
+        pfxl::tile_residual(t->stream, c, (const float4*)learned, t->width, t->height, tile, t->tile_buf);
+        HIP_TRY(hipMemcpyAsync(r.data(), t->tile_buf, tiles * 4, hipMemcpyDeviceToHost, t->stream));
+        HIP_TRY(hipStreamSynchronize(t->stream));
+        for (uint32_t k = 0; k < tiles; ++k)
+            if (!(r[k] <= threshold)) {  // NaN falls back too
+                mask[k] = 1;
+                ++marked;
+            }
+    });
+    if (st != PFXH_OK) return st;
+    if (marked) {
+        const void* repo = nullptr;
+        st = pfxh_denoise_device(d, t->width, t->height, c, c + pixels, c + 2 * pixels, nullptr, t->stream, err,
+                                 err_len, &repo);
+        if (st != PFXH_OK) return st;
+        st = guarded(err, err_len, [&] {
+            HIP_TRY(hipStreamSynchronize(t->stream));
+            uint8_t* m = (uint8_t*)(t->tile_buf + tiles);  // the second half of the tile buffer
+            HIP_TRY(hipMemcpyAsync(m, mask.data(), tiles, hipMemcpyHostToDevice, t->stream));
+            pfxl::tile_merge(t->stream, (float4*)learned, (const float4*)repo, m, t->width, t->height, tile);
+            HIP_TRY(hipGetLastError());
+        });
+        if (st != PFXH_OK) return st;
+    }
+    st = guarded(err, err_len, [&] {
+        HIP_TRY(hipMemcpyAsync(out, learned, pixels * 16, hipMemcpyDeviceToHost, t->stream));
+        HIP_TRY(hipStreamSynchronize(t->stream));
+    });
+    if (fallback_tiles) *fallback_tiles = marked;
+    return st;
+}
+
+pfxh_status pfxh_trace_atrous(pfxh_trace* t, float* out, char* err, size_t err_len) {
+    if (!t || !out) return PFXH_INVALID;
+    size_t pixels = (size_t)t->width * t->height;
+    return guarded(err, err_len, [&] {
+        if (t->samples == 0) throw DeviceError{PFXH_INVALID, "invalid denoise inputs"};
+        if (!pixels) return;
+        float4* c = t->images;
+        launch_develop(t->stream, t->A, (uint32_t)pixels, c, c + pixels, c + 2 * pixels);
+        if (!t->atrous) t->atrous = (float4*)t->alloc(pixels * 32, "a-trous images");
+        HIP_TRY(hipMemcpyAsync(t->atrous, c, pixels * 16, hipMemcpyDeviceToDevice, t->stream));
+        HIP_TRY(pfxh_atrous_device(t->stream, t->atrous, t->atrous + pixels, c + pixels, c + 2 * pixels, t->width,
+                                   t->height, t->samples));
+        HIP_TRY(hipMemcpyAsync(out, t->atrous, pixels * 16, hipMemcpyDeviceToHost, t->stream));
+        HIP_TRY(hipStreamSynchronize(t->stream));
+        HIP_TRY(hipGetLastError());
+    });
 }
 
 uint32_t pfxh_trace_samples(const pfxh_trace* t) { return t ? t->samples : 0; }

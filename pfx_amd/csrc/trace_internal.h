@@ -48,6 +48,9 @@ struct pfxh_trace {
     float4* snaps = nullptr;
     float4* images = nullptr;           // color, albedo, normal for readback
     pfxh_probe_hit* probe = nullptr;
+    float* tile_buf = nullptr;          // per-tile check of the learned denoiser
+    float4* atrous = nullptr;           // a-trous passes: two images
+    uint32_t tile_cap = 0;
     uint32_t width = 0, height = 0, samples = 0;
     uint32_t capacity = 0;
     bool any_frozen = false;

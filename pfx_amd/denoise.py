@@ -110,3 +110,18 @@ def _main() -> None:
 
 if __name__ == "__main__":
     _main()
+
+
+def atrous(color: np.ndarray, albedo: np.ndarray, normal: np.ndarray, samples: int) -> np.ndarray:
+    """The tracer's own denoiser, `pfx_trace::denoise::filter`
+    (crates/trace/src/denoise.rs:1-62): three edge-aware a-trous passes on
+    the GPU. (H, W, 4) float32 images in, the filtered colour out."""
+    height, width = color.shape[:2]
+    c, a, n = (_f32(x, height, width) for x in (color, albedo, normal))
+    out = np.empty_like(c)
+    err = N.err_buffer()
+    st = N.lib().pfxh_atrous(width, height, c.ctypes.data, a.ctypes.data, n.ctypes.data, int(samples),
+                             out.ctypes.data, err, len(err))
+    if st != N.OK:
+        raise (ValueError if st == N.INVALID else RuntimeError)(err.value.decode(errors="replace"))
+    return out

@@ -182,3 +182,28 @@ class Learner:
             self.close()
         except Exception:
             pass
+
+
+def _main() -> None:
+    """`python -m pfx_amd.learn [WIDTH HEIGHT [RUNS]]`: trace the benchmark
+    terrain at 4 spp and run the shipped learned denoiser on it RUNS times,
+    printing the time of each run."""
+    import sys
+    import time
+
+    from . import Bounces, Detail, Trace, scenes
+
+    w, h = (int(sys.argv[1]), int(sys.argv[2])) if len(sys.argv) > 2 else (3840, 2160)
+    runs = int(sys.argv[3]) if len(sys.argv) > 3 else 3
+    scene = scenes.terrain(width=w, height=h)
+    with Trace.new_detailed(scene, Detail(bounces=Bounces.deep(6)), w, h) as t, Learner(weights=WEIGHTS) as L:
+        t.sample(4, 1)
+        for k in range(runs):
+            s = time.perf_counter()
+            img = t.learned_denoise(L)
+            print(f"learned denoise {w}x{h} run {k}: {(time.perf_counter() - s) * 1e3:.1f} ms, "
+                  f"finite {bool(np.isfinite(img).all())}", flush=True)
+
+
+if __name__ == "__main__":
+    _main()

@@ -187,6 +187,53 @@ class Scene:
         return s, keep
 
 
+SCENE_MAGIC = b"PFXSCN01"
+
+
+def save_scene(scene: "Scene", path) -> None:
+    """A scene file for non-Python callers (pfxh-render): b"PFXSCN01", nine
+    little-endian u32 counts (triangles, world details, mesh triangles,
+    meshes, instances, materials, lights, sky width, sky height), the camera
+    (12 f32) and sun (7 f32), then the arrays in that order as the C
+    structs of include/pfxhip.h lay them out, then the sky texels."""
+    sky = scene.sky if scene.sky is not None and scene.sky.size else np.zeros((0, 0, 4), np.float32)
+    arrays = [(scene.triangles, N.TRIANGLE), (scene.world_details, N.TRIANGLE_DETAIL),
+              (scene.mesh_triangles, N.TRIANGLE_DETAIL), (scene.meshes, N.MESH), (scene.instances, N.INSTANCE),
+              (scene.materials, N.MATERIAL), (scene.lights, N.LIGHT)]
+    c, sun = scene.camera, scene.sun
+    with open(path, "wb") as f:
+        f.write(SCENE_MAGIC)
+        counts = [len(a) for a, _ in arrays] + [sky.shape[1], sky.shape[0]]
+        f.write(np.asarray(counts, "<u4").tobytes())
+        f.write(np.asarray(list(c.origin) + list(c.forward) + list(c.right) + list(c.up), "<f4").tobytes())
+        f.write(np.asarray(list(sun.direction) + list(sun.color) + [sun.intensity], "<f4").tobytes())
+        for a, dt in arrays:
+            f.write(np.ascontiguousarray(a, dt).tobytes())
+        f.write(np.ascontiguousarray(sky, "<f4").tobytes())
+
+
+def load_scene(path) -> "Scene":
+    data = memoryview(open(path, "rb").read())
+    if bytes(data[:8]) != SCENE_MAGIC:
+        raise ValueError(f"{path}: not a PFXSCN01 scene file")
+    counts = np.frombuffer(data[8:44], "<u4")
+    cam = np.frombuffer(data[44:92], "<f4")
+    sun = np.frombuffer(data[92:120], "<f4")
+    at = 120
+    out = []
+    for n, dt in zip(counts[:7], (N.TRIANGLE, N.TRIANGLE_DETAIL, N.TRIANGLE_DETAIL, N.MESH, N.INSTANCE, N.MATERIAL,
+                                  N.LIGHT)):
+        size = int(n) * np.dtype(dt).itemsize
+        out.append(np.frombuffer(data[at:at + size], dt).copy())
+        at += size
+    sw, sh = int(counts[7]), int(counts[8])
+    sky = np.frombuffer(data[at:at + sw * sh * 16], "<f4").reshape(sh, sw, 4).copy() if sw * sh else None
+    return Scene(triangles=out[0], world_details=out[1], mesh_triangles=out[2], meshes=out[3], instances=out[4],
+                 materials=out[5], lights=out[6], sky=sky,
+                 camera=Camera(tuple(cam[0:3]), tuple(cam[3:6]), tuple(cam[6:9]), tuple(cam[9:12])),
+                 sun=Sun(tuple(sun[0:3]), tuple(sun[3:6]), float(sun[6])))
+
+
 def material(base=(0.5, 0.5, 0.5), roughness=0.5, metalness=0.0, specular=0.04, clearcoat=0.0,
              clearcoat_roughness=0.05, sheen=0.0, emission=(0.0, 0.0, 0.0), fresnel_power=5.0) -> np.ndarray:
     """One material row; the defaults are `Material::default`

@@ -201,6 +201,30 @@ class Trace:
                     err)
         return out
 
+    def learned_denoise_checked(self, learner, denoiser, threshold: float, tile: int = 32) -> tuple[np.ndarray, int]:
+        """The learned denoiser with the repo denoiser as per-tile fallback:
+        tiles whose removed variance (mean squared log difference between the
+        noisy and the learned image) exceeds `threshold` take the repo
+        denoiser's output. Returns the image and the number of such tiles."""
+        out = np.empty((self.height, self.width, 4), np.float32)
+        n = C.c_uint32()
+        err = N.err_buffer()
+        self._check(N.lib().pfxh_trace_learned_denoise_checked(self._handle(), learner._h, denoiser._h,
+                                                               float(threshold), int(tile), out.ctypes.data,
+                                                               C.byref(n), err, len(err)), err)
+        return out, int(n.value)
+
+    def atrous(self) -> np.ndarray:
+        """denoise::filter (crates/trace/src/denoise.rs:1-62) on this trace's
+        colour, albedo and normal at its sample count, on the device."""
+        out = np.empty((self.height, self.width, 4), np.float32)
+        err = N.err_buffer()
+        st = N.lib().pfxh_trace_atrous(self._handle(), out.ctypes.data, err, len(err))
+        if st == N.INVALID:
+            raise ValueError(err.value.decode(errors="replace"))
+        self._check(st, err)
+        return out
+
     def samples(self) -> int:
         return int(N.lib().pfxh_trace_samples(self._handle()))
 

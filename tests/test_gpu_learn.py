@@ -160,3 +160,41 @@ def test_training_lowers_the_loss():
     first, last = float(losses[:10].mean()), float(losses[-10:].mean())
     print(f"loss {first:.4f} -> {last:.4f}", flush=True)
     assert last < 0.5 * first
+
+
+def test_checked_denoise_falls_back_per_tile():
+    """Tiles whose removed variance exceeds the threshold take the repo
+    denoiser's output; the others keep the learned output, byte for byte."""
+    print("learn checked fallback", flush=True)
+    from pfx_amd import Bounces, Detail, Trace, scenes
+    from pfx_amd import denoise as D
+
+    scene = scenes.terrain(seed=4, grid=(64, 32), meshes=2, mesh_tris=(200, 800), instances=8, lights=12,
+                           width=120, height=72)
+    with Trace.new_detailed(scene, Detail(bounces=Bounces.deep(4)), 120, 72) as t, \
+            Lm.Learner(weights=Lm.WEIGHTS) as L, D.Denoiser() as d:
+        t.sample(4, 3)
+        noisy = t.readback_color()
+        learned = t.learned_denoise(L)
+        repo = t.denoise(d)
+        keep, n_keep = t.learned_denoise_checked(L, d, float("inf"), 32)
+        swap, n_swap = t.learned_denoise_checked(L, d, -1.0, 32)
+        # a threshold between the tiles' values
+        r = []
+        for ty in range(0, 72, 32):
+            for tx in range(0, 120, 32):
+                a = np.log1p(np.maximum(noisy[ty:ty + 32, tx:tx + 32, :3], 0))
+                b = np.log1p(np.maximum(learned[ty:ty + 32, tx:tx + 32, :3], 0))
+                r.append(float(np.mean((a - b) ** 2)))
+        thr = float(np.median(r))
+        mixed, n_mixed = t.learned_denoise_checked(L, d, thr, 32)
+    assert n_keep == 0 and keep.tobytes() == learned.tobytes()
+    assert n_swap == len(r) and swap.tobytes() == repo.tobytes()
+    k = 0
+    for ty in range(0, 72, 32):
+        for tx in range(0, 120, 32):
+            src = repo if r[k] > thr * (1 + 1e-4) else (learned if r[k] < thr * (1 - 1e-4) else None)
+            if src is not None:
+                assert mixed[ty:ty + 32, tx:tx + 32].tobytes() == src[ty:ty + 32, tx:tx + 32].tobytes(), k
+            k += 1
+    assert 0 < n_mixed < len(r)
