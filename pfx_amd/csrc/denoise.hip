@@ -10,12 +10,14 @@
 //                1-pixel halo in LDS, and the nine taps. Bias, ReLU and the
 //                2x2 max pool are fused into the epilogue; nearest
 //                upsampling and concatenation are fused into the staging.
+//   k_conv_bf16  the same convolution with BF16 operands for the learned
+//                denoiser (v_mfma_f32_16x16x16_bf16, FP32 sums): inputs
+//                rounded to BF16 once as staged, two rows per wave for
+//                layers up to 64 outputs.
 //   k_bins / k_scale   autoexposure (process.wgsl:104-157)
 //   k_input / k_output PU transfer in and out of the network
 //                      (process.wgsl:33-102)
 #include <hip/hip_runtime.h>
-
-#include <type_traits>
 
 #include "denoise.h"
 
@@ -42,13 +44,9 @@ __device__ __forceinline__ v4s as_v4s(uint2 u) {
     return v4s{(short)(u.x & 0xffffu), (short)(u.x >> 16), (short)(u.y & 0xffffu), (short)(u.y >> 16)};
 }
 
-// BF16: the input chunk is rounded to bfloat16 as it is staged (uint2 = four
-// channels) and the packed weights are bfloat16 already, so the K loop feeds
-// v_mfma_f32_16x16x16_bf16 straight from LDS and memory.
-template <int NT, bool BF16>
+template <int NT>
 __global__ __launch_bounds__(256) void k_conv(ConvDev c) {
-    typedef typename std::conditional<BF16, uint2, float4>::type Staged;
-    __shared__ Staged patch[PH * PW * 4];    // [row][col][16 channels]
+    __shared__ float4 patch[PH * PW * 4];    // [row][col][16 channels]
     __shared__ float pooled[2][NT][64][2];   // odd rows' x-pooled values
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i = lane & 15, kk = lane >> 4;
@@ -82,33 +80,20 @@ __global__ __launch_bounds__(256) void k_conv(ConvDev c) {
                     v = *reinterpret_cast<const float4*>(c.src1 + ((size_t)y * w + x) * c.c1 + (cc - c.c0));
                 }
             }
-            if constexpr (BF16)
-                patch[e] = bf16x4_bits(v);
-            else
-                patch[e] = v;
+            patch[e] = v;
         }
         __syncthreads();
         for (int tap = 0; tap < 9; ++tap) {
             const int ky = tap / 3, kx = tap - ky * 3;
-            const Staged a = patch[((wave + ky) * PW + i + kx) * 4 + kk];
-            if constexpr (BF16) {
-                const uint16_t* wt = static_cast<const uint16_t*>(c.w) + ((size_t)ch * 9u + tap) * c.n_pad * 16u;
-                const v4s ab = as_v4s(a);
+            const float4 a = patch[((wave + ky) * PW + i + kx) * 4 + kk];
+            const float* wt = static_cast<const float*>(c.w) + ((size_t)ch * 9u + tap) * c.n_pad * 16u;
 #pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    const uint2 b = *reinterpret_cast<const uint2*>(wt + (size_t)(16 * t + i) * 16u + 4 * kk);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ab, as_v4s(b), acc[t], 0, 0, 0);
-                }
-            } else {
-                const float* wt = static_cast<const float*>(c.w) + ((size_t)ch * 9u + tap) * c.n_pad * 16u;
-#pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    const float4 b = *reinterpret_cast<const float4*>(wt + (size_t)(16 * t + i) * 16u + 4 * kk);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, acc[t], 0, 0, 0);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, acc[t], 0, 0, 0);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b.z, acc[t], 0, 0, 0);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, acc[t], 0, 0, 0);
-                }
+            for (int t = 0; t < NT; ++t) {
+                const float4 b = *reinterpret_cast<const float4*>(wt + (size_t)(16 * t + i) * 16u + 4 * kk);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b.z, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, acc[t], 0, 0, 0);
             }
         }
     }
@@ -153,11 +138,139 @@ __global__ __launch_bounds__(256) void k_conv(ConvDev c) {
         }
 }
 
-template <bool BF16>
-static void launch_conv_t(hipStream_t s, const ConvDev& c, dim3 grid) {
+// BF16 operands for the learned denoiser: a block computes 4 * ROWS rows x
+// 16 pixels, each wave ROWS rows, so with ROWS = 2 every weight fragment
+// read from memory feeds two matrix-core ops and the 2x2 pool sits inside a
+// wave; wide layers (more than 4 N tiles) keep one row per wave, where the
+// doubled accumulators would cost more occupancy than the reuse gains.
+template <int NT, int ROWS>
+__global__ __launch_bounds__(256) void k_conv_bf16(ConvDev c) {
+    constexpr int TR = 4 * ROWS, PR = TR + 2;
+    __shared__ uint2 patch[PR * PW * 4];  // [row][col][4 channels x 4]
+    __shared__ float pooled[2][NT][64][2];  // ROWS = 1: odd rows' x-pooled values
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 15, kk = lane >> 4;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TR;
+    const int h = (int)c.h, w = (int)c.w_;
+    const int sw = c.upsample ? w / 2 : w;
+    {
+        const size_t z = blockIdx.z, px = (size_t)h * w;
+        const size_t spx = c.upsample ? (size_t)(h / 2) * sw : px;
+        c.src0 += z * spx * c.c0;
+        if (c.src1) c.src1 += z * px * c.c1;
+        c.dst += z * (c.pool ? px / 4 : px) * c.n_pad;
+    }
+    v4f acc[ROWS][NT];
+    for (int r = 0; r < ROWS; ++r)
+        for (int t = 0; t < NT; ++t) acc[r][t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    const uint32_t chunks = (c.c0 + c.c1) / 16u;
+    for (uint32_t ch = 0; ch < chunks; ++ch) {
+        __syncthreads();
+        for (int e = tid; e < PR * PW * 4; e += 256) {
+            int q = e & 3, p = e >> 2;
+            int py = p / PW, px = p - py * PW;
+            int y = y0 + py - 1, x = x0 + px - 1;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (y >= 0 && y < h && x >= 0 && x < w) {
+                uint32_t cc = ch * 16u + (uint32_t)q * 4u;
+                if (cc < c.c0) {
+                    int yy = c.upsample ? y >> 1 : y, xx = c.upsample ? x >> 1 : x;
+                    v = *reinterpret_cast<const float4*>(c.src0 + ((size_t)yy * sw + xx) * c.c0 + cc);
+                } else {
+                    v = *reinterpret_cast<const float4*>(c.src1 + ((size_t)y * w + x) * c.c1 + (cc - c.c0));
+                }
+            }
+            patch[e] = bf16x4_bits(v);
+        }
+        __syncthreads();
+        const uint16_t* wb = static_cast<const uint16_t*>(c.w) + (size_t)ch * 9u * c.n_pad * 16u;
+#pragma unroll 3
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap - ky * 3;
+            v4s a[ROWS];
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r) a[r] = as_v4s(patch[((ROWS * wave + r + ky) * PW + i + kx) * 4 + kk]);
+            const uint16_t* wt = wb + (size_t)tap * c.n_pad * 16u;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const v4s b = as_v4s(*reinterpret_cast<const uint2*>(wt + (size_t)(16 * t + i) * 16u + 4 * kk));
+#pragma unroll
+                for (int r = 0; r < ROWS; ++r)
+                    acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[r], b, acc[r][t], 0, 0, 0);
+            }
+        }
+    }
+    // D[pixel 4*kk + v][channel i] of each N tile, rows y0 + ROWS * wave + r
+    if (!c.pool) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int n = 16 * t + i;
+            const float bias = c.bias ? c.bias[n] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r) {
+                const int y = y0 + ROWS * wave + r;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int x = x0 + 4 * kk + v;
+                    const float o = acc[r][t][v] + bias;
+                    if (y < h && x < w) c.dst[((size_t)y * w + x) * c.n_pad + n] = c.relu ? fmaxf(o, 0.0f) : o;
+                }
+            }
+        }
+        return;
+    }
+    // 2x2 max pool after the ReLU: x pairs (v 0-1, 2-3) sit in one lane; the
+    // row pair sits in one wave (ROWS = 2) or in waves (0,1), (2,3)
+    float m[NT][2];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const float bias = c.bias ? c.bias[16 * t + i] : 0.0f;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            float v = fmaxf(acc[0][t][2 * p] + bias, acc[0][t][2 * p + 1] + bias);
+            if (ROWS == 2) v = fmaxf(v, fmaxf(acc[ROWS - 1][t][2 * p] + bias, acc[ROWS - 1][t][2 * p + 1] + bias));
+            m[t][p] = fmaxf(v, 0.0f);
+        }
+    }
+    int oy;
+    if (ROWS == 1) {
+        if (wave & 1)
+            for (int t = 0; t < NT; ++t)
+                for (int p = 0; p < 2; ++p) pooled[wave >> 1][t][lane][p] = m[t][p];
+        __syncthreads();
+        if (wave & 1) return;
+        for (int t = 0; t < NT; ++t)
+            for (int p = 0; p < 2; ++p) m[t][p] = fmaxf(m[t][p], pooled[wave >> 1][t][lane][p]);
+        oy = (y0 + wave) / 2;
+    } else {
+        oy = (y0 + 2 * wave) / 2;
+    }
+    const int ho = h / 2, wo = w / 2;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int ox = x0 / 2 + 2 * kk + p;
+            if (oy < ho && ox < wo) c.dst[((size_t)oy * wo + ox) * c.n_pad + 16 * t + i] = m[t][p];
+        }
+}
+
+static void launch_conv_fp32(hipStream_t s, const ConvDev& c, dim3 grid) {
     switch (c.n_pad / 16u) {
 #define PFXD_CONV(n) \
-    case n: hipLaunchKernelGGL((k_conv<n, BF16>), grid, dim3(256), 0, s, c); break;
+    case n: hipLaunchKernelGGL((k_conv<n>), grid, dim3(256), 0, s, c); break;
+        PFXD_CONV(1) PFXD_CONV(2) PFXD_CONV(3) PFXD_CONV(4) PFXD_CONV(5) PFXD_CONV(6)
+        PFXD_CONV(7) PFXD_CONV(8) PFXD_CONV(9) PFXD_CONV(10) PFXD_CONV(11) PFXD_CONV(12)
+        PFXD_CONV(13) PFXD_CONV(14)
+#undef PFXD_CONV
+        default: break;  // the host refuses other widths
+    }
+}
+
+static void launch_conv_bf16(hipStream_t s, const ConvDev& c, dim3 grid) {
+    switch (c.n_pad / 16u) {
+#define PFXD_CONV(n) \
+    case n: hipLaunchKernelGGL((k_conv_bf16<n, (n <= 4 ? 2 : 1)>), grid, dim3(256), 0, s, c); break;
         PFXD_CONV(1) PFXD_CONV(2) PFXD_CONV(3) PFXD_CONV(4) PFXD_CONV(5) PFXD_CONV(6)
         PFXD_CONV(7) PFXD_CONV(8) PFXD_CONV(9) PFXD_CONV(10) PFXD_CONV(11) PFXD_CONV(12)
         PFXD_CONV(13) PFXD_CONV(14)
@@ -167,11 +280,13 @@ static void launch_conv_t(hipStream_t s, const ConvDev& c, dim3 grid) {
 }
 
 void launch_conv(hipStream_t s, const ConvDev& c) {
-    dim3 grid((c.w_ + TW - 1) / TW, (c.h + TH - 1) / TH, c.batch ? c.batch : 1u);
-    if (c.bf16)
-        launch_conv_t<true>(s, c, grid);
-    else
-        launch_conv_t<false>(s, c, grid);
+    const uint32_t z = c.batch ? c.batch : 1u;
+    if (c.bf16) {
+        const uint32_t rows = (c.n_pad / 16u <= 4u ? 2u : 1u) * 4u;
+        launch_conv_bf16(s, c, dim3((c.w_ + TW - 1) / TW, (c.h + rows - 1) / rows, z));
+    } else {
+        launch_conv_fp32(s, c, dim3((c.w_ + TW - 1) / TW, (c.h + TH - 1) / TH, z));
+    }
 }
 
 // ---- autoexposure (process.wgsl:104-157) --------------------------------------

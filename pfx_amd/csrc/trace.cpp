@@ -424,6 +424,7 @@ pfxh_status pfxh_trace_set_camera(pfxh_trace* t, const pfxh_camera* c, char* err
         t->S.up[k] = c->up[k];
     }
     t->samples = 0;
+    t->probe_fresh = false;
     return PFXH_OK;
 }
 
@@ -443,6 +444,7 @@ pfxh_status pfxh_trace_set_window(pfxh_trace* t, uint32_t fw, uint32_t fh, uint3
     t->S.window[2] = x;
     t->S.window[3] = y;
     t->samples = 0;
+    t->probe_fresh = false;
     return PFXH_OK;
 }
 
@@ -464,6 +466,7 @@ pfxh_status pfxh_trace_set_sun(pfxh_trace* t, const pfxh_sun* sun, char* err, si
     }
     t->S.sun_intensity = sun->intensity;
     t->samples = 0;
+    t->probe_fresh = false;
     return PFXH_OK;
 }
 
@@ -480,6 +483,7 @@ pfxh_status pfxh_trace_set_lens(pfxh_trace* t, float sx, float sy, float apertur
     t->S.lens[2] = aperture * 0.5f;
     t->S.lens[3] = focus;
     t->samples = 0;
+    t->probe_fresh = false;
     return PFXH_OK;
 }
 
@@ -495,6 +499,7 @@ pfxh_status pfxh_trace_set_projection(pfxh_trace* t, uint32_t kind, float w, flo
     }
     place_projection(t->S, kind, w, h);
     t->samples = 0;
+    t->probe_fresh = false;
     return PFXH_OK;
 }
 
@@ -605,7 +610,10 @@ pfxh_status pfxh_trace_learned_denoise(pfxh_trace* t, pfxh_learner* l, float* ou
         float4* c = t->images;
         launch_develop(t->stream, t->A, (uint32_t)pixels, c, c + pixels, c + 2 * pixels);
         if (!t->probe) t->probe = (pfxh_probe_hit*)t->alloc(pixels * sizeof(pfxh_probe_hit), "probe");
-        launch_probe(t->stream, t->S, t->W, t->probe);
+        if (!t->probe_fresh) {  // depth depends on the view only: kept until a setter changes it
+            launch_probe(t->stream, t->S, t->W, t->probe);
+            t->probe_fresh = true;
+        }
         HIP_TRY(hipGetLastError());
     });
     if (st != PFXH_OK || !pixels) return st;
@@ -626,7 +634,10 @@ pfxh_status pfxh_trace_learned_denoise_checked(pfxh_trace* t, pfxh_learner* l, p
         float4* c = t->images;
         launch_develop(t->stream, t->A, (uint32_t)pixels, c, c + pixels, c + 2 * pixels);
         if (!t->probe) t->probe = (pfxh_probe_hit*)t->alloc(pixels * sizeof(pfxh_probe_hit), "probe");
-        launch_probe(t->stream, t->S, t->W, t->probe);
+        if (!t->probe_fresh) {  // depth depends on the view only: kept until a setter changes it
+            launch_probe(t->stream, t->S, t->W, t->probe);
+            t->probe_fresh = true;
+        }
         HIP_TRY(hipGetLastError());
     });
     if (st != PFXH_OK) return st;
@@ -707,6 +718,7 @@ pfxh_status pfxh_trace_probe(pfxh_trace* t, pfxh_probe_hit* out, char* err, size
         if (!pixels) return;
         if (!t->probe) t->probe = (pfxh_probe_hit*)t->alloc(pixels * sizeof(pfxh_probe_hit), "probe");
         launch_probe(t->stream, t->S, t->W, t->probe);
+        t->probe_fresh = true;
         HIP_TRY(hipMemcpyAsync(out, t->probe, pixels * sizeof(pfxh_probe_hit), hipMemcpyDeviceToHost, t->stream));
         HIP_TRY(hipStreamSynchronize(t->stream));
         HIP_TRY(hipGetLastError());

@@ -48,6 +48,7 @@ struct pfxh_trace {
     float4* snaps = nullptr;
     float4* images = nullptr;           // color, albedo, normal for readback
     pfxh_probe_hit* probe = nullptr;
+    bool probe_fresh = false;           // probe holds the current view's hits
     float* tile_buf = nullptr;          // per-tile check of the learned denoiser
     float4* atrous = nullptr;           // a-trous passes: two images
     uint32_t tile_cap = 0;

@@ -198,3 +198,30 @@ def test_checked_denoise_falls_back_per_tile():
                 assert mixed[ty:ty + 32, tx:tx + 32].tobytes() == src[ty:ty + 32, tx:tx + 32].tobytes(), k
             k += 1
     assert 0 < n_mixed < len(r)
+
+
+def test_learned_denoise_follows_the_view():
+    """The depth guide is kept between denoiser calls and refreshed when the
+    view changes: after set_camera the result equals a fresh trace's."""
+    print("learn view change", flush=True)
+    from pfx_amd import Bounces, Camera, Detail, Trace, scenes
+
+    scene = scenes.terrain(seed=4, grid=(64, 32), meshes=2, mesh_tris=(200, 800), instances=8, lights=12,
+                           width=96, height=64)
+    other = Camera.look((10.0, 25.0, -30.0), (0.0, 0.0, 5.0), 50.0, 96 / 64)
+    detail = Detail(bounces=Bounces.deep(4))
+    with Lm.Learner(weights=Lm.WEIGHTS) as L:
+        with Trace.new_detailed(scene, detail, 96, 64) as t:
+            t.sample(4, 5)
+            first = t.learned_denoise(L)
+            again = t.learned_denoise(L)
+            t.set_camera(other)
+            t.sample(4, 5)
+            moved = t.learned_denoise(L)
+        with Trace.new_detailed(scene, detail, 96, 64) as fresh:
+            fresh.set_camera(other)
+            fresh.sample(4, 5)
+            want = fresh.learned_denoise(L)
+    assert first.tobytes() == again.tobytes()
+    assert moved.tobytes() == want.tobytes()
+    assert moved.tobytes() != first.tobytes()
