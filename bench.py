@@ -54,6 +54,8 @@ HBM_PEAK_GBS = 8000.0
 # entry (4), origin and direction (32), hit record (16) and hit instance
 # record (4)
 NODE_BYTES, PRIM_BYTES, RAY_IO_BYTES = 112, 48, 4 + 32 + 16 + 4
+# the reference's records (crates/trace/src/bvh.rs:5-13 Node, gpu.rs:17-26 GpuTriangle)
+REF_NODE_BYTES, REF_PRIM_BYTES = 48, 96
 # HBM bytes per closest-hit ray measured with rocprofv3 counters (FETCH_SIZE
 # and WRITE_SIZE of k_extend, calibrated on a copy kernel); absent -> null
 TRAFFIC_FILE = ROOT / "profiles" / "k_extend_traffic.json"
@@ -75,6 +77,7 @@ def counting_pass(trace: Trace, camera, seed: int) -> dict:
     rays = max(s["closest_rays"], 1)
     return {
         "nodes_per_ray": s["nodes_visited"] / rays,
+        "boxes_per_ray": s["boxes_tested"] / rays,
         "prims_per_ray": s["prims_tested"] / rays,
         "shadow_nodes_per_ray": s["shadow_nodes"] / max(s["shadow_rays"], 1),
         "shadow_prims_per_ray": s["shadow_prims"] / max(s["shadow_rays"], 1),
@@ -294,6 +297,10 @@ def main() -> int:
     rays_per_launch = s["closest_rays"] / launches
     avg_launch_ms = s["trace_ms"] / launches
     achieved = bytes_per_ray * rays_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    # the same walk priced in the reference's layout: a 48-byte binary node
+    # per box tested and a 96-byte triangle record per triangle tested
+    ref_bytes_per_ray = per_ray["boxes_per_ray"] * REF_NODE_BYTES + per_ray["prims_per_ray"] * REF_PRIM_BYTES \
+        + RAY_IO_BYTES
     traffic = None
     if TRAFFIC_FILE.exists():
         measured = json.loads(TRAFFIC_FILE.read_text())
@@ -331,6 +338,8 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "bytes_per_ray": round(bytes_per_ray, 1),
+            "reference_layout_bytes_per_ray": round(ref_bytes_per_ray, 1),
+            "reference_layout_gbs": round(ref_bytes_per_ray * rays_per_launch / (avg_launch_ms * 1e-3) / 1e9, 1),
             "rays_per_launch": round(rays_per_launch, 1),
             "avg_launch_ms": round(avg_launch_ms, 4),
             **{k: round(v, 2) for k, v in per_ray.items()},

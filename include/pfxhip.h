@@ -210,6 +210,8 @@ typedef struct pfxh_stats {
     double shadow_ms;       /* GPU time of any-hit kernels */
     double shade_ms;        /* GPU time of raygen, shading and accumulation kernels */
     uint32_t launches;      /* closest-hit launches behind trace_ms */
+    uint64_t boxes_tested;  /* closest-hit child boxes tested (the reference's binary node
+                             * visits), when counting is on */
 } pfxh_stats;
 
 typedef struct pfxh_trace pfxh_trace;

@@ -164,6 +164,7 @@ class StatsC(C.Structure):
         ("shadow_ms", C.c_double),
         ("shade_ms", C.c_double),
         ("launches", C.c_uint32),
+        ("boxes_tested", C.c_uint64),
     ]
 
 

@@ -85,6 +85,7 @@ enum Counter : uint32_t {
     C_PRIMS = 5 * SLOT + 2,
     C_SNODES = 5 * SLOT + 4,
     C_SPRIMS = 5 * SLOT + 6,
+    C_BOXES = 5 * SLOT + 8,            // closest-hit child boxes tested
     C_FETCH = 8 * SLOT,                // PARTS work counters of k_extend
     C_SFETCH = C_FETCH + PARTS * SLOT, // PARTS work counters of k_shadow
     C_N = C_SFETCH + PARTS * SLOT,     // batch: rays, shadow rays, shadow paths of the bounce

@@ -224,6 +224,14 @@ struct Closest {
 // Test the four children of a Node4, then continue with the first hit
 // child and push the other hit children right to left, so children pop in
 // the reference's left-first order. Returns false when no child was hit.
+// Children whose boxes a visit of node `index` tests: the binary node
+// visits of the reference's walk that the 4-wide visit stands for.
+template <class G>
+DEV uint32_t child_boxes(const G& S, uint32_t index) {
+    uint4 rf = reinterpret_cast<const uint4*>(S.nodes + index)[6];
+    return 1u + (rf.y != REF_NONE) + (rf.z != REF_NONE) + (rf.w != REF_NONE);
+}
+
 template <class G>
 DEV bool visit4(const G& S, uint32_t index, const Ray& R, float t, Stack& st, uint32_t& cur) {
     const float4* np = reinterpret_cast<const float4*>(S.nodes + index);
@@ -1056,7 +1064,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
-    uint32_t nodes = 0, prims = 0;
+    uint32_t nodes = 0, prims = 0, boxes = 0;
     Walk<false> w;
     Geo g{};
     if (BATCH) n = W.counters[C_N];
@@ -1082,7 +1090,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         bool out = live && !in;
         if (inner_round(in, out)) {
             if (in) {
-                if (COUNT) nodes++;
+                if (COUNT) {
+                    nodes++;
+                    boxes += child_boxes(walk_scene<BATCH>(S, g), w.cur);
+                }
                 if (!visit4(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) {
                     store_closest(W, ray, w.h);
                     live = false;
@@ -1103,6 +1114,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
     if (COUNT) {
         atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_NODES), (unsigned long long)nodes);
         atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_PRIMS), (unsigned long long)prims);
+        atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_BOXES), (unsigned long long)boxes);
     }
 }
 

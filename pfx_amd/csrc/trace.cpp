@@ -736,6 +736,7 @@ pfxh_status pfxh_trace_stats(const pfxh_trace* t, pfxh_stats* out) {
     auto u64 = [&](int k) { return (uint64_t)c[k] | ((uint64_t)c[k + 1] << 32); };
     out->nodes_visited = u64(C_NODES);
     out->prims_tested = u64(C_PRIMS);
+    out->boxes_tested = u64(C_BOXES);
     out->shadow_nodes = u64(C_SNODES);
     out->shadow_prims = u64(C_SPRIMS);
     return PFXH_OK;

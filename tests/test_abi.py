@@ -78,3 +78,46 @@ def test_invalid_projection_is_refused_before_any_device_work(native_lib):
     assert native_lib.pfxh_trace_new(C.byref(sc), C.byref(flat), 8, 8, C.byref(out), err, 256) == N.SETUP
     assert b"needs forward, right and up" in err.value
     assert not out.value
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every struct the bindings mirror has the header's size, as a C
+    compiler lays it out."""
+    import subprocess
+    from pathlib import Path
+
+    pairs = {"pfxh_camera": N.Camera, "pfxh_sun": N.Sun, "pfxh_environment": N.Environment,
+             "pfxh_detail": N.DetailC, "pfxh_adaptive": N.AdaptiveC, "pfxh_converged": N.ConvergedC,
+             "pfxh_scene": N.SceneC, "pfxh_stats": N.StatsC, "pfxh_denoise_tile": N.DenoiseTile,
+             "pfxh_checkpoint": N.CheckpointC, "pfxh_learn_layer_info": N.LearnLayerC,
+             "pfxh_batch_still": N.BatchStillC, "pfxh_batch_tuning": N.BatchTuningC}
+    header = Path(__file__).resolve().parent.parent / "include" / "pfxhip.h"
+    src = tmp_path / "sizes.c"
+    src.write_text(f'#include "{header}"\n#include <stdio.h>\nint main(void) {{\n'
+                   + "".join(f'    printf("{k} %zu\\n", sizeof({k}));\n' for k in pairs) + "    return 0;\n}\n")
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-std=c99", str(src), "-o", str(exe)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    for k, t in pairs.items():
+        assert int(out[k]) == C.sizeof(t), (k, out[k], C.sizeof(t))
+
+
+def test_host_side_argument_errors(native_lib):
+    """Calls the library refuses before touching a device."""
+    lib = native_lib
+    err = N.err_buffer()
+    img = (C.c_float * 16)()
+    assert lib.pfxh_atrous(2, 2, img, img, img, 0, img, err, len(err)) == N.INVALID
+    assert b"invalid denoise inputs" in err.value
+    assert lib.pfxh_atrous(2, 2, None, img, img, 4, img, err, len(err)) == N.INVALID
+    out = C.c_void_p()
+    assert lib.pfxh_batch_new(None, 0, None, C.byref(out), err, len(err)) == N.INVALID
+    assert lib.pfxh_batch_trace(None, 0) is None
+    assert lib.pfxh_batch_count(None) == 0
+    assert lib.pfxh_learner_set_params(None, None, 0, err, len(err)) == N.INVALID
+    assert lib.pfxh_learner_train(None, 1, 1, C.c_float(1e-3), 1, None, err, len(err)) == N.INVALID
+    assert lib.pfxh_trace_learned_denoise_checked(None, None, None, C.c_float(1.0), 32, None, None, err,
+                                                  len(err)) == N.INVALID
+    info = N.LearnLayerC()
+    assert lib.pfxh_learn_layer(N.LEARN_LAYERS, C.byref(info)) == N.INVALID
