@@ -293,6 +293,16 @@ pfxh_status pfxh_layout_lights(const pfxh_light* lights, uint32_t count, float* 
                                uint32_t* row_count, uint32_t* looped, uint32_t* tree, char* err,
                                size_t err_len);
 
+/* The split step of the light-tree build, emitters.select_nth_unstable_by
+ * on the centroid (crates/trace/src/tree.rs:281-283): Rust core's
+ * slice::select_nth_unstable_by (Rust 1.97) over `len` (key, id) pairs, keys
+ * compared with f32::total_cmp, permuted in place. mode 0 selects `index`
+ * (result = index); mode 1 runs one partition around keys[index] (result =
+ * the count of smaller keys, now the pivot's position); mode 2 runs the
+ * median-of-medians fallback alone. index < len. */
+pfxh_status pfxh_layout_select(float* keys, uint32_t* ids, size_t len, size_t index, int mode,
+                               size_t* result);
+
 /* lights::build_emitters (crates/trace/src/lights.rs:391-440) for the
  * emissive triangles of a scene: entries (16 floats each, `GpuEmitter`,
  * header first) and the CDF. Call with entries == NULL to learn the count. */

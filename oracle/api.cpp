@@ -3,6 +3,7 @@
 #include <string.h>
 
 #include "oracle.h"
+#include "select_nth.h"
 
 using namespace oracle;
 
@@ -122,6 +123,39 @@ uint32_t or_rounds(const pfxh_adaptive* a, uint32_t* out, uint32_t cap) {
     uint32_t n = (uint32_t)r.size() < cap ? (uint32_t)r.size() : cap;
     memcpy(out, r.data(), n * 4);
     return (uint32_t)r.size();
+}
+
+// Rust's select_nth_unstable_by (select_nth.h) over (key, id) pairs, keys
+// compared with f32::total_cmp as the light tree does (tree.rs:281-283);
+// mode 1 runs one `partition` around keys[index] instead, mode 2 the
+// median-of-medians fallback alone.
+struct KeyId {
+    float key;
+    uint32_t id;
+};
+static bool key_total_less(const KeyId& a, const KeyId& b) {
+    int32_t x, y;
+    memcpy(&x, &a.key, 4);
+    memcpy(&y, &b.key, 4);
+    x ^= (int32_t)((uint32_t)(x >> 31) >> 1);
+    y ^= (int32_t)((uint32_t)(y >> 31) >> 1);
+    return x < y;
+}
+size_t or_select_nth(float* keys, uint32_t* ids, size_t len, size_t index, int mode) {
+    std::vector<KeyId> v(len);
+    for (size_t i = 0; i < len; ++i) v[i] = KeyId{keys[i], ids[i]};
+    size_t r = index;
+    if (mode == 1)
+        r = rsel::partition(v.data(), len, index, key_total_less);
+    else if (mode == 2)
+        rsel::median_of_medians(v.data(), len, index, key_total_less);
+    else
+        rsel::select_nth_unstable_by(v.data(), len, index, key_total_less);
+    for (size_t i = 0; i < len; ++i) {
+        keys[i] = v[i].key;
+        ids[i] = v[i].id;
+    }
+    return r;
 }
 
 }  // extern "C"

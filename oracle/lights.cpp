@@ -9,6 +9,7 @@
 #include <cmath>
 
 #include "oracle.h"
+#include "select_nth.h"
 
 namespace oracle {
 
@@ -328,10 +329,19 @@ static GLight tree_row(const Measure& m, size_t count, size_t link) {
     g.d[3] = 0.0f;
     return g;
 }
-// tree.rs:263-293. Rust's select_nth_unstable_by is replaced by a stable
-// sort on the centroid (ties keep their current order): it yields the same
-// two sets around `middle`, but the order inside each side -- and so the
-// leaf order -- is our own.
+// f32::total_cmp(a, b) == Less: IEEE 754 totalOrder on the bit patterns
+// (negative values have all but the sign bit flipped, then a signed compare)
+static bool total_less(float a, float b) {
+    int32_t x, y;
+    memcpy(&x, &a, 4);
+    memcpy(&y, &b, 4);
+    x ^= (int32_t)((uint32_t)(x >> 31) >> 1);
+    y ^= (int32_t)((uint32_t)(y >> 31) >> 1);
+    return x < y;
+}
+// tree.rs:263-293; the split around `middle` is Rust's
+// select_nth_unstable_by on the centroid (select_nth.h), so each side keeps
+// the order that selection leaves, which becomes the leaf order.
 static void grow(Emitter* emitters, size_t count, size_t start, size_t base, size_t root, std::vector<GLight>& nodes) {
     size_t at = nodes.size();
     nodes.push_back(GLight{});
@@ -341,8 +351,9 @@ static void grow(Emitter* emitters, size_t count, size_t start, size_t base, siz
     Measure measure = should_split(emitters, count, &split, &middle, &dim);
     if (split) {
         if (dim >= 0)
-            std::stable_sort(emitters, emitters + count,
-                             [dim](const Emitter& l, const Emitter& r) { return l.centroid[dim] < r.centroid[dim]; });
+            rsel::select_nth_unstable_by(emitters, count, middle, [dim](const Emitter& l, const Emitter& r) {
+                return total_less(l.centroid[dim], r.centroid[dim]);
+            });
         grow(emitters, middle, start, base, root, nodes);
         size_t right_at = root + nodes.size();
         grow(emitters + middle, count - middle, start + middle, base, root, nodes);

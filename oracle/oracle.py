@@ -70,6 +70,7 @@ def lib() -> C.CDLL:
             "or_emitters": (_U, [_P, _U, _U, _U, _P, _P]),
             "or_sky_cdf": (None, [_P, _U, _U, _U, _U, _P, _P]),
             "or_rounds": (_U, [_P, _P, _U]),
+            "or_select_nth": (C.c_size_t, [_P, _P, C.c_size_t, C.c_size_t, C.c_int]),
             "or_adaptive_error": (_F, [_P, _F, _P, _F]),
         }
         for name, (res, args) in sig.items():

@@ -805,8 +805,210 @@ void tree_row(const Measure& m, size_t count, size_t link, float* g) {
     g[14] = (float)link;
     g[15] = 0.0f;
 }
-// tree.rs:263-293; the selection around `middle` is a stable sort on the
-// centroid (a deterministic stand-in for select_nth_unstable_by).
+// Rust's slice::select_nth_unstable_by (rust-version 1.97, Cargo.toml:7;
+// library/core/src/slice/select.rs with sort/shared/pivot.rs,
+// sort/unstable/quicksort.rs and sort/shared/smallsort.rs), which the tree
+// build uses to split its emitters (tree.rs:281-283). The order it leaves on
+// each side becomes the leaf order, so the same moves are made here:
+// introselect with at most 16 partition rounds on pseudo-median pivots
+// (median of three below 64 elements, recursive ninther above), insertion
+// sort at 16 elements or fewer, the branchless cyclic Lomuto partition (the
+// 72-byte Rust element is under the 96-byte limit for it), a run equal to
+// the previous pivot split off whole, and the median-of-medians fallback.
+template <class E, class KeyOf>
+struct Selector {
+    E* v;
+    KeyOf key;
+
+    bool less(const E& a, const E& b) const { return total_less(key(a), key(b)); }
+    // smallsort.rs: insertion sort of [lo, hi), shifting each new element left
+    void insertion(size_t lo, size_t hi) const {
+        for (size_t t = lo + 1; t < hi; ++t) {
+            if (!less(v[t], v[t - 1])) continue;
+            E held = v[t];
+            size_t j = t;
+            do {
+                v[j] = v[j - 1];
+                --j;
+            } while (j > lo && less(held, v[j - 1]));
+            v[j] = held;
+        }
+    }
+    // pivot.rs median3 / median3_rec / choose_pivot, as indices
+    size_t median3(size_t a, size_t b, size_t c) const {
+        bool x = less(v[a], v[b]), y = less(v[a], v[c]);
+        if (x != y) return a;
+        return (less(v[b], v[c]) != x) ? c : b;
+    }
+    size_t ninther_rec(size_t a, size_t b, size_t c, size_t n) const {
+        if (n * 8 >= 64) {
+            size_t s = n / 8;
+            a = ninther_rec(a, a + 4 * s, a + 7 * s, s);
+            b = ninther_rec(b, b + 4 * s, b + 7 * s, s);
+            c = ninther_rec(c, c + 4 * s, c + 7 * s, s);
+        }
+        return median3(a, b, c);
+    }
+    // (the pivot's position relative to lo)
+    size_t pivot_of(size_t lo, size_t len) const {
+        size_t e = len / 8;
+        if (len < 64) return median3(lo, lo + 4 * e, lo + 7 * e) - lo;
+        return ninther_rec(lo, lo + 4 * e, lo + 7 * e, e) - lo;
+    }
+    // quicksort.rs partition: pivot to the front, the cyclic Lomuto pass over
+    // the rest (equal in effect to swap-Lomuto over it rotated left by one),
+    // pivot to the boundary; `or_equal` partitions by !(pivot < x)
+    size_t partition(size_t lo, size_t len, size_t p, bool or_equal) const {
+        std::swap(v[lo], v[lo + p]);
+        const E pivot = v[lo];
+        auto goes_left = [&](const E& x) { return or_equal ? !less(pivot, x) : less(x, pivot); };
+        size_t base = lo + 1, n = len - 1, lt = 0;
+        if (n > 0) {
+            E lifted = v[base];
+            size_t gap = base;
+            for (size_t r = 1; r <= n; ++r) {
+                E next = r == n ? lifted : v[base + r];
+                bool left = goes_left(next);
+                v[gap] = v[base + lt];
+                v[base + lt] = next;
+                gap = base + r;  // unused after the last step
+                lt += left ? 1 : 0;
+            }
+        }
+        std::swap(v[lo], v[lo + lt]);
+        return lt;
+    }
+    size_t first_min(size_t lo, size_t len) const {
+        size_t m = lo;
+        for (size_t i = lo + 1; i < lo + len; ++i)
+            if (less(v[i], v[m])) m = i;
+        return m;
+    }
+    size_t first_max(size_t lo, size_t len) const {
+        size_t m = lo;
+        for (size_t i = lo + 1; i < lo + len; ++i)
+            if (less(v[m], v[i])) m = i;
+        return m;
+    }
+    // select.rs median_idx / ninther / median_of_ninthers / median_of_medians
+    size_t median_idx(size_t a, size_t b, size_t c) const {
+        if (less(v[c], v[a])) std::swap(a, c);
+        if (less(v[c], v[b])) return c;
+        if (less(v[b], v[a])) return a;
+        return b;
+    }
+    void ninther(size_t a, size_t b, size_t c, size_t d, size_t e, size_t f, size_t g, size_t h, size_t i) const {
+        b = median_idx(a, b, c);
+        h = median_idx(g, h, i);
+        if (less(v[h], v[b])) std::swap(b, h);
+        if (less(v[f], v[d])) std::swap(d, f);
+        if (!less(v[e], v[d])) {
+            if (less(v[f], v[e])) {
+                d = f;
+            } else {
+                if (less(v[e], v[b]))
+                    std::swap(v[e], v[b]);
+                else if (less(v[h], v[e]))
+                    std::swap(v[e], v[h]);
+                return;
+            }
+        }
+        if (less(v[d], v[b]))
+            d = b;
+        else if (less(v[h], v[d]))
+            d = h;
+        std::swap(v[d], v[e]);
+    }
+    size_t median_of_ninthers(size_t lo, size_t len) const {
+        size_t frac = len <= 1024 ? len / 12 : (len <= 131072 ? len / 64 : len / 1024);
+        size_t half = frac / 2, mlo = len / 2 - half, mhi = frac + mlo, gap = (len - 9 * frac) / 4;
+        size_t a = mlo - 4 * frac - gap, b = mhi + gap;
+        for (size_t i = mlo; i < mhi; ++i, a += 3, b += 3)
+            ninther(lo + a, lo + i - frac, lo + b, lo + a + 1, lo + i, lo + b + 1, lo + a + 2, lo + i + frac,
+                    lo + b + 2);
+        median_of_medians(lo + mlo, frac, half);
+        return partition(lo, len, mlo + half, false);
+    }
+    void median_of_medians(size_t lo, size_t len, size_t k) const {
+        for (;;) {
+            if (len <= 16) {
+                insertion(lo, lo + len);
+                return;
+            }
+            if (k == len - 1) {
+                std::swap(v[first_max(lo, len)], v[lo + k]);
+                return;
+            }
+            if (k == 0) {
+                std::swap(v[first_min(lo, len)], v[lo]);
+                return;
+            }
+            size_t p = median_of_ninthers(lo, len);
+            if (p == k) return;
+            if (p > k) {
+                len = p;
+            } else {
+                lo += p + 1;
+                len -= p + 1;
+                k -= p + 1;
+            }
+        }
+    }
+    // select.rs partition_at_index + partition_at_index_loop
+    void select(size_t len, size_t index) const {
+        if (index >= len) return;
+        if (index == len - 1) {
+            std::swap(v[first_max(0, len)], v[index]);
+            return;
+        }
+        if (index == 0) {
+            std::swap(v[first_min(0, len)], v[0]);
+            return;
+        }
+        size_t lo = 0;
+        bool floor_set = false;
+        E floor{};
+        for (int rounds = 16;; --rounds) {
+            if (len <= 16) {
+                insertion(lo, lo + len);
+                return;
+            }
+            if (rounds == 0) {
+                median_of_medians(lo, len, index);
+                return;
+            }
+            size_t p = pivot_of(lo, len);
+            if (floor_set && !less(floor, v[lo + p])) {
+                size_t mid = partition(lo, len, p, true) + 1;
+                if (mid > index) return;
+                lo += mid;
+                len -= mid;
+                index -= mid;
+                floor_set = false;
+                continue;
+            }
+            size_t mid = partition(lo, len, p, false);
+            if (mid < index) {
+                floor = v[lo + mid];
+                floor_set = true;
+                lo += mid + 1;
+                len -= mid + 1;
+                index -= mid + 1;
+            } else if (mid > index) {
+                len = mid;
+            } else {
+                return;
+            }
+        }
+    }
+};
+
+struct CentroidOf {
+    int dim;
+    float operator()(const LightEntry& e) const { return e.centroid[dim]; }
+};
+
+// tree.rs:263-293
 void grow(LightEntry* e, size_t count, size_t start, size_t base, size_t root, std::vector<float>& nodes) {
     size_t at = nodes.size() / 16;
     nodes.resize(nodes.size() + 16, 0.0f);
@@ -816,8 +1018,7 @@ void grow(LightEntry* e, size_t count, size_t start, size_t base, size_t root, s
     Measure measure = should_split(e, count, &split, &middle, &dim);
     if (split) {
         if (dim >= 0)
-            std::stable_sort(e, e + count,
-                             [dim](const LightEntry& l, const LightEntry& r) { return total_less(l.centroid[dim], r.centroid[dim]); });
+            Selector<LightEntry, CentroidOf>{e, CentroidOf{dim}}.select(count, middle);
         grow(e, middle, start, base, root, nodes);
         size_t right_at = root + nodes.size() / 16;
         grow(e + middle, count - middle, start + middle, base, root, nodes);
@@ -863,6 +1064,33 @@ float tri_area(const float* v) {
 }
 
 }  // namespace
+
+size_t select_keys(float* keys, uint32_t* ids, size_t len, size_t index, int mode) {
+    struct KeyId {
+        float key;
+        uint32_t id;
+    };
+    struct KeyOf {
+        float operator()(const KeyId& k) const { return k.key; }
+    };
+    std::vector<KeyId> v(len);
+    for (size_t i = 0; i < len; ++i) v[i] = KeyId{keys[i], ids[i]};
+    Selector<KeyId, KeyOf> sel{v.data(), KeyOf{}};
+    size_t r = index;
+    if (index < len) {
+        if (mode == 1)
+            r = sel.partition(0, len, index, false);
+        else if (mode == 2)
+            sel.median_of_medians(0, len, index);
+        else
+            sel.select(len, index);
+    }
+    for (size_t i = 0; i < len; ++i) {
+        keys[i] = v[i].key;
+        ids[i] = v[i].id;
+    }
+    return r;
+}
 
 RefBvh build_bvh(const float* v9, size_t count) {
     std::vector<Box> boxes(count);

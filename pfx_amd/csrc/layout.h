@@ -60,6 +60,11 @@ std::string build_scene(const pfxh_scene& scene, HostScene& out, bool emitter_sa
 std::string lay_levels(const pfxh_scene& scene, RefLevels& out, std::vector<uint32_t>* slot_prim);
 
 // tree::lay (crates/trace/src/tree.rs:220-243)
+// Rust's slice::select_nth_unstable_by as the light tree uses it, over
+// (key, id) pairs compared by f32::total_cmp: mode 0 selects `index`, mode 1
+// runs one partition around keys[index] and returns the number of smaller
+// keys, mode 2 runs the median-of-medians fallback alone.
+size_t select_keys(float* keys, uint32_t* ids, size_t len, size_t index, int mode);
 void lay_lights(const pfxh_light* lights, size_t count, std::vector<float>& rows, uint32_t* looped,
                 uint32_t* tree);
 bool light_valid(const pfxh_light& l);

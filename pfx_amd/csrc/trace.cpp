@@ -812,6 +812,12 @@ pfxh_status pfxh_layout_lights(const pfxh_light* lights, uint32_t count, float* 
     return PFXH_OK;
 }
 
+pfxh_status pfxh_layout_select(float* keys, uint32_t* ids, size_t len, size_t index, int mode, size_t* result) {
+    if ((len && (!keys || !ids)) || index >= len || mode < 0 || mode > 2 || !result) return PFXH_INVALID;
+    *result = select_keys(keys, ids, len, index, mode);
+    return PFXH_OK;
+}
+
 pfxh_status pfxh_layout_emitters(const pfxh_scene* scene, float* entries, float* cdf, uint32_t* emitter_count,
                                  char* err, size_t err_len) {
     if (!scene || !emitter_count) return PFXH_INVALID;

@@ -280,9 +280,35 @@ def test_few_lights_keep_their_order_and_their_own_slots():
     assert (looped, tree) == (0, 5)
 
 
+def scattered(count, seed, grid=None):
+    """Point and spot lights at random positions, as the terrain scene
+    scatters them; with `grid`, coordinates snap to a few values so many
+    lights tie on the split axis."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for k in range(count):
+        pos = rng.uniform(-50.0, 50.0, 3)
+        if grid:
+            pos = np.round(pos / (100.0 / grid)) * (100.0 / grid)
+        intensity = float(np.exp(rng.uniform(0.0, np.log(1000.0))))
+        if k % 2 == 0:
+            rows.append(point_light(tuple(pos), (1.0, 0.9, 0.8), intensity, 30.0))
+        else:
+            d = np.array([rng.uniform(-0.5, 0.5), -1.0, rng.uniform(-0.5, 0.5)])
+            outer = float(rng.uniform(15.0, 60.0))
+            rows.append(spot_light(tuple(pos), (1.0, 0.9, 0.8), intensity, 30.0, tuple(d / np.linalg.norm(d)),
+                                   0.8 * outer, outer))
+    return np.concatenate(rows)
+
+
 def test_product_light_tree_is_byte_identical():
-    for count in (3, 5, 128):
-        lights = street(count)
+    """The split order (Rust's select_nth_unstable_by, restated on both
+    sides) included: random positions exercise its partition rounds,
+    snapped ones its runs of equal keys."""
+    cases = [street(c) for c in (3, 5, 128)]
+    cases += [scattered(c, 30 + c) for c in (9, 17, 40, 128, 1000)]
+    cases += [scattered(c, 40 + c, grid=3) for c in (64, 128, 700)]
+    for lights in cases:
         a, b = product_lay(lights), lay(lights)
         assert a[1:] == b[1:] and a[0].tobytes() == b[0].tobytes()
 
