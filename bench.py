@@ -59,6 +59,8 @@ REF_NODE_BYTES, REF_PRIM_BYTES = 48, 96
 # per-tile fallback of the learned denoiser to the repo denoiser: on the 4K
 # terrain at 4 spp about 1% of 32x32 tiles remove more variance than this
 CHECK_THRESHOLD, CHECK_TILE = 1.0, 32
+# samples per pixel of the fixed passes that compare the two triangle tests
+FIXED_SPP = 8
 # HBM bytes per closest-hit ray measured with rocprofv3 counters (FETCH_SIZE
 # and WRITE_SIZE of k_extend, calibrated on a copy kernel); absent -> null
 TRAFFIC_FILE = ROOT / "profiles" / "k_extend_traffic.json"
@@ -214,6 +216,46 @@ def quality_leg(scene) -> dict:
     return out
 
 
+def fixed_pass(trace: Trace, camera, spp: int, seed: int) -> dict:
+    """One fixed-spp render of the whole frame: wall ms and Mrays/s."""
+    trace.set_camera(camera)
+    trace.reset_stats()
+    t = time.perf_counter()
+    trace.sample(spp, seed)
+    ms = (time.perf_counter() - t) * 1e3
+    s = trace.stats()
+    rays = s["closest_rays"] + s["shadow_rays"]
+    return {"ms": round(ms, 1), "mrays_per_s": round(rays / ms / 1e3, 1), "rays": rays,
+            "trace_ms": round(s["trace_ms"], 1)}
+
+
+def watertight_leg(scene, mt_probe, mt_pass) -> dict:
+    """The watertight ray-triangle test (Woop, Benthin and Wald 2013) beside
+    the reference's Moller-Trumbore test on the same 4K still: how many
+    primary rays hit a different triangle (or hit where the other misses),
+    and the cost of a fixed-spp pass in each mode."""
+    t = time.perf_counter()
+    with Trace.new_detailed(scene, Detail(bounces=BOUNCES, watertight=True), WIDTH, HEIGHT) as wt:
+        log(f"watertight trace built in {time.perf_counter() - t:.1f} s")
+        probe = wt.probe()
+        wt_pass = fixed_pass(wt, scene.camera, FIXED_SPP, 5)
+    a, b = mt_probe.primitive, probe.primitive
+    miss = np.uint32(0xFFFFFFFF)
+    out = {
+        "primary_rays": int(a.size),
+        "primitive_differs": int(np.sum(a != b)),
+        "differs_frac": round(float(np.mean(a != b)), 8),
+        "misses_moller_trumbore": int(np.sum(a == miss)),
+        "misses_watertight": int(np.sum(b == miss)),
+        "hit_only_with_watertight": int(np.sum((a == miss) & (b != miss))),
+        "spp": FIXED_SPP,
+        "moller_trumbore": mt_pass,
+        "watertight": wt_pass,
+    }
+    log(f"watertight: {out}")
+    return out
+
+
 def batch_leg() -> dict:
     """Uneven stills in one batch (pfx_amd.batch): 32 stills of seeded
     sphere-and-terrain scenes (10k-500k triangles, 1-64 lights), sizes drawn
@@ -269,6 +311,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-quality", action="store_true", help="skip the image-quality leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-stills leg")
+    ap.add_argument("--no-watertight", action="store_true", help="skip the watertight-test leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); across GPUs the still would be "
@@ -359,7 +402,11 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    mt_probe = trace.probe() if not args.no_watertight else None
+    mt_pass = fixed_pass(trace, scene.camera, FIXED_SPP, 5) if not args.no_watertight else None
     trace.close()
+    if not args.no_watertight:
+        result["watertight_test"] = watertight_leg(scene, mt_probe, mt_pass)
     if not args.no_quality:
         result["quality"] = quality_leg(scene)
     if not args.no_batch:

@@ -152,6 +152,16 @@ enum { PFXH_ACCUM_COMPENSATED = 0, PFXH_ACCUM_PLAIN = 1 };
 /* gpu::Projection (crates/trace/src/gpu.rs:181-189) */
 enum { PFXH_PROJ_PERSPECTIVE = 0, PFXH_PROJ_ORTHOGRAPHIC = 1, PFXH_PROJ_EQUIRECT = 2 };
 
+/* Ray-triangle test. PFXH_TRI_MOLLER_TRUMBORE is the reference's tri_hit /
+ * tri_blocks (crates/trace/src/shaders/trace.wgsl:780-804, 1076-1095) and
+ * the default, so hit ids match the reference. PFXH_TRI_WATERTIGHT is the
+ * watertight test of Woop, Benthin and Wald (JCGT 2(1), 2013) in FP32: a
+ * ray crossing a mesh through a shared edge or vertex hits one of the
+ * triangles that meet there (no cracks). Same acceptance otherwise:
+ * t > 1e-4 and closer than the best hit, one-sided culling by the sign of
+ * the determinant, barycentrics u (weight of b) and v (weight of c). */
+enum { PFXH_TRI_MOLLER_TRUMBORE = 0, PFXH_TRI_WATERTIGHT = 1 };
+
 /* The parts of detail::Detail (crates/trace/src/detail.rs:374-398) the
  * path uses. The sampler is PCG (`Sampler::Pcg`, sobol.rs:13-23). */
 typedef struct pfxh_detail {
@@ -166,6 +176,7 @@ typedef struct pfxh_detail {
     float ortho_size[2]; /* orthographic view width and height (PFXH_PROJ_ORTHOGRAPHIC) */
     uint32_t no_emitter_sampling; /* 1: emissive triangles are not light-sampled, as
                                    * Trace::build with emitter_sampling = false (gpu.rs:781) */
+    uint32_t triangle_test;       /* PFXH_TRI_*; 0 is the reference's test */
 } pfxh_detail;
 
 /* adaptive::Adaptive (crates/trace/src/adaptive.rs:9-26) */

@@ -53,6 +53,7 @@ def lib() -> C.CDLL:
             "or_tree_pick": (None, [_P, _P, _P, _F, _P]),
             "or_light_meta": (_U, [_P, C.POINTER(_U), C.POINTER(_U)]),
             "or_trace_counts": (None, [_P, _P, _P, _P]),
+            "or_cast": (None, [_P, _P, _P, _U, C.c_int, _F, _P, _P]),
             "or_pcg": (_U, [_U]),
             "or_offset_axis": (_F, [_F, _F]),
             "or_spawn_point": (None, [_P, _P, _P, _F, _P]),
@@ -152,6 +153,16 @@ class Oracle:
         looped, tree = _U(), _U()
         rows = lib().or_light_meta(self._h, C.byref(looped), C.byref(tree))
         return int(rows), int(looped.value), int(tree.value)
+
+    def cast(self, origins, directions, any_hit: bool = False, limit: float = 1e20):
+        """Closest-hit caller triangle ids (UINT32_MAX on a miss) and distances
+        of a ray list; with any_hit, 1 where blocked before `limit`."""
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
+        ids = np.zeros(len(o), np.uint32)
+        t = np.zeros(len(o), np.float32)
+        lib().or_cast(self._h, _ptr(o), _ptr(d), len(o), 1 if any_hit else 0, limit, _ptr(ids), _ptr(t))
+        return ids, t
 
     def trace_counts(self, o, d):
         o = np.asarray(o, np.float32)

@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <atomic>
+#include <utility>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -66,6 +67,7 @@ struct Scene {
     uint32_t width = 0, height = 0;
     pfxh_bounces bounces{16, 4, 8, 12};
     uint32_t accumulate = ACC_COMP;
+    bool watertight = false;  // PFXH_TRI_WATERTIGHT
     uint32_t samples = 0;
     // accumulation: 12 sums per pixel (color.xyzw, albedo, normal)
     std::vector<float> sum, comp;
@@ -115,11 +117,53 @@ static void tri_fill(const Scene& sc, uint32_t i, uint32_t record, float t, floa
     uint32_t info_y = sc.laid.surfaces[record].info[1];
     hit.material = info_y == NO_MATERIAL ? (uint32_t)tri.a[3] : info_y;
 }
+// Watertight ray/triangle test (Woop, Benthin and Wald, JCGT 2(1), 2013),
+// FP32 without the paper's FP64 recheck of an edge function equal to 0
+// (then counted inside). The ray frame: kz = axis of the largest |d|
+// component (first of equals), kx = kz + 1, ky = kx + 1 (mod 3), swapped
+// when d[kz] < 0; shear sx = d[kx] / d[kz], sy = d[ky] / d[kz], sz =
+// 1 / d[kz]. Each vertex minus the origin is sheared to (x, y) and the edge
+// functions U, V, W (the barycentric weights of a, b, c times det) must
+// share a sign. Candidate t = (U az + V bz + W cz) / det with az = sz (a -
+// o)[kz]; u = V / det (weight of b), v = W / det (weight of c), each as a
+// product with 1 / det. Returns false when the ray misses the triangle.
+static bool watertight_test(const Tri& tri, v3 o, v3 d, float& t, float& u, float& v, float& det) {
+    float dx = fabsf(d.x), dy = fabsf(d.y), dz = fabsf(d.z);
+    int kz = (dx >= dy && dx >= dz) ? 0 : (dy >= dz ? 1 : 2);
+    int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+    if (pfx::comp(d, kz) < 0.0f) std::swap(kx, ky);
+    float sx = pfx::comp(d, kx) / pfx::comp(d, kz);
+    float sy = pfx::comp(d, ky) / pfx::comp(d, kz);
+    float sz = 1.0f / pfx::comp(d, kz);
+    v3 A = t3(tri.a) - o, B = t3(tri.b) - o, C = t3(tri.c) - o;
+    float ax = pfx::comp(A, kx) - sx * pfx::comp(A, kz), ay = pfx::comp(A, ky) - sy * pfx::comp(A, kz);
+    float bx = pfx::comp(B, kx) - sx * pfx::comp(B, kz), by = pfx::comp(B, ky) - sy * pfx::comp(B, kz);
+    float cx = pfx::comp(C, kx) - sx * pfx::comp(C, kz), cy = pfx::comp(C, ky) - sy * pfx::comp(C, kz);
+    float U = cx * by - cy * bx;
+    float V = ax * cy - ay * cx;
+    float W = bx * ay - by * ax;
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    det = U + V + W;
+    if (det == 0.0f) return false;
+    float T = U * (sz * pfx::comp(A, kz)) + V * (sz * pfx::comp(B, kz)) + W * (sz * pfx::comp(C, kz));
+    float rcp = 1.0f / det;
+    t = T * rcp;
+    u = V * rcp;
+    v = W * rcp;
+    return true;
+}
 static void tri_hit(const Scene& sc, uint32_t i, uint32_t record, uint32_t word, float inv_det, v3 o, v3 d,
                     bool shadow_ray, Hit& hit) {
     const Tri& tri = sc.laid.tris[i];
     if ((word & 1u) != 0u && !shadow_ray) return;
     if ((word & 2u) != 0u && shadow_ray) return;
+    if (sc.watertight) {
+        float t, u, v, det;
+        if (!watertight_test(tri, o, d, t, u, v, det)) return;
+        if (!shadow_ray && (word & ONE_SIDED) != 0u && det * inv_det < 0.0f) return;
+        if (t > 0.0001f && t < hit.t) tri_fill(sc, i, record, t, u, v, hit);
+        return;
+    }
     v3 a = t3(tri.a);
     v3 e1 = t3(tri.b) - a, e2 = t3(tri.c) - a;
     v3 p = pfx::cross(d, e2);
@@ -219,6 +263,11 @@ static Hit scene_hit(const Scene& sc, v3 o, v3 d, float limit, bool shadow_ray, 
 static bool tri_blocks(const Scene& sc, uint32_t i, uint32_t word, float inv_det, v3 o, v3 d, float limit) {
     const Tri& tri = sc.laid.tris[i];
     if ((word & 2u) != 0u) return false;
+    if (sc.watertight) {
+        float t, u, v, det;
+        if (!watertight_test(tri, o, d, t, u, v, det)) return false;
+        return !(t <= 0.0001f || t >= limit);
+    }
     v3 a = t3(tri.a);
     v3 e1 = t3(tri.b) - a, e2 = t3(tri.c) - a;
     v3 p = pfx::cross(d, e2);
@@ -1158,6 +1207,7 @@ static std::string prepare(Scene& sc, const pfxh_scene& in, const pfxh_detail& d
     sc.window[1] = h;
     sc.bounces = detail.bounces;
     sc.accumulate = detail.accumulate;
+    sc.watertight = detail.triangle_test == PFXH_TRI_WATERTIGHT;
     sc.sum.assign((size_t)w * h * 12, 0.0f);
     sc.comp.assign((size_t)w * h * 12, 0.0f);
     sc.sum64.assign((size_t)w * h * 12, 0.0);
@@ -1282,6 +1332,25 @@ uint32_t or_light_meta(void* h, uint32_t* looped, uint32_t* tree) {
     return (uint32_t)sc->lights.rows.size();
 }
 // Instrumented traversal counts for one closest-hit query (algorithmic bytes)
+// Closest hits of n rays (origins and directions, 3 floats each) as
+// caller triangle ids (UINT32_MAX on a miss) and distances; with `any`,
+// occlusion before `limit` instead (id 1 blocked, 0 clear).
+void or_cast(void* h, const float* o, const float* d, uint32_t n, int any, float limit, uint32_t* ids, float* t) {
+    Scene* sc = (Scene*)h;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        v3 ro = t3(o + 3 * i), rd = t3(d + 3 * i);
+        if (any) {
+            ids[i] = occluded(*sc, ro, rd, limit) ? 1u : 0u;
+            t[i] = 0.0f;
+            continue;
+        }
+        Hit hit = scene_hit(*sc, ro, rd, limit, false);
+        bool missed = hit.t >= limit;
+        ids[i] = missed ? 0xffffffffu : sc->laid.slot_prim[hit.surface];
+        t[i] = missed ? 0.0f : hit.t;
+    }
+}
 void or_trace_counts(void* h, const float* o, const float* d, uint64_t* out) {
     Counter c;
     scene_hit(*(Scene*)h, t3(o), t3(d), 1e20f, false, &c);

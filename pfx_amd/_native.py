@@ -130,6 +130,7 @@ class DetailC(C.Structure):
         ("projection", C.c_uint32),
         ("ortho_size", C.c_float * 2),
         ("no_emitter_sampling", C.c_uint32),
+        ("triangle_test", C.c_uint32),
     ]
 
 

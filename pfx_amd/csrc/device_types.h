@@ -34,7 +34,10 @@ struct alignas(16) Node4 {
 static_assert(sizeof(Node4) == 128, "Node4 is 128 bytes");
 
 // Positions in edge form: a, e1 = b - a, e2 = c - a (the same subtractions
-// the Moller-Trumbore test makes, done once on the host).
+// the Moller-Trumbore test makes, done once on the host). A scene built for
+// the watertight test keeps the vertices themselves instead (e1 holds b, e2
+// holds c): that test needs a vertex shared by two triangles to be the same
+// bits in both.
 struct alignas(16) TriPos {
     float a[3];
     float e1[3];

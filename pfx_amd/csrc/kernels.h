@@ -8,6 +8,11 @@
 namespace pfxh {
 
 // Scene constants and read-only device buffers (passed by value).
+// SceneDev::geo_flags: GEO_TAME when every box bound is 0 or has magnitude
+// >= 2^-72 (the fast exact slab applies); GEO_WATERTIGHT when the triangles
+// hold vertices and are tested watertight.
+constexpr uint32_t GEO_TAME = 1u, GEO_WATERTIGHT = 2u;
+
 struct SceneDev {
     const Node4* nodes;
     const TriPos* tris;
@@ -27,7 +32,7 @@ struct SceneDev {
     uint32_t env_w, env_h, env_black, lit;
     uint32_t looped, tree, emitter_count, light_slots;
     uint32_t entries;  // NEE entries a vertex may queue
-    uint32_t bounds_tame;  // every box bound is 0 or has magnitude >= 2^-72
+    uint32_t geo_flags;    // GEO_TAME | GEO_WATERTIGHT
     uint32_t total, diffuse, glossy, transmission;
     float sun_dir[3], sun_color[3], sun_intensity, sun_w;
     float origin[3], forward[3], right[3], up[3], lens[4];

@@ -265,6 +265,33 @@ DEV bool visit4(const G& S, uint32_t index, const Ray& R, float t, Stack& st, ui
 // equal-distance ties resolve to the same triangle. Keeping the walk in a
 // state lets a wave retire a finished ray and start a new one in its lane
 // while the other lanes keep walking.
+// The watertight test's per-ray frame: kz is the axis of the largest
+// direction component (the first of equals), kx and ky follow it cyclically
+// and swap when that component is negative; the shear takes the direction
+// to (0, 0, 1) in (kx, ky, kz).
+struct Shear {
+    uint32_t kx, ky, kz;
+    float sx, sy, sz;
+};
+DEV float pick(v3 v, uint32_t k) { return k == 0u ? v.x : (k == 1u ? v.y : v.z); }
+DEV Shear shear_of(v3 d) {
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    Shear s;
+    s.kz = (ax >= ay && ax >= az) ? 0u : (ay >= az ? 1u : 2u);
+    s.kx = s.kz == 2u ? 0u : s.kz + 1u;
+    s.ky = s.kx == 2u ? 0u : s.kx + 1u;
+    float dz = pick(d, s.kz);
+    if (dz < 0.0f) {
+        uint32_t k = s.kx;
+        s.kx = s.ky;
+        s.ky = k;
+    }
+    s.sx = pick(d, s.kx) / dz;
+    s.sy = pick(d, s.ky) / dz;
+    s.sz = 1.0f / dz;
+    return s;
+}
+
 template <bool ANY>
 struct Walk {
     Ray world, R;
@@ -283,7 +310,7 @@ struct Walk {
         h.v = 0.0f;
         h.slot = 0xffffffffu;
         h.rec = 0u;
-        world = make_ray(o, d, S.bounds_tame != 0u);
+        world = make_ray(o, d, (S.geo_flags & GEO_TAME) != 0u);
         R = world;
         base = 0;
         rec = 0;
@@ -327,6 +354,7 @@ struct Walk {
         if (skip) return false;
         uint32_t first = base + (cur & REF_FIRST_MASK);
         uint32_t count = ((cur >> 27) & 15u) + 1u;
+        if (S.geo_flags & GEO_WATERTIGHT) return leaf_watertight<COUNT>(S, first, count, prims_ctr);
         for (uint32_t j = 0; j < count; ++j) {
             uint32_t i = first + j;
             if (COUNT) prims_ctr++;
@@ -360,6 +388,53 @@ struct Walk {
         }
         return false;
     }
+    // The same leaf with the watertight test (Woop, Benthin and Wald, "Watertight
+    // Ray/Triangle Intersection", JCGT 2(1), 2013): vertices are moved to the
+    // ray origin, sheared so the ray runs along +z of a permuted frame (kz the
+    // largest direction component, kx/ky swapped when it is negative to keep
+    // the winding), and the three 2D edge functions decide the hit. A vertex
+    // or edge shared by two triangles gives bit-identical edge functions of
+    // opposite sign in both, so a ray can not pass between them. FP32 only:
+    // an edge function that rounds to 0 counts as inside (the paper's FP64
+    // recheck of that case is not made; a ray can then hit both triangles at
+    // the same t, and the first in order is kept, as for any tie).
+    template <bool COUNT, class G>
+    DEV bool leaf_watertight(const G& S, uint32_t first, uint32_t count, uint32_t& prims_ctr) {
+        const Shear sh = shear_of(R.d);
+        for (uint32_t j = 0; j < count; ++j) {
+            uint32_t i = first + j;
+            if (COUNT) prims_ctr++;
+            const float4* tp = reinterpret_cast<const float4*>(S.tris + i);
+            float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+            v3 A = mk(t0.x, t0.y, t0.z) - R.o, B = mk(t0.w, t1.x, t1.y) - R.o, C = mk(t1.z, t1.w, t2.x) - R.o;
+            float az = pick(A, sh.kz), bz = pick(B, sh.kz), cz = pick(C, sh.kz);
+            float ax = pick(A, sh.kx) - sh.sx * az, ay = pick(A, sh.ky) - sh.sy * az;
+            float bx = pick(B, sh.kx) - sh.sx * bz, by = pick(B, sh.ky) - sh.sy * bz;
+            float cx = pick(C, sh.kx) - sh.sx * cz, cy = pick(C, sh.ky) - sh.sy * cz;
+            float U = cx * by - cy * bx;
+            float V = ax * cy - ay * cx;
+            float W = bx * ay - by * ax;
+            if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) continue;
+            float det = U + V + W;
+            if (det == 0.0f) continue;
+            if (!ANY && (word & ONE_SIDED) != 0u && det * inv_det < 0.0f) continue;
+            float T = U * (sh.sz * az) + V * (sh.sz * bz) + W * (sh.sz * cz);
+            float rcp = 1.0f / det;
+            float t = T * rcp;
+            if (ANY) {
+                if (t <= 0.0001f || t >= limit) continue;
+                return true;
+            }
+            if (t > 0.0001f && t < h.t) {
+                h.t = t;
+                h.u = V * rcp;
+                h.v = W * rcp;
+                h.slot = i;
+                h.rec = rec;
+            }
+        }
+        return false;
+    }
     // Enter the current instance: move the ray into its object space and
     // continue at its bottom-level root; false when its box is missed.
     template <class G>
@@ -370,7 +445,7 @@ struct Walk {
         word = g.word;
         inv_det = g.inv_det;
         if (g.has_xform)
-            R = make_ray(to_object(g.inv, world.o, 1.0f), to_object(g.inv, world.d, 0.0f), S.bounds_tame != 0u);
+            R = make_ray(to_object(g.inv, world.o, 1.0f), to_object(g.inv, world.d, 0.0f), (S.geo_flags & GEO_TAME) != 0u);
         inst_top = st.top;
         float nr;
         if (slab(g.lo[0], g.lo[1], g.lo[2], g.hi[0], g.hi[1], g.hi[2], R, h.t, nr)) {
@@ -419,9 +494,9 @@ struct Geo {
     const TriPos* tris;
     const Group* groups;
     RootEntry root;
-    uint32_t bounds_tame;
+    uint32_t geo_flags;
 };
-DEV Geo geo_of(const SceneDev& S) { return Geo{S.nodes, S.tris, S.groups, S.root, S.bounds_tame}; }
+DEV Geo geo_of(const SceneDev& S) { return Geo{S.nodes, S.tris, S.groups, S.root, S.geo_flags}; }
 template <bool BATCH>
 DEV decltype(auto) walk_scene(const SceneDev& S, const Geo& g) {
     if constexpr (BATCH)
@@ -484,6 +559,11 @@ DEV Surf surface_at(const SceneDev& S, uint32_t slot, uint32_t rec, float u, flo
     const float4* tp = reinterpret_cast<const float4*>(S.tris + slot);
     float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
     v3 e1 = mk(t0.w, t1.x, t1.y), e2 = mk(t1.z, t1.w, t2.x);
+    if (S.geo_flags & GEO_WATERTIGHT) {  // vertex form: the host's edge subtractions, here
+        v3 a = mk(t0.x, t0.y, t0.z);
+        e1 = e1 - a;
+        e2 = e2 - a;
+    }
     const float4* ap = reinterpret_cast<const float4*>(S.attrs + slot);
     float4 a0 = ap[0], a1 = ap[1], a2 = ap[2];
     Surf s;

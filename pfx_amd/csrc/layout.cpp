@@ -1220,7 +1220,7 @@ void build_emitters(const std::vector<float>& tris12, const std::vector<uint32_t
     if (n) cdf[n - 1] = 1.0f;
 }
 
-std::string build_scene(const pfxh_scene& sc, HostScene& out, bool emitter_sampling) {
+std::string build_scene(const pfxh_scene& sc, HostScene& out, bool emitter_sampling, bool watertight) {
     if (sc.material_count == 0) return "invalid trace scene detail";
     for (uint32_t i = 0; i < sc.light_count; ++i)
         if (!light_valid(sc.lights[i])) return "invalid trace light";
@@ -1261,8 +1261,8 @@ std::string build_scene(const pfxh_scene& sc, HostScene& out, bool emitter_sampl
         memset(&a, 0, sizeof a);
         for (int k = 0; k < 3; ++k) {
             p.a[k] = t.v[k];
-            p.e1[k] = t.v[3 + k] - t.v[k];
-            p.e2[k] = t.v[6 + k] - t.v[k];
+            p.e1[k] = watertight ? t.v[3 + k] : t.v[3 + k] - t.v[k];
+            p.e2[k] = watertight ? t.v[6 + k] : t.v[6 + k] - t.v[k];
             a.na[k] = t.n[k];
             a.nb[k] = t.n[3 + k];
             a.nc[k] = t.n[6 + k];
@@ -1446,6 +1446,7 @@ std::string build_scene(const pfxh_scene& sc, HostScene& out, bool emitter_sampl
         }
     }
     if (sc.instance_count == 0) out.max_stack = bottom_stack;
+    out.watertight = watertight;
     out.bounds_tame = true;
     auto tame = [](float v) { return v == 0.0f || std::isinf(v) || (std::fabs(v) >= 0x1p-72f && std::fabs(v) <= 0x1p60f); };
     for (const Node4& n : out.nodes)

@@ -52,11 +52,13 @@ struct HostScene {
     uint32_t max_depth = 0;   // binary-tree depth (top + bottom)
     uint32_t max_stack = 0;   // worst-case traversal stack entries
     bool bounds_tame = true;  // every node box bound is 0 or >= 2^-72 in magnitude
+    bool watertight = false;  // tris hold vertices a, b, c (watertight test), not a, b - a, c - a
 };
 
 // `emitter_sampling` false leaves emissive triangles unmarked, so they are
 // reached only by BSDF sampling (gpu.rs:781).
-std::string build_scene(const pfxh_scene& scene, HostScene& out, bool emitter_sampling = true);
+std::string build_scene(const pfxh_scene& scene, HostScene& out, bool emitter_sampling = true,
+                        bool watertight = false);
 std::string lay_levels(const pfxh_scene& scene, RefLevels& out, std::vector<uint32_t>* slot_prim);
 
 // tree::lay (crates/trace/src/tree.rs:220-243)

@@ -73,7 +73,8 @@ void place_projection(SceneDev& S, uint32_t kind, float w, float h) {
 
 void trace_setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& detail, uint32_t width, uint32_t height,
                  bool with_wave) {
-    std::string e = build_scene(scene, t->host, detail.no_emitter_sampling == 0u);
+    std::string e = build_scene(scene, t->host, detail.no_emitter_sampling == 0u,
+                                detail.triangle_test == PFXH_TRI_WATERTIGHT);
     if (!e.empty()) throw DeviceError{PFXH_SETUP, e};
     const HostScene& h = t->host;
     HIP_TRY(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
@@ -95,7 +96,7 @@ void trace_setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& deta
     S.slot_prim = t->upload(h.slot_prim, "primitive map");
     S.record_instance = t->upload(h.record_instance, "instance map");
     S.root = h.root;
-    S.bounds_tame = h.bounds_tame ? 1u : 0u;
+    S.geo_flags = (h.bounds_tame ? GEO_TAME : 0u) | (h.watertight ? GEO_WATERTIGHT : 0u);
     S.env_w = h.env_w;
     S.env_h = h.env_h;
     S.env_black = h.env_black ? 1u : 0u;
@@ -296,7 +297,8 @@ pfxh_status trace_check(const pfxh_scene* scene, const pfxh_detail* detail, uint
         put_err(err, err_len, "the frame is too large");
         return PFXH_REFUSED;
     }
-    if (!projection_valid(detail->projection, detail->ortho_size[0], detail->ortho_size[1])) {
+    if (!projection_valid(detail->projection, detail->ortho_size[0], detail->ortho_size[1]) ||
+        detail->triangle_test > PFXH_TRI_WATERTIGHT) {
         put_err(err, err_len, "invalid trace detail");
         return PFXH_SETUP;
     }

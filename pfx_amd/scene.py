@@ -99,6 +99,7 @@ class Detail:
     projection: int = PERSPECTIVE  # gpu::Projection
     ortho_size: tuple = (0.0, 0.0)  # orthographic view width, height
     emitter_sampling: bool = True  # False: emissive triangles only through BSDF sampling (gpu.rs:781)
+    watertight: bool = False  # True: the watertight ray-triangle test instead of the reference's
 
     def to_c(self) -> N.DetailC:
         d = N.DetailC()
@@ -112,6 +113,7 @@ class Detail:
         d.projection = int(self.projection)
         d.ortho_size[:] = [float(v) for v in self.ortho_size]
         d.no_emitter_sampling = 0 if self.emitter_sampling else 1
+        d.triangle_test = 1 if self.watertight else 0
         return d
 
 
