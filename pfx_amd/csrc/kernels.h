@@ -21,6 +21,7 @@ struct SceneDev {
     const Surface* surfaces;
     const MatDev* materials;
     const float4* lights;    // GpuLight rows, 4 float4 each
+    const float4* light_cones;  // per light row: (cos θo, sin θo, cos(θo+θe), cos(θe−θo)), (θo, θe, 0, 0)
     const float4* emitters;  // GpuEmitter rows, 4 float4 each
     const float* cdf;
     const float* sky_rows;

@@ -571,8 +571,11 @@ uint32_t triangle_leaf(const pfxh_bvh_node& n, const void*) {
     if (n.count == 0 || n.count > 16 || n.first > REF_FIRST_MASK) return REF_NONE;
     return REF_LEAF | ((n.count - 1u) << 27) | n.first;
 }
-uint32_t group_leaf(const pfxh_bvh_node& n, const void* order) {
-    return REF_INST | static_cast<const uint32_t*>(order)[n.first];
+// top-level leaves hold one group each; groups are stored in the top tree's
+// depth-first leaf order, so a leaf's position there is its group index
+uint32_t group_leaf(const pfxh_bvh_node& n, const void*) {
+    if (n.count != 1 || n.first > REF_INDEX_MASK) return REF_NONE;
+    return REF_INST | n.first;
 }
 
 float to_radians(float deg) { return deg * (PI_F / 180.0f); }
@@ -1130,6 +1133,30 @@ bool light_valid(const pfxh_light& l) {  // lights.rs:101-142
            0.0f <= l.inner_deg && l.inner_deg <= l.outer_deg && l.outer_deg > 0.0f && l.outer_deg <= 90.0f;
 }
 
+void light_cones(const std::vector<float>& rows, uint32_t lights, std::vector<float>& out) {
+    size_t count = rows.size() / 16;
+    out.assign(count * 8, 0.0f);
+    for (size_t r = 0; r < count; ++r) {
+        const float* g = &rows[r * 16];
+        float theta_o, theta_e;
+        if (r < lights) {
+            theta_o = 0.0f;
+            theta_e = (uint32_t)g[3] == 1u ? pfx::fm_acos(pfx::fm_clamp(-g[13] / g[12], -1.0f, 1.0f))
+                                           : 3.141592653589793f / 2.0f;
+        } else {
+            theta_o = g[11];
+            theta_e = g[12];
+        }
+        float* c = &out[r * 8];
+        c[0] = pfx::fm_cos(theta_o);
+        c[1] = pfx::fm_sin(theta_o);
+        c[2] = pfx::fm_cos(theta_o + theta_e);
+        c[3] = pfx::fm_cos(theta_e - theta_o);
+        c[4] = theta_o;
+        c[5] = theta_e;
+    }
+}
+
 void lay_lights(const pfxh_light* lights, size_t n, std::vector<float>& rows, uint32_t* looped, uint32_t* tree) {
     auto treed = [](const pfxh_light& l) {
         return lit_light(l) && (l.kind == PFXH_LIGHT_POINT || l.kind == PFXH_LIGHT_SPOT);
@@ -1296,6 +1323,7 @@ std::string build_scene(const pfxh_scene& sc, HostScene& out, bool emitter_sampl
         }
     }
     lay_lights(sc.lights, sc.light_count, out.lights, &out.looped, &out.tree);
+    light_cones(out.lights, out.looped + out.tree, out.light_cones);
     build_emitters(cand, cand_rec, out.looped, out.tree, out.emitters, out.emitter_cdf);
     out.emitter_count = (uint32_t)cand_rec.size();
     out.lit = out.emitter_count > 0 || sc.light_count > 0;
@@ -1430,10 +1458,15 @@ std::string build_scene(const pfxh_scene& sc, HostScene& out, bool emitter_sampl
         }
         if (!out.groups.empty()) {
             RefBvh top = build_over(boxes, 1, 0);
-            // top leaves point at groups through the instance order
+            // groups renumbered in the top tree's depth-first leaf order: the
+            // traversal breaks ties at equal distance by group index, in the
+            // order the reference's left-first walk meets the groups
+            std::vector<Group> dfs(out.groups.size());
+            for (size_t k = 0; k < top.order.size(); ++k) dfs[k] = out.groups[top.order[k]];
+            out.groups.swap(dfs);
             Collapse top_conv(out.nodes);
             uint32_t top_stack = 0;
-            uint32_t root = top_conv.tree(top.nodes, group_leaf, top.order.data(), err, &top_stack);
+            uint32_t root = top_conv.tree(top.nodes, group_leaf, nullptr, err, &top_stack);
             if (!err.empty()) return err;
             top_depth = top_conv.depth;
             out.max_stack = top_stack + bottom_stack;

@@ -39,6 +39,7 @@ struct HostScene {
     std::vector<uint32_t> record_instance;  // surface record -> caller instance + 1
     std::vector<MatDev> materials;
     std::vector<float> lights;         // GpuLight rows, 16 floats each
+    std::vector<float> light_cones;    // per light row: cos/sin of its cone angles, 8 floats (light_cones)
     std::vector<float> emitters;       // GpuEmitter rows, 16 floats each (header first)
     std::vector<float> emitter_cdf;
     std::vector<float> sky_rows, sky_columns;
@@ -67,6 +68,14 @@ std::string lay_levels(const pfxh_scene& scene, RefLevels& out, std::vector<uint
 // runs one partition around keys[index] and returns the number of smaller
 // keys, mode 2 runs the median-of-medians fallback alone.
 size_t select_keys(float* keys, uint32_t* ids, size_t len, size_t index, int mode);
+// Per light row, the trigonometric terms the light tree's importance takes
+// of the row's constant cone angles (lights.wgsl:204-237 tree_importance):
+// cos θo, sin θo, cos(θo + θe), cos(θe − θo), θo, θe, 0, 0, with θo = c.w
+// and θe = d.x for a tree node, and θo = 0 with θe = acos(−d.y / d.x) for a
+// spot light or π/2 for a point light (lights.wgsl:255-271), each computed
+// by the same functions the device would use, so the device reads them
+// instead of recomputing them per shading point, bit for bit.
+void light_cones(const std::vector<float>& rows, uint32_t lights, std::vector<float>& out);
 void lay_lights(const pfxh_light* lights, size_t count, std::vector<float>& rows, uint32_t* looped,
                 uint32_t* tree);
 bool light_valid(const pfxh_light& l);

@@ -88,6 +88,7 @@ void trace_setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& deta
     S.surfaces = t->upload(h.surfaces, "instance surfaces");
     S.materials = t->upload(h.materials, "materials");
     S.lights = (const float4*)t->upload(h.lights, "lights");
+    S.light_cones = (const float4*)t->upload(h.light_cones, "light cones");
     S.emitters = (const float4*)t->upload(h.emitters, "emitters");
     S.cdf = t->upload(h.emitter_cdf, "emitter cdf");
     S.sky_rows = t->upload(h.sky_rows, "sky rows");

@@ -136,6 +136,58 @@ def test_adaptive_rounds_match_oracle():
 PROJECTIONS = {"orthographic": (ORTHOGRAPHIC, (0.9, 0.9)), "equirect": (EQUIRECT, (0.0, 0.0))}
 
 
+def tied_scene(w, h):
+    """Every hit is an exact tie: each world triangle appears twice (with
+    different materials) and one mesh is placed twice with the same
+    transform, so equal distances must resolve to the triangle and the
+    instance the reference's left-first walk meets first."""
+    from pfx_amd import _native as N
+    from pfx_amd.scene import Camera, Scene, material
+
+    rng = np.random.default_rng(4)
+    balls = []
+    for _ in range(12):
+        c = rng.uniform(-2.0, 2.0, 3)
+        balls.append(scenes.icosphere(1) * rng.uniform(0.3, 0.8) + c)
+    world = np.concatenate(balls).astype(np.float32)
+    verts = np.concatenate([world, world])
+    mats = np.concatenate([np.zeros(len(world)), np.ones(len(world))])
+    scene = Scene(triangles=scenes._triangles(verts, mats),
+                  materials=np.concatenate([material((0.5, 0.5, 0.5)), material((0.2, 0.6, 0.2)),
+                                            material((0.6, 0.2, 0.2))]),
+                  camera=Camera.look((0.0, 0.5, -7.0), (0.0, 0.0, 0.0), 60.0, w / h))
+    unit = scenes.icosphere(2).astype(np.float32)
+    rows = np.zeros(len(unit), N.TRIANGLE_DETAIL)
+    rows["vertices"] = unit.reshape(len(unit), 9)
+    rows["material"] = 2
+    scene.mesh_triangles = rows
+    scene.meshes = np.array([(0, len(unit))], N.MESH)
+    inst = np.zeros(4, N.INSTANCE)
+    for k, (pos, yaw) in enumerate([((0.5, -1.0, 1.0), 0.3), ((0.5, -1.0, 1.0), 0.3),
+                                    ((-1.5, 1.0, 0.0), 1.1), ((-1.5, 1.0, 0.0), 1.1)]):
+        inst[k]["mesh"] = 0
+        inst[k]["transform"] = scenes.transform(pos, yaw, 0.9)
+    scene.instances = inst
+    return scene
+
+
+def test_equal_distance_ties_resolve_like_the_reference():
+    w, h = 160, 120
+    scene = tied_scene(w, h)
+    for watertight in (False, True):
+        detail = Detail(bounces=Bounces.deep(2), watertight=watertight)
+        print(f"ties watertight={watertight}", flush=True)
+        ref = Oracle(scene, detail, w, h).probe()
+        with Trace.new_detailed(scene, detail, w, h) as t:
+            got = t.probe()
+        hit = ref["primitive"] != 0xFFFFFFFF
+        assert hit.mean() > 0.1
+        assert np.array_equal(got.primitive, ref["primitive"])
+        assert np.array_equal(got.instance, ref["instance"])
+        assert np.array_equal(got.id, ref["material_id"])
+        np.testing.assert_array_equal(got.distance, ref["distance"])
+
+
 @pytest.mark.parametrize("name", sorted(PROJECTIONS))
 def test_projections_match_oracle(name):
     """Orthographic and equirect cameras (trace.wgsl:2546-2553, probe.wgsl:18-34):
