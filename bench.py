@@ -40,6 +40,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 from pfx_amd import Adaptive, Bounces, Detail, Trace, scenes  # noqa: E402
+from pfx_amd.trace import hbm_probe  # noqa: E402
 from pfx_amd import denoise as denoise_mod  # noqa: E402
 from pfx_amd import learn as learn_mod  # noqa: E402
 from pfx_amd.scene import ACCUM_FP64  # noqa: E402
@@ -357,6 +358,9 @@ def main() -> int:
     # per box tested and a 96-byte triangle record per triangle tested
     ref_bytes_per_ray = per_ray["boxes_per_ray"] * REF_NODE_BYTES + per_ray["prims_per_ray"] * REF_PRIM_BYTES \
         + RAY_IO_BYTES
+    # achievable HBM bandwidth on this GPU: the library's streaming copy of 1 GiB
+    achievable = hbm_probe(1 << 30, 5)
+    log(f"streaming copy: {achievable:.0f} GB/s (read + write)")
     traffic = None
     if TRAFFIC_FILE.exists():
         measured = json.loads(TRAFFIC_FILE.read_text())
@@ -392,6 +396,10 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "achievable": round(achievable, 1),
+            "frac_of_achievable": round(achieved / achievable, 4),
+            "achievable_probe": "best of 5 device-to-device streaming copies of 1 GiB (pfxh_hbm_probe), "
+                                "read + write bytes over time",
             "traffic": traffic,
             "bytes_per_ray": round(bytes_per_ray, 1),
             "reference_layout_bytes_per_ray": round(ref_bytes_per_ray, 1),

@@ -554,6 +554,14 @@ pfxh_status pfxh_trace_learned_denoise_checked(pfxh_trace* trace, pfxh_learner* 
                                                size_t err_len);
 void pfxh_learner_free(pfxh_learner* learner);
 
+/* Achievable HBM bandwidth: the best of `reps` timed device-to-device
+ * streaming copies of `bytes` (a multiple of 16, at least 1 MiB) after one
+ * warm-up copy; *gbs = (bytes read + bytes written) / time in GB/s. The
+ * denominator the traversal's algorithmic rate is set against, beside the
+ * published peak. No counterpart in the reference (it publishes no HBM
+ * figures). */
+pfxh_status pfxh_hbm_probe(uint64_t bytes, uint32_t reps, double* gbs, char* err, size_t err_len);
+
 /* Library version string. */
 const char* pfxh_version(void);
 

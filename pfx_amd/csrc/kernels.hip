@@ -1846,7 +1846,19 @@ __global__ void k_zero(uint32_t* p, uint32_t n) {
 }
 
 // ---- launchers ---------------------------------------------------------------------
+// Streaming copy for the achievable-HBM probe: each thread moves float4s
+// with a grid stride, so every wave reads and writes whole 1 KiB runs.
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                    uint64_t n) {
+    uint64_t stride = (uint64_t)gridDim.x * 256u;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+
+void launch_stream_copy(hipStream_t s, const float4* src, float4* dst, uint64_t n, uint32_t blocks) {
+    hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, s, src, dst, n);
+}
 
 void launch_camera(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* active, uint32_t npix,
                    uint32_t nsamples, uint32_t acc, uint32_t seed) {

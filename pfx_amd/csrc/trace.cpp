@@ -714,6 +714,48 @@ pfxh_status pfxh_trace_atrous(pfxh_trace* t, float* out, char* err, size_t err_l
 uint32_t pfxh_trace_samples(const pfxh_trace* t) { return t ? t->samples : 0; }
 uint64_t pfxh_trace_gpu_bytes(const pfxh_trace* t) { return t ? t->gpu_bytes : 0; }
 
+pfxh_status pfxh_hbm_probe(uint64_t bytes, uint32_t reps, double* gbs, char* err, size_t err_len) {
+    if (!gbs || reps == 0 || bytes < (1u << 20) || bytes % 16 != 0) return PFXH_INVALID;
+    *gbs = 0.0;
+    void* src = nullptr;
+    void* dst = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    pfxh_status st = guarded(err, err_len, [&] {
+        HIP_TRY(hipMalloc(&src, bytes));
+        HIP_TRY(hipMalloc(&dst, bytes));
+        HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreate(&ev[0]));
+        HIP_TRY(hipEventCreate(&ev[1]));
+        HIP_TRY(hipMemsetAsync(src, 0, bytes, s));
+        int device = 0, cus = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&device) == hipSuccess && hipGetDeviceProperties(&prop, device) == hipSuccess &&
+            prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+        uint64_t n = bytes / 16;
+        uint32_t blocks = (uint32_t)cus * 8u;
+        float best = 0.0f;
+        for (uint32_t r = 0; r <= reps; ++r) {  // the first copy warms up
+            HIP_TRY(hipEventRecord(ev[0], s));
+            launch_stream_copy(s, (const float4*)src, (float4*)dst, n, blocks);
+            HIP_TRY(hipEventRecord(ev[1], s));
+            HIP_TRY(hipEventSynchronize(ev[1]));
+            HIP_TRY(hipGetLastError());
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            if (r > 0 && (best == 0.0f || ms < best)) best = ms;
+        }
+        *gbs = 2.0 * (double)bytes / (best * 1e-3) / 1e9;
+    });
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (s) (void)hipStreamDestroy(s);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    return st;
+}
+
 pfxh_status pfxh_trace_probe(pfxh_trace* t, pfxh_probe_hit* out, char* err, size_t err_len) {
     if (!t || !out) return PFXH_INVALID;
     return guarded(err, err_len, [&] {

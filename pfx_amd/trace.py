@@ -276,3 +276,14 @@ class Trace:
             self.close()
         except Exception:
             pass
+
+
+def hbm_probe(nbytes: int = 1 << 30, reps: int = 5) -> float:
+    """Achievable HBM bandwidth in GB/s: the best of `reps` device-to-device
+    streaming copies of `nbytes` (read + write bytes over the copy time)."""
+    err = N.err_buffer()
+    gbs = C.c_double()
+    st = N.lib().pfxh_hbm_probe(nbytes, reps, C.byref(gbs), err, len(err))
+    if st != 0:
+        raise TraceError(err.value.decode(errors="replace"))
+    return float(gbs.value)

@@ -121,3 +121,7 @@ def test_host_side_argument_errors(native_lib):
                                                   len(err)) == N.INVALID
     info = N.LearnLayerC()
     assert lib.pfxh_learn_layer(N.LEARN_LAYERS, C.byref(info)) == N.INVALID
+    gbs = C.c_double()
+    assert lib.pfxh_hbm_probe(1 << 10, 3, C.byref(gbs), err, len(err)) == N.INVALID  # under 1 MiB
+    assert lib.pfxh_hbm_probe((1 << 20) + 4, 3, C.byref(gbs), err, len(err)) == N.INVALID  # not whole float4s
+    assert lib.pfxh_hbm_probe(1 << 20, 0, C.byref(gbs), err, len(err)) == N.INVALID

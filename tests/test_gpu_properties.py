@@ -63,3 +63,15 @@ def test_split_sampling_and_reruns_are_identical(terrain):
         again = t.readback().color
     assert np.array_equal(split, whole)
     assert np.array_equal(whole, again)
+
+
+def test_hbm_probe_reads_a_plausible_bandwidth():
+    """The achievable-bandwidth probe bench.py divides by: a 1 GiB streaming
+    copy on an MI355X (8 TB/s published peak) lands well inside (1, 8) TB/s,
+    and a second call agrees within 15%."""
+    from pfx_amd.trace import hbm_probe
+
+    print("hbm probe", flush=True)
+    a, b = hbm_probe(1 << 30, 3), hbm_probe(1 << 30, 3)
+    print(f"  {a:.0f} and {b:.0f} GB/s")
+    assert 1000.0 < a < 8000.0 and abs(a - b) < 0.15 * a
