@@ -398,8 +398,8 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "achievable": round(achievable, 1),
             "frac_of_achievable": round(achieved / achievable, 4),
-            "achievable_probe": "best of 5 device-to-device streaming copies of 1 GiB (pfxh_hbm_probe), "
-                                "read + write bytes over time",
+            "achievable_probe": "best of 5 device-to-device streaming copies of 1 GiB at each of 2, 4 and 8 "
+                                "blocks per CU (pfxh_hbm_probe, nontemporal float4), read + write bytes over time",
             "traffic": traffic,
             "bytes_per_ray": round(bytes_per_ray, 1),
             "reference_layout_bytes_per_ray": round(ref_bytes_per_ray, 1),

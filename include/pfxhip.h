@@ -555,8 +555,9 @@ pfxh_status pfxh_trace_learned_denoise_checked(pfxh_trace* trace, pfxh_learner* 
 void pfxh_learner_free(pfxh_learner* learner);
 
 /* Achievable HBM bandwidth: the best of `reps` timed device-to-device
- * streaming copies of `bytes` (a multiple of 16, at least 1 MiB) after one
- * warm-up copy; *gbs = (bytes read + bytes written) / time in GB/s. The
+ * streaming copies of `bytes` (a multiple of 16, at least 1 MiB; nontemporal
+ * float4 loads and stores, 2, 4 and 8 blocks of 256 threads per CU, one
+ * warm-up copy each); *gbs = (bytes read + bytes written) / time in GB/s. The
  * denominator the traversal's algorithmic rate is set against, beside the
  * published peak. No counterpart in the reference (it publishes no HBM
  * figures). */

@@ -127,7 +127,7 @@ struct BatchDev {
 };
 
 constexpr int TRACE_BLOCK = 128;
-constexpr int STACK_SHORT = 8;
+constexpr int STACK_SHORT = 12;  // LDS head per lane: 12 KiB per block, 12 blocks per CU fit 160 KiB
 constexpr int STACK_DEPTH = STACK_LIMIT;
 
 void launch_camera(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* active, uint32_t npix,

@@ -1847,11 +1847,22 @@ __global__ void k_zero(uint32_t* p, uint32_t n) {
 
 // ---- launchers ---------------------------------------------------------------------
 // Streaming copy for the achievable-HBM probe: each thread moves float4s
-// with a grid stride, so every wave reads and writes whole 1 KiB runs.
+// with a grid stride, so every wave reads and writes whole 1 KiB runs;
+// nontemporal loads and stores keep the stream from churning the caches.
 __global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ src, float4* __restrict__ dst,
                                                     uint64_t n) {
     uint64_t stride = (uint64_t)gridDim.x * 256u;
-    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) {
+        float4 v;
+        v.x = __builtin_nontemporal_load(&src[i].x);
+        v.y = __builtin_nontemporal_load(&src[i].y);
+        v.z = __builtin_nontemporal_load(&src[i].z);
+        v.w = __builtin_nontemporal_load(&src[i].w);
+        __builtin_nontemporal_store(v.x, &dst[i].x);
+        __builtin_nontemporal_store(v.y, &dst[i].y);
+        __builtin_nontemporal_store(v.z, &dst[i].z);
+        __builtin_nontemporal_store(v.w, &dst[i].w);
+    }
 }
 
 static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }

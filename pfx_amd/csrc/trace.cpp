@@ -734,17 +734,20 @@ pfxh_status pfxh_hbm_probe(uint64_t bytes, uint32_t reps, double* gbs, char* err
             prop.multiProcessorCount > 0)
             cus = prop.multiProcessorCount;
         uint64_t n = bytes / 16;
-        uint32_t blocks = (uint32_t)cus * 8u;
         float best = 0.0f;
-        for (uint32_t r = 0; r <= reps; ++r) {  // the first copy warms up
-            HIP_TRY(hipEventRecord(ev[0], s));
-            launch_stream_copy(s, (const float4*)src, (float4*)dst, n, blocks);
-            HIP_TRY(hipEventRecord(ev[1], s));
-            HIP_TRY(hipEventSynchronize(ev[1]));
-            HIP_TRY(hipGetLastError());
-            float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
-            if (r > 0 && (best == 0.0f || ms < best)) best = ms;
+        // the best over a few launch shapes: 2, 4 and 8 blocks of 256 per CU
+        for (uint32_t per : {2u, 4u, 8u}) {
+            uint32_t blocks = (uint32_t)cus * per;
+            for (uint32_t r = 0; r <= reps; ++r) {  // the first copy of each shape warms up
+                HIP_TRY(hipEventRecord(ev[0], s));
+                launch_stream_copy(s, (const float4*)src, (float4*)dst, n, blocks);
+                HIP_TRY(hipEventRecord(ev[1], s));
+                HIP_TRY(hipEventSynchronize(ev[1]));
+                HIP_TRY(hipGetLastError());
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+                if (r > 0 && (best == 0.0f || ms < best)) best = ms;
+            }
         }
         *gbs = 2.0 * (double)bytes / (best * 1e-3) / 1e9;
     });
