@@ -30,7 +30,7 @@ struct pfxh_batch {
     hipEvent_t ev[2] = {};
     std::vector<void*> allocations;
     WaveDev W{};
-    uint32_t *qa = nullptr, *qb = nullptr, *active = nullptr;
+    uint32_t *qa = nullptr, *qb = nullptr, *qh = nullptr, *active = nullptr;
     uint32_t capacity = 0, max_pixels = 0, total_bounces = 0, cus = 256;
     SceneDev* scenes = nullptr;
     AccumDev* accums = nullptr;
@@ -266,6 +266,7 @@ pfxh_status pfxh_batch_new(const pfxh_batch_still* stills, uint32_t count, const
         W.e_c = (float4*)b->alloc(cap * entries * 16, "shadow ray contributions");
         b->qa = (uint32_t*)b->alloc(cap * 4, "ray queue");
         b->qb = (uint32_t*)b->alloc(cap * 4, "ray queue");
+        b->qh = (uint32_t*)b->alloc(cap * 4, "hit queue");
         W.q_sh = (uint32_t*)b->alloc(cap * 4, "shadow path queue");
         W.q_sray = (uint32_t*)b->alloc(cap * entries * 4, "shadow ray queue");
         W.counters = (uint32_t*)b->alloc(C_COUNT * 4, "counters");
@@ -283,6 +284,7 @@ pfxh_status pfxh_batch_new(const pfxh_batch_still* stills, uint32_t count, const
             t->W = W;
             t->qa = b->qa;
             t->qb = b->qb;
+            t->qh = b->qh;
             t->active = b->active;
             t->capacity = b->capacity;
             t->wave_shared = true;

@@ -96,7 +96,8 @@ enum Counter : uint32_t {
     C_SFETCH = C_FETCH + PARTS * SLOT, // PARTS work counters of k_shadow
     C_N = C_SFETCH + PARTS * SLOT,     // batch: rays, shadow rays, shadow paths of the bounce
     C_STATS = C_N + SLOT,              // batch: 64-bit closest-hit and any-hit ray counts
-    C_COUNT = C_STATS + SLOT
+    C_HITS = C_STATS + SLOT,           // paths of the bounce whose ray hit (the shading queue)
+    C_COUNT = C_HITS + SLOT
 };
 
 // A batch renders many stills (own scene, size, sample count and seed each)
@@ -134,8 +135,10 @@ void launch_camera(hipStream_t s, const SceneDev& S, const WaveDev& W, const uin
                    uint32_t nsamples, uint32_t acc, uint32_t seed);
 void launch_extend(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* queue, uint32_t n,
                    bool count, uint32_t grid);
+void launch_hits(hipStream_t s, const WaveDev& W, const uint32_t* queue, uint32_t n, uint32_t* hitq);
+// Shades queue[0, n), or queue[0, *count) when count is given (n then bounds it).
 void launch_shade(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* queue, uint32_t n,
-                  uint32_t bounce);
+                  uint32_t bounce, const uint32_t* count = nullptr);
 void launch_shadow(hipStream_t s, const SceneDev& S, const WaveDev& W, uint32_t n, bool count, uint32_t grid);
 void launch_fold(hipStream_t s, const WaveDev& W, uint32_t n);
 void launch_finish(hipStream_t s, const WaveDev& W, const AccumDev& A, const uint32_t* active, uint32_t npix,

@@ -77,6 +77,26 @@ DEV uint32_t wave_scan(uint32_t v) {
     return v;
 }
 
+// Append a items per thread (0 or 1) to the queue whose length is *counter
+// with one atomic for the whole 256-thread block; returns each thread's
+// position. Every thread of the block must call it.
+DEV uint32_t block_append1(uint32_t* counter, uint32_t a) {
+    __shared__ uint32_t tot[4];
+    __shared__ uint32_t base;
+    uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    uint32_t ia = wave_scan(a);
+    if (lane == 63) tot[wave] = ia;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = tot[0] + tot[1] + tot[2] + tot[3];
+        base = sum ? atomicAdd(counter, sum) : 0u;
+    }
+    __syncthreads();
+    uint32_t r = base + ia - a;
+    for (uint32_t w = 0; w < wave; ++w) r += tot[w];
+    return r;
+}
+
 // Append a, b and c items per thread to the next-bounce, shadow path and
 // shadow ray queues with one atomic per queue for the whole 256-thread
 // block; returns each thread's first position in the three queues. Every
@@ -1469,10 +1489,28 @@ DEV void shade_step(const SceneDev& S, const SceneDev* __restrict__ scenes, cons
 // Four waves per SIMD (128 registers, a few spilled) hide more of the
 // shading's memory latency than three with no spills (+4% Mrays/s on the
 // 4K terrain).
+// The paths of a bounce whose ray hit something, for shading. Under a black
+// sky a missed path ends with no shading work at all (no draws, no state
+// change), so only hits need shading lanes; the order of the hit queue does
+// not matter, every path being shaded on its own.
+__global__ __launch_bounds__(256) void k_hits(WaveDev W, const uint32_t* queue, uint32_t n, uint32_t* hitq) {
+    uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    uint32_t i = 0;
+    bool hit = false;
+    if (k < n) {
+        i = queue ? queue[k] : k;
+        hit = !(W.hit[i].x >= 1e19f);  // shade_path's miss test, negated
+    }
+    uint32_t at = block_append1(W.counters + C_HITS, hit ? 1u : 0u);
+    if (hit) hitq[at] = i;
+}
+
 template <bool BATCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(SceneDev S, const SceneDev* __restrict__ scenes, WaveDev W, const uint32_t* queue,
-                                               uint32_t n, uint32_t bounce) {
+                                               uint32_t n, uint32_t bounce, const uint32_t* count) {
     if (!BATCH) {
+        if (count) n = *count;
+        if (blockIdx.x * 256u >= n) return;  // the whole block is past the queue
         shade_step<false>(S, scenes, W, queue, n, bounce, blockIdx.x * 256u + threadIdx.x);
         return;
     }
@@ -1887,11 +1925,16 @@ void launch_extend(hipStream_t s, const SceneDev& S, const WaveDev& W, const uin
     else
         hipLaunchKernelGGL((k_extend<false, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, queue, n);
 }
+void launch_hits(hipStream_t s, const WaveDev& W, const uint32_t* queue, uint32_t n, uint32_t* hitq) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_hits, dim3(blocks_for(n, 256)), dim3(256), 0, s, W, queue, n, hitq);
+}
 void launch_shade(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* queue, uint32_t n,
-                  uint32_t bounce) {
+                  uint32_t bounce, const uint32_t* count) {
     if (!n) return;
     const SceneDev* none = nullptr;
-    hipLaunchKernelGGL(k_shade<false>, dim3(blocks_for(n, 256)), dim3(256), 0, s, S, none, W, queue, n, bounce);
+    hipLaunchKernelGGL(k_shade<false>, dim3(blocks_for(n, 256)), dim3(256), 0, s, S, none, W, queue, n, bounce,
+                       count);
 }
 void launch_shadow(hipStream_t s, const SceneDev& S, const WaveDev& W, uint32_t n, bool count, uint32_t grid) {
     if (!n) return;
@@ -1961,7 +2004,8 @@ void launch_extend_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, con
 void launch_shade_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t bounce,
                         uint32_t grid) {
     SceneDev none{};
-    hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, s, none, B.scenes, W, queue, 0u, bounce);
+    const uint32_t* count = nullptr;
+    hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, s, none, B.scenes, W, queue, 0u, bounce, count);
 }
 void launch_shadow_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid) {
     SceneDev none{};

@@ -178,6 +178,7 @@ void trace_setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& deta
     W.e_c = (float4*)t->alloc(cap * entries * 16, "shadow ray contributions");
     t->qa = (uint32_t*)t->alloc(cap * 4, "ray queue");
     t->qb = (uint32_t*)t->alloc(cap * 4, "ray queue");
+    t->qh = (uint32_t*)t->alloc(cap * 4, "hit queue");
     W.q_sh = (uint32_t*)t->alloc(cap * 4, "shadow path queue");
     W.q_sray = (uint32_t*)t->alloc(cap * entries * 4, "shadow ray queue");
     W.counters = (uint32_t*)t->alloc(C_COUNT * 4, "counters");
@@ -241,8 +242,17 @@ void sample(pfxh_trace* t, uint32_t count, uint32_t seed) {
             HIP_TRY(hipEventRecord(t->ev[0], s));
             launch_extend(s, t->S, W, queue, n, t->counting, grid);
             HIP_TRY(hipEventRecord(t->ev[1], s));
+            // under a black sky only the paths that hit need shading
+            const uint32_t* shade_queue = queue;
+            const uint32_t* shade_count = nullptr;
+            if (t->S.env_black && t->qh) {
+                launch_zero(s, W.counters + C_HITS, 1);
+                launch_hits(s, W, queue, n, t->qh);
+                shade_queue = t->qh;
+                shade_count = W.counters + C_HITS;
+            }
             launch_zero(s, W.counters, C_SRAYS + 1);
-            launch_shade(s, t->S, W, queue, n, bounce);
+            launch_shade(s, t->S, W, shade_queue, n, bounce, shade_count);
             HIP_TRY(hipEventRecord(t->ev[2], s));
             HIP_TRY(hipMemcpyAsync(t->host_counters, W.counters, (C_SRAYS + 1) * sizeof(uint32_t),
                                    hipMemcpyDeviceToHost, s));

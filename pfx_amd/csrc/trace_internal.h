@@ -43,6 +43,7 @@ struct pfxh_trace {
     AccumDev A{};
     uint32_t* qa = nullptr;
     uint32_t* qb = nullptr;
+    uint32_t* qh = nullptr;  // hit queue of a bounce (black sky: misses need no shading)
     uint32_t* active = nullptr;
     uint32_t* host_counters = nullptr;  // pinned
     float4* snaps = nullptr;
