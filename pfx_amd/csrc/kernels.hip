@@ -1505,6 +1505,23 @@ __global__ __launch_bounds__(256) void k_hits(WaveDev W, const uint32_t* queue, 
     if (hit) hitq[at] = i;
 }
 
+// The same for a batch wave: the queue length is on the device, and a
+// missed path is dropped only when its own still's sky is black.
+__global__ __launch_bounds__(256) void k_hits_batch(BatchDev B, WaveDev W, const uint32_t* queue, uint32_t* hitq) {
+    uint32_t n = W.counters[C_N];
+    for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+        uint32_t k = base + threadIdx.x;
+        uint32_t i = 0;
+        bool keep = false;
+        if (k < n) {
+            i = queue ? queue[k] : k;
+            keep = !(W.hit[i].x >= 1e19f) || !B.scenes[W.sid[i]].env_black;
+        }
+        uint32_t at = block_append1(W.counters + C_HITS, keep ? 1u : 0u);
+        if (keep) hitq[at] = i;
+    }
+}
+
 template <bool BATCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(SceneDev S, const SceneDev* __restrict__ scenes, WaveDev W, const uint32_t* queue,
                                                uint32_t n, uint32_t bounce, const uint32_t* count) {
@@ -1514,7 +1531,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         shade_step<false>(S, scenes, W, queue, n, bounce, blockIdx.x * 256u + threadIdx.x);
         return;
     }
-    n = W.counters[C_N];
+    n = count ? *count : W.counters[C_N];
     for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u)
         shade_step<true>(S, scenes, W, queue, n, bounce, base + threadIdx.x);
 }
@@ -1711,6 +1728,7 @@ __global__ void k_roll(BatchDev B, uint32_t* counters, uint32_t phase, uint32_t 
             counters[C_NEXT] = 0u;
             counters[C_SHADOW] = 0u;
             counters[C_SRAYS] = 0u;
+            counters[C_HITS] = 0u;
         }
     } else {
         uint32_t rays = counters[C_SRAYS], paths = counters[C_SHADOW];
@@ -2001,10 +2019,13 @@ void launch_extend_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, con
     SceneDev none{};
     hipLaunchKernelGGL((k_extend<false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W, queue, 0u);
 }
+void launch_hits_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t* hitq,
+                       uint32_t grid) {
+    hipLaunchKernelGGL(k_hits_batch, dim3(grid), dim3(256), 0, s, B, W, queue, hitq);
+}
 void launch_shade_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t bounce,
-                        uint32_t grid) {
+                        uint32_t grid, const uint32_t* count) {
     SceneDev none{};
-    const uint32_t* count = nullptr;
     hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, s, none, B.scenes, W, queue, 0u, bounce, count);
 }
 void launch_shadow_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid) {
