@@ -166,6 +166,13 @@ def quality_leg(scene) -> dict:
         return round(float(np.mean((x - A) ** 2 / (A * A + 1e-2))), 5), \
             round(float(np.mean((x - A) * (x - B) / (M * M + 1e-2))), 5)
 
+    def worst_share(x):
+        """Share of the unbiased error held by the worst 1% of pixels."""
+        e = ((x[..., :3].astype(np.float64) - A) * (x[..., :3] - B) / (M * M + 1e-2)).sum(axis=-1).ravel()
+        k = max(1, e.size // 100)
+        top = np.partition(e, e.size - k)[e.size - k:]
+        return round(float(top.sum() / e.sum()), 3) if e.sum() > 0 else None
+
     out = {"window": [x0, y0, ww, wh], "reference": "4096 spp, FP64 sums, seeds 7777 (A) and 4242 (B)",
            "relmse_B_vs_A": round(float(np.mean((B - A) ** 2 / (A * A + 1e-2))), 5), "runs": []}
     learner = None
@@ -198,7 +205,8 @@ def quality_leg(scene) -> dict:
             rel, rel_true = scores(den[win])
             out["runs"].append({"spp": spp, "denoiser": how, "trace_ms": round(traced * 1e3, 1),
                                 "denoise_ms": round(denoised * 1e3, 1), "relmse": rel, "relmse_true": rel_true,
-                                "raw_relmse": raw, "raw_relmse_true": raw_true})
+                                "raw_relmse": raw, "raw_relmse_true": raw_true,
+                                "worst_1pct_share": worst_share(den[win])})
             if fallback is not None:
                 out["runs"][-1]["fallback_tiles"] = fallback
             log(f"quality: {spp} spp + {how} denoiser: relMSE {rel} (true {rel_true}; raw {raw}), "
