@@ -101,3 +101,84 @@ def brightness(color: np.ndarray) -> np.ndarray:
     """gpu.rs:4232-4234: Rec. 709 luma of the RGB channels."""
     c = np.asarray(color, np.float64)
     return 0.2126 * c[..., 0] + 0.7152 * c[..., 1] + 0.0722 * c[..., 2]
+
+
+# ---- direct light from point and spot lights (gpu.rs:4443-4560, reduced) ---
+LIT_FLOOR_BASE = (0.5, 0.45, 0.4)
+
+
+def lit_floor() -> tuple:
+    """A Lambertian floor under six point and spot lights, seen from above
+    (the camera of gpu.rs:4477-4482), direct light only: the reference's
+    `local_lights_match_the_live_direct_term` without its analytic occluder
+    and rect light, which this tracer does not have. Six lights exceed the
+    four the reference loops over, so every vertex picks one through the
+    light tree. The floor has specular 0 and metalness 0, where the
+    reference's BRDF is exactly base / pi (brdf.wgsl:206-220: f0 = 0 gives a
+    zero Fresnel term, no specular lobe, full diffuse weight)."""
+    floor = quad((-4.0, 0.0, -4.0), (4.0, 0.0, -4.0), (4.0, 0.0, 4.0), (-4.0, 0.0, 4.0))
+    mats = material(LIT_FLOOR_BASE, roughness=0.6, specular=0.0)
+    lights = np.concatenate([
+        point_light((-0.8, 1.2, 0.3), (1.0, 0.9, 0.8), 3.0, 8.0),
+        point_light((0.6, 0.7, -0.9), (0.8, 1.0, 0.9), 1.5, 2.5),   # range cuts in: the window shows
+        point_light((-1.3, 2.0, -1.1), (0.7, 0.8, 1.0), 6.0, 12.0),
+        spot_light((1.2, 1.5, 1.2), (0.9, 1.0, 0.8), 4.0, 10.0, (0.0, -1.0, 0.0), 15.0, 25.0),
+        spot_light((-1.0, 1.0, 1.4), (1.0, 0.7, 0.6), 2.5, 6.0, (0.3, -1.0, 0.1), 20.0, 40.0),
+        spot_light((0.2, 2.5, -0.2), (0.8, 0.8, 1.0), 8.0, 9.0, (-0.2, -1.0, -0.3), 10.0, 30.0),
+    ])
+    cam = Camera(origin=(0.0, 4.0, 0.0), forward=(0.0, -1.0, 0.0), right=(0.6, 0.0, 0.0), up=(0.0, 0.0, -0.6))
+    scene = Scene(triangles=triangles(floor, 0), materials=mats, camera=cam, sun=NO_SUN, lights=lights)
+    return scene, lights
+
+
+def light_falloff(lights, k, distance):
+    """lights.rs falloff (intensity x window^2 / max(d, radius)^2), vectorised;
+    tests check it against the oracle's, which the reference's known answers
+    pin (lights.rs:578-628)."""
+    l = lights[k]
+    ratio = distance / l["range"]
+    window = np.clip(1.0 - ratio ** 4, 0.0, 1.0)
+    out = l["intensity"] * window * window / np.maximum(distance, l["radius"]) ** 2
+    return np.where((distance >= l["range"]) | (distance <= 1e-6), 0.0, out)
+
+
+def light_cone(lights, k, toward):
+    """lights.rs cone for a spot (1 for a point), vectorised; `toward` runs
+    from the light to the lit point."""
+    l = lights[k]
+    if l["kind"] != N.LIGHT_SPOT:
+        return np.ones(toward.shape[:-1])
+    inner, outer = np.cos(np.radians(l["inner_deg"])), np.cos(np.radians(l["outer_deg"]))
+    scale = 1.0 / max(inner - outer, 0.001)
+    offset = -outer * scale
+    axis = np.asarray(l["direction"], np.float64)
+    axis = axis / np.linalg.norm(axis)
+    t = np.clip((toward / np.linalg.norm(toward, axis=-1, keepdims=True)) @ axis * scale + offset, 0.0, 1.0)
+    return t * t
+
+
+def lit_floor_expected(scene, lights, size, sub=16) -> np.ndarray:
+    """Each pixel's expected colour: the direct term summed over the lights,
+    (base / pi) cos(theta) colour falloff(d) cone, averaged over a sub x sub
+    grid across the pixel (the tracer jitters uniformly inside it)."""
+    w, h = size
+    cam = scene.camera
+    o = np.asarray(cam.origin, np.float64)
+    fwd, right, up = (np.asarray(v, np.float64) for v in (cam.forward, cam.right, cam.up))
+    s = (np.arange(sub) + 0.5) / sub
+    px = (np.arange(w)[:, None] + s[None, :]).ravel()
+    py = (np.arange(h)[:, None] + s[None, :]).ravel()
+    ndc_x = px / w * 2.0 - 1.0
+    ndc_y = py / h * 2.0 - 1.0
+    d = fwd[None, None] + right[None, None] * ndc_x[None, :, None] + up[None, None] * (-ndc_y[:, None, None])
+    d = d / np.linalg.norm(d, axis=-1, keepdims=True)
+    p = o + d * (-o[1] / d[..., 1:2])
+    total = np.zeros(p.shape)
+    base = np.asarray(LIT_FLOOR_BASE) / np.pi
+    for k in range(len(lights)):
+        to = np.asarray(lights[k]["position"], np.float64) - p
+        dist = np.linalg.norm(to, axis=-1)
+        cos = np.clip(to[..., 1] / dist, 0.0, None)
+        val = cos * light_falloff(lights, k, dist) * light_cone(lights, k, -to)
+        total += val[..., None] * np.asarray(lights[k]["colour"], np.float64) * base
+    return total.reshape(h, sub, w, sub, 3).mean(axis=(1, 3))

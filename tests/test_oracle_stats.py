@@ -93,3 +93,38 @@ def test_many_lights_through_the_tree_match_light_by_light():
     assert reference.mean() > 0.0
     assert abs(ratio - 1.0) < 0.01
     assert p95 < 0.04
+
+
+def test_point_and_spot_lights_match_the_direct_term():
+    """gpu.rs:4443-4560 reduced to point and spot lights: a Lambertian floor
+    lit by six lights through the light tree, direct light only, converges
+    to the closed-form direct term (base / pi) cos colour falloff cone. The
+    falloff and cone of the closed form are checked here against the
+    oracle's, which the reference's known answers pin."""
+    import ctypes as C
+
+    from oracle import oracle as O
+
+    scene, lights = S.lit_floor()
+    lib = O.lib()
+    rng = np.random.default_rng(3)
+    for k in range(len(lights)):
+        one = np.ascontiguousarray(lights[k:k + 1])
+        for _ in range(40):
+            dist = float(rng.uniform(0.05, 13.0))
+            ref = lib.or_light_falloff(one.ctypes.data, dist)
+            assert abs(float(S.light_falloff(lights, k, np.float64(dist))) - ref) <= 2e-5 * max(ref, 1e-3)
+            toward = rng.normal(size=3).astype(np.float32)
+            toward[1] = -abs(toward[1]) - 0.5
+            ref = lib.or_light_cone(one.ctypes.data, toward.ctypes.data_as(C.c_void_p))
+            assert abs(float(S.light_cone(lights, k, toward.astype(np.float64))) - ref) <= 2e-4
+    size = (32, 32)
+    img = render(scene, Detail(bounces=Bounces(0, 0, 0, 0)), size, 1024, 11)[0][..., :3].astype(np.float64)
+    expected = S.lit_floor_expected(scene, lights, size)
+    ratio = img.mean() / expected.mean()
+    print(f"image mean / direct term {ratio:.4f}")
+    assert abs(ratio - 1.0) < 0.01
+    blocks = lambda a: a.reshape(8, 4, 8, 4, 3).mean(axis=(1, 3))
+    e, g = blocks(expected), blocks(img)
+    bright = e > 0.05 * e.max()
+    assert np.all(np.abs(g[bright] - e[bright]) <= 0.03 * e[bright])
