@@ -140,7 +140,7 @@ void record_wave(pfxh_batch* b, const pfxh_batch_tuning& t) {
         launch_extend_batch(s, B, Wb, queue, trace_grid);
         launch_roll(s, B, Wb, 1, bounce);
         // only paths that hit, or whose still has a sky, need shading
-        launch_hits_batch(s, B, Wb, queue, b->qh, shade_grid);
+        launch_hits_batch(s, B, Wb, queue, b->qh, flat_grid);
         launch_shade_batch(s, B, Wb, b->qh, bounce, shade_grid, Wb.counters + C_HITS);
         launch_roll(s, B, Wb, 2, bounce);
         launch_shadow_batch(s, B, Wb, trace_grid);

@@ -77,26 +77,6 @@ DEV uint32_t wave_scan(uint32_t v) {
     return v;
 }
 
-// Append a items per thread (0 or 1) to the queue whose length is *counter
-// with one atomic for the whole 256-thread block; returns each thread's
-// position. Every thread of the block must call it.
-DEV uint32_t block_append1(uint32_t* counter, uint32_t a) {
-    __shared__ uint32_t tot[4];
-    __shared__ uint32_t base;
-    uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
-    uint32_t ia = wave_scan(a);
-    if (lane == 63) tot[wave] = ia;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t sum = tot[0] + tot[1] + tot[2] + tot[3];
-        base = sum ? atomicAdd(counter, sum) : 0u;
-    }
-    __syncthreads();
-    uint32_t r = base + ia - a;
-    for (uint32_t w = 0; w < wave; ++w) r += tot[w];
-    return r;
-}
-
 // Append a, b and c items per thread to the next-bounce, shadow path and
 // shadow ray queues with one atomic per queue for the whole 256-thread
 // block; returns each thread's first position in the three queues. Every
@@ -1489,37 +1469,73 @@ DEV void shade_step(const SceneDev& S, const SceneDev* __restrict__ scenes, cons
 // Four waves per SIMD (128 registers, a few spilled) hide more of the
 // shading's memory latency than three with no spills (+4% Mrays/s on the
 // 4K terrain).
+// A block compacts the chunk [lo, hi) of a queue into `out` in two passes:
+// it counts the entries to keep (remembering each thread's answers in a
+// bit mask), takes their room with one atomic on `counter`, then writes
+// them in queue order. One atomic per chunk, not per 256 entries:
+// device-scope atomics on one counter serialize across the XCDs (one per
+// 256 entries cost 0.3 ms per 8M-entry launch). Every thread of the block
+// calls it with the same chunk.
+template <class Keep>
+DEV void compact_chunk(uint32_t lo, uint32_t hi, const uint32_t* queue, Keep keep, uint32_t* counter,
+                       uint32_t* out) {
+    __shared__ uint32_t tot[4];
+    __shared__ uint32_t base;
+    uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    unsigned long long bits = 0ull;
+    uint32_t count = 0, it = 0;
+    for (uint32_t k = lo + threadIdx.x; k < hi; k += 256u, ++it) {
+        bool kp = keep(queue ? queue[k] : k);
+        if (kp && it < 64u) bits |= 1ull << it;
+        count += kp ? 1u : 0u;
+    }
+    uint32_t ws = wave_scan(count);
+    if (lane == 63) tot[wave] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = tot[0] + tot[1] + tot[2] + tot[3];
+        base = sum ? atomicAdd(counter, sum) : 0u;
+    }
+    __syncthreads();
+    uint32_t pos = base;
+    it = 0;
+    for (uint32_t t0 = lo; t0 < hi; t0 += 256u, ++it) {
+        uint32_t k = t0 + threadIdx.x, i = 0;
+        bool kp = false;
+        if (k < hi) {
+            i = queue ? queue[k] : k;
+            kp = it < 64u ? ((bits >> it) & 1ull) != 0ull : keep(i);
+        }
+        uint32_t sc = wave_scan(kp ? 1u : 0u);
+        __syncthreads();  // the previous tile's totals have been read
+        if (lane == 63) tot[wave] = sc;
+        __syncthreads();
+        uint32_t at = pos + sc - (kp ? 1u : 0u);
+        for (uint32_t w = 0; w < wave; ++w) at += tot[w];
+        if (kp) out[at] = i;
+        pos += tot[0] + tot[1] + tot[2] + tot[3];
+    }
+}
+
 // The paths of a bounce whose ray hit something, for shading. Under a black
 // sky a missed path ends with no shading work at all (no draws, no state
 // change), so only hits need shading lanes; the order of the hit queue does
 // not matter, every path being shaded on its own.
 __global__ __launch_bounds__(256) void k_hits(WaveDev W, const uint32_t* queue, uint32_t n, uint32_t* hitq) {
-    uint32_t k = blockIdx.x * 256u + threadIdx.x;
-    uint32_t i = 0;
-    bool hit = false;
-    if (k < n) {
-        i = queue ? queue[k] : k;
-        hit = !(W.hit[i].x >= 1e19f);  // shade_path's miss test, negated
-    }
-    uint32_t at = block_append1(W.counters + C_HITS, hit ? 1u : 0u);
-    if (hit) hitq[at] = i;
+    uint32_t chunk = (n + gridDim.x - 1u) / gridDim.x;
+    uint32_t lo = min(n, blockIdx.x * chunk), hi = min(n, lo + chunk);
+    auto hit = [&](uint32_t i) { return !(W.hit[i].x >= 1e19f); };  // shade_path's miss test, negated
+    compact_chunk(lo, hi, queue, hit, W.counters + C_HITS, hitq);
 }
 
 // The same for a batch wave: the queue length is on the device, and a
 // missed path is dropped only when its own still's sky is black.
 __global__ __launch_bounds__(256) void k_hits_batch(BatchDev B, WaveDev W, const uint32_t* queue, uint32_t* hitq) {
     uint32_t n = W.counters[C_N];
-    for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
-        uint32_t k = base + threadIdx.x;
-        uint32_t i = 0;
-        bool keep = false;
-        if (k < n) {
-            i = queue ? queue[k] : k;
-            keep = !(W.hit[i].x >= 1e19f) || !B.scenes[W.sid[i]].env_black;
-        }
-        uint32_t at = block_append1(W.counters + C_HITS, keep ? 1u : 0u);
-        if (keep) hitq[at] = i;
-    }
+    uint32_t chunk = (n + gridDim.x - 1u) / gridDim.x;
+    uint32_t lo = min(n, blockIdx.x * chunk), hi = min(n, lo + chunk);
+    auto keep = [&](uint32_t i) { return !(W.hit[i].x >= 1e19f) || !B.scenes[W.sid[i]].env_black; };
+    compact_chunk(lo, hi, queue, keep, W.counters + C_HITS, hitq);
 }
 
 template <bool BATCH>
@@ -1945,7 +1961,9 @@ void launch_extend(hipStream_t s, const SceneDev& S, const WaveDev& W, const uin
 }
 void launch_hits(hipStream_t s, const WaveDev& W, const uint32_t* queue, uint32_t n, uint32_t* hitq) {
     if (!n) return;
-    hipLaunchKernelGGL(k_hits, dim3(blocks_for(n, 256)), dim3(256), 0, s, W, queue, n, hitq);
+    // chunks of at most 64 tiles of 256 (the bit mask), at least 2048 blocks
+    uint32_t blocks = std::max(2048u, blocks_for(n, 64u * 256u));
+    hipLaunchKernelGGL(k_hits, dim3(blocks), dim3(256), 0, s, W, queue, n, hitq);
 }
 void launch_shade(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* queue, uint32_t n,
                   uint32_t bounce, const uint32_t* count) {
