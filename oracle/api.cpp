@@ -18,9 +18,10 @@ void or_spawn_point(const float* p, const float* dir, const float* origin, float
     out[1] = r.y;
     out[2] = r.z;
 }
-// 0 sin, 1 cos, 2 acos, 3 atan2, 4 exp2, 5 log2, 6 pow, 7 asin
+// 0 sin, 1 cos, 2 acos, 3 atan2, 4 exp2, 5 log2, 6 pow, 7 asin, 8 cos(max(0, acos x - acos y))
 float or_fm(int which, float x, float y) {
     switch (which) {
+        case 8: return pfx::fm_cos_acos_gap(x, y);
         case 0: return pfx::fm_sin(x);
         case 1: return pfx::fm_cos(x);
         case 2: return pfx::fm_acos(x);

@@ -612,8 +612,9 @@ static F2 emitter_importance(const GLight& light, v3 p, v3 n) {  // :255-271
     float energy = (light.b[0] + light.b[1] + light.b[2]) / 3.0f;
     if ((uint32_t)light.a[3] == 1u) {
         float theta_e = pfx::fm_acos(fm_clamp(-light.d[1] / light.d[0], -1.0f, 1.0f));
-        float cos_min_out = fm_cos(fmaxf(0.0f, pfx::fm_acos(fm_clamp(dot(t3(light.c), -toward), -1.0f, 1.0f)) -
-                                                 pfx::fm_acos(cos_u)));
+        // cos(max(0, acos(cone . -toward) - acos(cos_u))) through the identity in
+        // fmath.h (the same value in real arithmetic, no transcendental calls)
+        float cos_min_out = pfx::fm_cos_acos_gap(fm_clamp(dot(t3(light.c), -toward), -1.0f, 1.0f), cos_u);
         energy *= spot_cone(light, cos_min_out);
         return tree_importance(n, toward, cos_u, t3(light.c), 0.0f, theta_e, far_, near_, energy);
     }

@@ -85,6 +85,17 @@ PFX_HD float fm_acos(float x) {
     if (x > 0.5f) return 2.0f * fm_asin_core(sqrtf(0.5f * (1.0f - x)));
     return 1.5707963267948966f - fm_asin_core(x);
 }
+// cos(max(0, acos(a) - acos(b))) for a, b in [-1, 1], without transcendental
+// calls: acos falls as its argument grows, so the gap is 0 (and the cosine 1)
+// when a >= b; otherwise cos(x - y) = cos x cos y + sin x sin y with the
+// sines of angles in [0, pi] the non-negative square roots (1 - v^2 taken
+// as (1 - v)(1 + v), whose factors are exact near v = +-1).
+PFX_HD float fm_cos_acos_gap(float a, float b) {
+    if (a >= b) return 1.0f;
+    float sa = (1.0f - a) * (1.0f + a), sb = (1.0f - b) * (1.0f + b);
+    float r = a * b + sqrtf(sa > 0.0f ? sa : 0.0f) * sqrtf(sb > 0.0f ? sb : 0.0f);
+    return r < 1.0f ? r : 1.0f;
+}
 PFX_HD float fm_asin(float x) {
     if (x > 0.5f) return 1.5707963267948966f - 2.0f * fm_asin_core(sqrtf(0.5f * (1.0f - x)));
     if (x < -0.5f) return -(1.5707963267948966f - 2.0f * fm_asin_core(sqrtf(0.5f * (1.0f + x))));

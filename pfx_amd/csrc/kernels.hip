@@ -893,8 +893,8 @@ DEV float2 emitter_importance(const SceneDev& S, uint32_t li, v3 p, v3 n) {  // 
     // θo = 0 and θe = acos(-ld.y / ld.x) for a spot, π/2 otherwise: in light_cones
     float4 cone = S.light_cones[2u * li], angles = S.light_cones[2u * li + 1u];
     if ((uint32_t)la.w == 1u) {
-        float cos_min_out =
-            fm_cos(fmaxf(0.0f, fm_acos(fm_clamp(dot(f3(lc), -toward), -1.0f, 1.0f)) - fm_acos(cos_u)));
+        // cos(max(0, acos(cone . -toward) - acos(cos_u))), lights.wgsl:262-264
+        float cos_min_out = pfx::fm_cos_acos_gap(fm_clamp(dot(f3(lc), -toward), -1.0f, 1.0f), cos_u);
         float along = fm_clamp(cos_min_out * ld.x + ld.y, 0.0f, 1.0f);
         energy *= along * along;
         return tree_importance(n, toward, cos_u, f3(lc), cone, angles, far_, near_, energy);

@@ -11,7 +11,7 @@ import pytest
 
 from oracle import oracle as O
 
-SIN, COS, ACOS, ATAN2, EXP2, LOG2, POW, ASIN = range(8)
+SIN, COS, ACOS, ATAN2, EXP2, LOG2, POW, ASIN, COS_ACOS_GAP = range(9)
 
 
 def run(which, x, y=None):
@@ -67,6 +67,23 @@ def test_atan2_all_quadrants():
     want = np.arctan2(y.astype(np.float64), x.astype(np.float64))
     assert np.max(np.abs(got - want)) < 5e-7
     assert run(ATAN2, np.float32([1.0]), np.float32([0.0]))[0] == np.float32(np.pi / 2)
+
+
+def test_cos_of_an_acos_gap():
+    """cos(max(0, acos a - acos b)) through its identity, against numpy in
+    float64 over the light tree's arguments (a a clamped dot product, b the
+    cosine of a light's half angle, in (0, 1]), near the fold a = b and at the
+    ends of the range."""
+    a = np.concatenate([rng.uniform(-1, 1, 20000), rng.uniform(0.9, 1.0, 5000), [-1.0, 1.0, 0.0]])
+    b = np.concatenate([rng.uniform(0, 1, 20000), rng.uniform(0.9, 1.0, 5000), [1.0, 1.0, 0.5]])
+    near = rng.uniform(0.0, 1.0, 2000)
+    a = np.concatenate([a, near * (1.0 - 1e-6)]).astype(np.float32)
+    b = np.concatenate([b, near]).astype(np.float32)
+    got = run(COS_ACOS_GAP, a, b)
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    want = np.cos(np.maximum(0.0, np.arccos(a64) - np.arccos(b64)))
+    assert np.max(np.abs(got - want)) < 2.5e-7
+    assert np.all(got <= 1.0) and np.all(got[a >= b] == 1.0)
 
 
 def test_pow_shading_range():
