@@ -263,6 +263,17 @@ pfxh_status pfxh_trace_set_projection(pfxh_trace* trace, uint32_t kind, float wi
 uint32_t pfxh_trace_samples(const pfxh_trace* trace);
 /* Trace::gpu_bytes (gpu.rs:2360) */
 uint64_t pfxh_trace_gpu_bytes(const pfxh_trace* trace);
+/* Closest hits of `count` caller rays (origins and directions, 3 floats each,
+ * in world space) against the trace's scene, with the trace's triangle test:
+ * primitive[k] is the caller's triangle id as in pfxh_probe_hit (UINT32_MAX
+ * on a miss) and distance[k] the hit distance along the direction (0 on a
+ * miss). Hits closer than 1e-4 do not count; limit bounds the distance.
+ * No counterpart in the reference (its only ray query is the camera probe);
+ * for callers and tests that aim rays themselves. */
+pfxh_status pfxh_trace_cast(pfxh_trace* trace, const float* origins, const float* directions,
+                            uint32_t count, float limit, uint32_t* primitive, float* distance,
+                            char* err, size_t err_len);
+
 /* Trace::probe (crates/trace/src/probe.rs:42; probe.wgsl): pixel-centre
  * primary rays. `out` holds width*height records. */
 pfxh_status pfxh_trace_probe(pfxh_trace* trace, pfxh_probe_hit* out, char* err, size_t err_len);

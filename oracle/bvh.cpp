@@ -408,6 +408,9 @@ std::string lay_scene(const pfxh_scene& sc, Laid& out) {
         }
         return ts;
     };
+    for (uint32_t m = 0; m < sc.mesh_count; ++m)
+        if ((uint64_t)sc.meshes[m].first + sc.meshes[m].count > sc.mesh_triangle_count)
+            return "mesh range is outside mesh_triangles";
     for (uint32_t i = 0; i < sc.instance_count; ++i) {
         const pfxh_instance& inst = sc.instances[i];
         if (inst.mesh >= sc.mesh_count) return "instance names a missing mesh";

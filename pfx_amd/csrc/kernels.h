@@ -148,6 +148,8 @@ void launch_develop(hipStream_t s, const AccumDev& A, uint32_t pixels, float4* c
 void launch_clear(hipStream_t s, const AccumDev& A, uint32_t pixels);
 void launch_active(hipStream_t s, const AccumDev& A, uint32_t pixels, uint32_t* active, uint32_t* counter);
 void launch_probe(hipStream_t s, const SceneDev& S, const WaveDev& W, void* out);
+void launch_cast(hipStream_t s, const SceneDev& S, const WaveDev& W, const float4* o, const float4* d, uint32_t n,
+                 float limit, uint32_t* ids, float* t);
 void launch_stream_copy(hipStream_t s, const float4* src, float4* dst, uint64_t n, uint32_t blocks);
 void launch_measure(hipStream_t s, const AccumDev& A, float4* snaps, uint32_t width, uint32_t height);
 void launch_mark(hipStream_t s, const AccumDev& A, const float4* snaps, uint32_t width, uint32_t height,

@@ -2003,6 +2003,28 @@ void launch_active(hipStream_t s, const AccumDev& A, uint32_t pixels, uint32_t* 
     if (!pixels) return;
     hipLaunchKernelGGL(k_active, dim3(blocks_for(pixels, 256)), dim3(256), 0, s, A, pixels, active, counter);
 }
+// Closest hits of a caller's rays (origins and directions as float4, w
+// unused): the caller's triangle id per ray, UINT32_MAX on a miss, and the
+// distance (0 on a miss).
+__global__ __launch_bounds__(TRACE_BLOCK) void k_cast(SceneDev S, WaveDev W, const float4* o, const float4* d,
+                                                      uint32_t n, float limit, uint32_t* ids, float* t) {
+    __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
+    uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    uint32_t stride = gridDim.x * TRACE_BLOCK;
+    Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
+    for (uint32_t k = gtid; k < n; k += stride) {
+        Closest h = trace_closest(S, f3(o[k]), f3(d[k]), limit, st);
+        bool hit = h.t < limit;
+        ids[k] = hit ? S.slot_prim[h.slot] : 0xffffffffu;
+        t[k] = hit ? h.t : 0.0f;
+    }
+}
+void launch_cast(hipStream_t s, const SceneDev& S, const WaveDev& W, const float4* o, const float4* d, uint32_t n,
+                 float limit, uint32_t* ids, float* t) {
+    if (!n) return;
+    uint32_t blocks = std::min(blocks_for(n, TRACE_BLOCK), W.spill_threads / TRACE_BLOCK);
+    hipLaunchKernelGGL(k_cast, dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, W, o, d, n, limit, ids, t);
+}
 void launch_probe(hipStream_t s, const SceneDev& S, const WaveDev& W, void* out) {
     uint32_t pixels = S.width * S.height;
     if (!pixels) return;

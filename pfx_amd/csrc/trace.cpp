@@ -783,6 +783,36 @@ pfxh_status pfxh_trace_probe(pfxh_trace* t, pfxh_probe_hit* out, char* err, size
     });
 }
 
+pfxh_status pfxh_trace_cast(pfxh_trace* t, const float* origins, const float* directions, uint32_t count,
+                            float limit, uint32_t* primitive, float* distance, char* err, size_t err_len) {
+    if (!t || (count && (!origins || !directions || !primitive || !distance)) || !(limit > 0.0f))
+        return PFXH_INVALID;
+    if (!count) return PFXH_OK;
+    void* buf = nullptr;
+    pfxh_status st = guarded(err, err_len, [&] {
+        size_t n = count;
+        HIP_TRY(hipMalloc(&buf, n * 40));
+        float4* o = (float4*)buf;
+        float4* d = o + n;
+        uint32_t* ids = (uint32_t*)(d + n);
+        float* dist = (float*)(ids + n);
+        std::vector<float4> ho(n), hd(n);
+        for (size_t k = 0; k < n; ++k) {
+            ho[k] = make_float4(origins[3 * k], origins[3 * k + 1], origins[3 * k + 2], 0.0f);
+            hd[k] = make_float4(directions[3 * k], directions[3 * k + 1], directions[3 * k + 2], 0.0f);
+        }
+        HIP_TRY(hipMemcpyAsync(o, ho.data(), n * 16, hipMemcpyHostToDevice, t->stream));
+        HIP_TRY(hipMemcpyAsync(d, hd.data(), n * 16, hipMemcpyHostToDevice, t->stream));
+        launch_cast(t->stream, t->S, t->W, o, d, count, limit, ids, dist);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(primitive, ids, n * 4, hipMemcpyDeviceToHost, t->stream));
+        HIP_TRY(hipMemcpyAsync(distance, dist, n * 4, hipMemcpyDeviceToHost, t->stream));
+        HIP_TRY(hipStreamSynchronize(t->stream));
+    });
+    if (buf) (void)hipFree(buf);
+    return st;
+}
+
 pfxh_status pfxh_trace_stats(const pfxh_trace* t, pfxh_stats* out) {
     if (!t || !out) return PFXH_INVALID;
     *out = t->stats;

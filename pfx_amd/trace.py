@@ -231,6 +231,18 @@ class Trace:
     def gpu_bytes(self) -> int:
         return int(N.lib().pfxh_trace_gpu_bytes(self._handle()))
 
+    def cast(self, origins, directions, limit: float = 1e20):
+        """Closest hits of caller rays: (caller triangle ids, UINT32_MAX on a
+        miss; distances, 0 on a miss)."""
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
+        ids = np.zeros(len(o), np.uint32)
+        t = np.zeros(len(o), np.float32)
+        err = N.err_buffer()
+        self._check(N.lib().pfxh_trace_cast(self._handle(), o.ctypes.data, d.ctypes.data, len(o), limit,
+                                            ids.ctypes.data, t.ctypes.data, err, len(err)), err)
+        return ids, t
+
     def probe(self) -> Probe:
         hits = np.zeros(self.width * self.height, dtype=N.PROBE_HIT)
         err = N.err_buffer()

@@ -125,3 +125,6 @@ def test_host_side_argument_errors(native_lib):
     assert lib.pfxh_hbm_probe(1 << 10, 3, C.byref(gbs), err, len(err)) == N.INVALID  # under 1 MiB
     assert lib.pfxh_hbm_probe((1 << 20) + 4, 3, C.byref(gbs), err, len(err)) == N.INVALID  # not whole float4s
     assert lib.pfxh_hbm_probe(1 << 20, 0, C.byref(gbs), err, len(err)) == N.INVALID
+    f3 = (C.c_float * 3)()
+    ids = (C.c_uint32 * 1)()
+    assert lib.pfxh_trace_cast(None, f3, f3, 1, C.c_float(1.0), ids, f3, err, len(err)) == N.INVALID
