@@ -1827,7 +1827,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_probe(SceneDev S, WaveDev W, pf
     uint32_t pixels = S.width * S.height;
     v3 fwd = mk(S.forward[0], S.forward[1], S.forward[2]), right = mk(S.right[0], S.right[1], S.right[2]),
        up = mk(S.up[0], S.up[1], S.up[2]);
-    for (uint32_t pix = gtid; pix < pixels; pix += stride) {
+    for (uint64_t p64 = gtid; p64 < pixels; p64 += stride) {  // 64-bit: no wrap for pixels near 2^32
+        uint32_t pix = (uint32_t)p64;
         uint32_t y = pix / S.width, x = pix - y * S.width;
         float ndcx = ((float)(x + S.window[2]) + 0.5f) / (float)S.window[0] * 2.0f - 1.0f;
         float ndcy = ((float)(y + S.window[3]) + 0.5f) / (float)S.window[1] * 2.0f - 1.0f;
@@ -2010,9 +2011,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_cast(SceneDev S, WaveDev W, con
                                                       uint32_t n, float limit, uint32_t* ids, float* t) {
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
-    uint32_t stride = gridDim.x * TRACE_BLOCK;
+    uint64_t stride = (uint64_t)gridDim.x * TRACE_BLOCK;
     Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
-    for (uint32_t k = gtid; k < n; k += stride) {
+    for (uint64_t k = gtid; k < n; k += stride) {  // 64-bit: no wrap for n near 2^32
         Closest h = trace_closest(S, f3(o[k]), f3(d[k]), limit, st);
         bool hit = h.t < limit;
         ids[k] = hit ? S.slot_prim[h.slot] : 0xffffffffu;
