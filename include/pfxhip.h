@@ -162,6 +162,16 @@ enum { PFXH_PROJ_PERSPECTIVE = 0, PFXH_PROJ_ORTHOGRAPHIC = 1, PFXH_PROJ_EQUIRECT
  * the determinant, barycentrics u (weight of b) and v (weight of c). */
 enum { PFXH_TRI_MOLLER_TRUMBORE = 0, PFXH_TRI_WATERTIGHT = 1 };
 
+/* Closest-hit walk order. PFXH_WALK_NEAREST (the default) visits the hit
+ * children of a node nearest first; it returns the reference's hit except
+ * where the reference's own answer depends on its walk order (a ray through
+ * a shared edge or vertex whose hit lies within an ulp of a box face).
+ * PFXH_WALK_LEFT_FIRST is the reference's order (trace.wgsl:999-1075, left
+ * child popped first, :1049-1054): it tests the reference's triangles in
+ * the reference's order, so the closest hit is the reference's on every
+ * ray. Any-hit (shadow) walks are left first in both modes. */
+enum { PFXH_WALK_NEAREST = 0, PFXH_WALK_LEFT_FIRST = 1 };
+
 /* The parts of detail::Detail (crates/trace/src/detail.rs:374-398) the
  * path uses. The sampler is PCG (`Sampler::Pcg`, sobol.rs:13-23). */
 typedef struct pfxh_detail {
@@ -177,6 +187,7 @@ typedef struct pfxh_detail {
     uint32_t no_emitter_sampling; /* 1: emissive triangles are not light-sampled, as
                                    * Trace::build with emitter_sampling = false (gpu.rs:781) */
     uint32_t triangle_test;       /* PFXH_TRI_*; 0 is the reference's test */
+    uint32_t walk_order;          /* PFXH_WALK_*; 0 is nearest first */
 } pfxh_detail;
 
 /* adaptive::Adaptive (crates/trace/src/adaptive.rs:9-26) */
@@ -263,8 +274,8 @@ pfxh_status pfxh_trace_set_projection(pfxh_trace* trace, uint32_t kind, float wi
 uint32_t pfxh_trace_samples(const pfxh_trace* trace);
 /* Trace::gpu_bytes (gpu.rs:2360) */
 uint64_t pfxh_trace_gpu_bytes(const pfxh_trace* trace);
-/* Closest hits of `count` caller rays (origins and directions, 3 floats each,
- * in world space) against the trace's scene, with the trace's triangle test:
+/* Closest hits of `count` caller rays (origins and directions each hold
+ * 3*count floats: x, y, z per ray, in world space) against the trace's scene, with the trace's triangle test:
  * primitive[k] is the caller's triangle id as in pfxh_probe_hit (UINT32_MAX
  * on a miss) and distance[k] the hit distance along the direction (0 on a
  * miss). Hits closer than 1e-4 do not count; limit bounds the distance.

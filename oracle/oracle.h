@@ -6,21 +6,33 @@
 // only code allowed to load it. Every function cites the reference
 // file:line it follows. Built with -ffp-contract=off and no fast-math.
 //
-// Parity status per part (pins live in tests/test_oracle.py):
-//   bvh / levels ........ pinned: brute force (bvh.rs:330-381 design),
-//                         deterministic rebuild, two-level == flattened
+// Parity status per part (pins in tests/test_oracle.py, test_oracle_pins.py,
+// test_select.py, test_fmath.py, test_oracle_stats.py):
+//   bvh ................. pinned: brute force on the reference's own case
+//                         (bvh.rs:330-381, LCG seed 17), deterministic rebuild
+//   levels .............. pinned: the reference's two-level test restated
+//                         (levels.rs:591-681, LCG seed 29): 3 bottom builds,
+//                         130 slots, 3,000 rays == brute force over the
+//                         flattened triangles
 //   emitter CDF ......... pinned: lights.rs:465-515 known answers
 //   light pack/falloff .. pinned: lights.rs:578-628 known answers
-//   light tree build .... partly pinned: tree.rs:433-492 invariants and
-//                         sum-of-pick-pdf == 1; the leaf ORDER follows our
-//                         own deterministic selection, not Rust's
-//                         select_nth_unstable_by (Rust std is absent here):
-//                         leaf order vs the reference is parity unpinned
+//   light tree build .... pinned: tree.rs:433-492 invariants; the split
+//                         restates Rust core's select_nth_unstable_by
+//                         (select_nth.h), pinned by its selection
+//                         properties; its pivot order beyond 16 elements
+//                         has no Rust output to compare: parity unpinned
+//   light tree pick ..... pinned: enumeration over the uniform (each
+//                         light's share == its pdf; pdfs + failed share
+//                         == 1) and a chi-squared test
 //   PCG / offset_ray .... pinned: closed form, math.rs:80-105 known answers
 //   adaptive rounds ..... pinned: adaptive.rs:391-461 known answers
 //   radiance ............ pinned statistically: white furnace, emitter
-//                         NEE vs BSDF-only, tree vs light-by-light, FP64
-//                         converged means (gpu.rs:3787-3923, 4238-4441)
+//                         NEE vs BSDF-only, tree vs light-by-light, the
+//                         closed-form direct term, FP64 converged means
+//                         (gpu.rs:3787-3923, 4238-4560)
+//   transcendentals ..... our Cephes-method functions (fmath.h), pinned
+//                         against numpy; the reference's driver's bits are
+//                         parity unpinned
 #pragma once
 #include <stdint.h>
 #include <string>

@@ -51,6 +51,7 @@ def lib() -> C.CDLL:
             "or_probe": (None, [_P, _P]),
             "or_sample_adaptive": (C.c_int, [_P, _P, _U, _P]),
             "or_tree_pick": (None, [_P, _P, _P, _F, _P]),
+            "or_tree_pick_many": (None, [_P, _P, _P, _P, _U, _P]),
             "or_light_meta": (_U, [_P, C.POINTER(_U), C.POINTER(_U)]),
             "or_trace_counts": (None, [_P, _P, _P, _P]),
             "or_cast": (None, [_P, _P, _P, _U, C.c_int, _F, _P, _P]),
@@ -148,6 +149,15 @@ class Oracle:
         out = np.zeros(2, np.float32)
         lib().or_tree_pick(self._h, p.ctypes.data, n.ctypes.data, u, out.ctypes.data)
         return float(out[0]), float(out[1])
+
+    def tree_picks(self, p, n, u):
+        """tree_pick at every uniform of `u`: (light rows, pdfs)."""
+        p = np.asarray(p, np.float32)
+        n = np.asarray(n, np.float32)
+        u = np.ascontiguousarray(u, np.float32)
+        out = np.zeros((len(u), 2), np.float32)
+        lib().or_tree_pick_many(self._h, p.ctypes.data, n.ctypes.data, u.ctypes.data, len(u), out.ctypes.data)
+        return out[:, 0].astype(np.int64), out[:, 1]
 
     def light_meta(self):
         looped, tree = _U(), _U()

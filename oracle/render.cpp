@@ -1326,6 +1326,18 @@ void or_tree_pick(void* h, const float* p, const float* n, float u, float* out) 
     out[0] = r.x;
     out[1] = r.y;
 }
+// tree_pick at `count` uniforms u[k] for one shading point: out[2k] the
+// picked light row (as a float, as tree_pick returns it), out[2k+1] its pdf.
+void or_tree_pick_many(void* h, const float* p, const float* n, const float* u, uint32_t count, float* out) {
+    const Scene& sc = *(Scene*)h;
+    v3 pp = t3(p), nn = t3(n);
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < (int64_t)count; ++k) {
+        F2 r = tree_pick(sc, pp, nn, u[k]);
+        out[2 * k] = r.x;
+        out[2 * k + 1] = r.y;
+    }
+}
 uint32_t or_light_meta(void* h, uint32_t* looped, uint32_t* tree) {
     Scene* sc = (Scene*)h;
     *looped = sc->lights.looped;

@@ -131,6 +131,7 @@ class DetailC(C.Structure):
         ("ortho_size", C.c_float * 2),
         ("no_emitter_sampling", C.c_uint32),
         ("triangle_test", C.c_uint32),
+        ("walk_order", C.c_uint32),
     ]
 
 

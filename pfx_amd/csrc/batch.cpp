@@ -32,6 +32,7 @@ struct pfxh_batch {
     WaveDev W{};
     uint32_t *qa = nullptr, *qb = nullptr, *qh = nullptr, *active = nullptr;
     uint32_t capacity = 0, max_pixels = 0, total_bounces = 0, cus = 256;
+    bool left_first = false;  // closest-hit walks in the reference's left-first order
     SceneDev* scenes = nullptr;
     AccumDev* accums = nullptr;
     WavePlan* waves = nullptr;
@@ -137,7 +138,7 @@ void record_wave(pfxh_batch* b, const pfxh_batch_tuning& t) {
         WaveDev Wb = W;
         Wb.q_next = outq;
         launch_roll(s, B, Wb, 0, bounce);
-        launch_extend_batch(s, B, Wb, queue, trace_grid);
+        launch_extend_batch(s, B, Wb, queue, trace_grid, b->left_first);
         launch_roll(s, B, Wb, 1, bounce);
         // only paths that hit, or whose still has a sky, need shading
         launch_hits_batch(s, B, Wb, queue, b->qh, flat_grid);
@@ -248,6 +249,7 @@ pfxh_status pfxh_batch_new(const pfxh_batch_still* stills, uint32_t count, const
             work += (uint64_t)stills[i].width * stills[i].height * stills[i].spp;
         }
         b->total_bounces = detail->bounces.total;
+        b->left_first = detail->walk_order == PFXH_WALK_LEFT_FIRST;
         // the wave: at least the largest still once, at most 2^25 paths
         size_t cap = std::max<size_t>(b->max_pixels, std::min<size_t>((size_t)1 << 25, std::max<uint64_t>(work, 1)));
         if (cap >= ((size_t)1 << 28)) throw DeviceError{PFXH_REFUSED, "a still of more than 2^28 pixels"};

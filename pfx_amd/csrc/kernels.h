@@ -10,8 +10,9 @@ namespace pfxh {
 // Scene constants and read-only device buffers (passed by value).
 // SceneDev::geo_flags: GEO_TAME when every box bound is 0 or has magnitude
 // >= 2^-72 (the fast exact slab applies); GEO_WATERTIGHT when the triangles
-// hold vertices and are tested watertight.
-constexpr uint32_t GEO_TAME = 1u, GEO_WATERTIGHT = 2u;
+// hold vertices and are tested watertight; GEO_LEFT_FIRST when closest-hit
+// walks take the children of a node left first, as the reference does.
+constexpr uint32_t GEO_TAME = 1u, GEO_WATERTIGHT = 2u, GEO_LEFT_FIRST = 4u;
 
 struct SceneDev {
     const Node4* nodes;
@@ -33,7 +34,7 @@ struct SceneDev {
     uint32_t env_w, env_h, env_black, lit;
     uint32_t looped, tree, emitter_count, light_slots;
     uint32_t entries;  // NEE entries a vertex may queue
-    uint32_t geo_flags;    // GEO_TAME | GEO_WATERTIGHT
+    uint32_t geo_flags;    // GEO_TAME | GEO_WATERTIGHT | GEO_LEFT_FIRST
     uint32_t total, diffuse, glossy, transmission;
     float sun_dir[3], sun_color[3], sun_intensity, sun_w;
     float origin[3], forward[3], right[3], up[3], lens[4];
@@ -158,7 +159,8 @@ void launch_zero(hipStream_t s, uint32_t* p, uint32_t n);
 
 void launch_camera_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid);
 void launch_roll(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t phase, uint32_t bounce);
-void launch_extend_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t grid);
+void launch_extend_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t grid,
+                         bool left_first);
 void launch_hits_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t* hitq,
                        uint32_t grid);
 // Shades the wave's queue (length on the device), or queue[0, *count) when count is given.

@@ -242,19 +242,30 @@ DEV void order2(uint32_t& ka, uint32_t& ra, uint32_t& kb, uint32_t& rb) {
     ra = r;
 }
 
-// Test the four children of a Node4. With NEAR (closest-hit walks) continue
-// with the nearest hit child (smallest slab entry distance) and push the
-// other hit children farthest first, so they pop nearest first; otherwise
-// (any-hit walks) continue with the first hit child and push the others
-// right to left, the reference's left-first order. Returns false when no
-// child was hit. The closest hit does not depend on the order: the walk
-// visits every node whose entry distance is within the best hit so far,
-// which covers every triangle the reference's left-first walk tests at the
-// final distance, and ties at one distance go to the triangle the
-// reference meets first (Walk::closer). Nearest first cuts the closest-hit
-// walk on the 4K terrain from 25.0 to 20.3 node visits per ray; shadow rays
-// start on a surface whose own boxes are nearest, and walk fewer nodes
-// left first (22.6 against 23.2), which is also where any-hit may stop.
+// Test the four children of a Node4. With NEAR (closest-hit walks by
+// default) continue with the nearest hit child (smallest slab entry
+// distance) and push the other hit children farthest first, so they pop
+// nearest first; otherwise (any-hit walks, and closest-hit walks of a scene
+// built with the left-first order) continue with the first hit child and
+// push the others right to left, the reference's left-first order
+// (trace.wgsl:1049-1054). Returns false when no child was hit.
+// Left first, the walk tests the reference's triangles in the reference's
+// order: a child box is tested at its parent's visit, and its deferred
+// entry distance is compared with the best hit when it is popped, which is
+// the reference's pop-time box test (the intermediate binary node's box
+// contains its children's, so its own test never decides anything).
+// Nearest first, the walk still tests every triangle whose box is entered
+// within the best hit so far, and ties at one distance go to the triangle
+// the left-first walk meets first (Walk::closer); the result can differ
+// from the reference's only where the reference's own answer depends on its
+// order: a hit within an ulp of a box face, where a box's entry distance and
+// the triangle's distance round apart (measured on rays aimed at the
+// vertices and edges of a closed icosphere: 251 of 46,080 Moller-Trumbore
+// and 525 watertight rays keep another triangle, each within 4 ulps of the
+// oracle's distance). Nearest first cuts the closest-hit walk on the 4K
+// terrain from 25.0 to 20.3 node visits per ray; shadow rays start on a
+// surface whose own boxes are nearest, and walk fewer nodes left first
+// (22.6 against 23.2), which is also where any-hit may stop.
 template <bool NEAR, class G>
 DEV bool visit4(const G& S, uint32_t index, const Ray& R, float t, Stack& st, uint32_t& cur) {
     const float4* np = reinterpret_cast<const float4*>(S.nodes + index);
@@ -513,7 +524,11 @@ struct Walk {
     DEV bool step(const G& S, Stack& st, bool& blocked, uint32_t& nodes_ctr, uint32_t& prims_ctr) {
         if (inner()) {
             if (COUNT) nodes_ctr++;
-            if (visit4<!ANY>(S, cur, R, h.t, st, cur)) return true;
+            if (ANY || (S.geo_flags & GEO_LEFT_FIRST) != 0u) {
+                if (visit4<false>(S, cur, R, h.t, st, cur)) return true;
+            } else if (visit4<true>(S, cur, R, h.t, st, cur)) {
+                return true;
+            }
         } else if (leafy()) {
             if (leaf<COUNT, G>(S, prims_ctr)) {
                 blocked = true;
@@ -1197,7 +1212,9 @@ DEV void store_closest(const WaveDev& W, uint32_t i, const Closest& h, uint32_t 
 // In a batch (BATCH) each path belongs to a still of its own (W.sid) whose
 // scene is scenes[sid], and the queue length is the one k_roll left in the
 // counters.
-template <bool COUNT, bool BATCH>
+// NEAR: nearest-first walk (the default); otherwise the reference's
+// left-first order (a scene built with PFXH_WALK_LEFT_FIRST).
+template <bool COUNT, bool BATCH, bool NEAR>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(SceneDev S, const SceneDev* scenes, WaveDev W, const uint32_t* queue, uint32_t n) {
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
@@ -1232,7 +1249,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
                     nodes++;
                     boxes += child_boxes(walk_scene<BATCH>(S, g), w.cur);
                 }
-                if (!visit4<true>(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) {
+                if (!visit4<NEAR>(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) {
                     store_closest(W, ray, w.h, w.record(walk_scene<BATCH>(S, g)));
                     live = false;
                 }
@@ -1955,10 +1972,15 @@ void launch_extend(hipStream_t s, const SceneDev& S, const WaveDev& W, const uin
     if (!n) return;
     uint32_t blocks = std::min(blocks_for(n, TRACE_BLOCK), grid);
     const SceneDev* none = nullptr;
-    if (count)
-        hipLaunchKernelGGL((k_extend<true, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, queue, n);
+    bool left = (S.geo_flags & GEO_LEFT_FIRST) != 0u;
+    if (count && left)
+        hipLaunchKernelGGL((k_extend<true, false, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, queue, n);
+    else if (count)
+        hipLaunchKernelGGL((k_extend<true, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, queue, n);
+    else if (left)
+        hipLaunchKernelGGL((k_extend<false, false, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, queue, n);
     else
-        hipLaunchKernelGGL((k_extend<false, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, queue, n);
+        hipLaunchKernelGGL((k_extend<false, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, queue, n);
 }
 void launch_hits(hipStream_t s, const WaveDev& W, const uint32_t* queue, uint32_t n, uint32_t* hitq) {
     if (!n) return;
@@ -2056,9 +2078,13 @@ void launch_camera_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uin
 void launch_roll(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t phase, uint32_t bounce) {
     hipLaunchKernelGGL(k_roll, dim3(1), dim3(64), 0, s, B, W.counters, phase, bounce);
 }
-void launch_extend_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t grid) {
+void launch_extend_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t grid,
+                         bool left_first) {
     SceneDev none{};
-    hipLaunchKernelGGL((k_extend<false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W, queue, 0u);
+    if (left_first)
+        hipLaunchKernelGGL((k_extend<false, true, false>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W, queue, 0u);
+    else
+        hipLaunchKernelGGL((k_extend<false, true, true>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W, queue, 0u);
 }
 void launch_hits_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t* hitq,
                        uint32_t grid) {

@@ -97,7 +97,8 @@ void trace_setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& deta
     S.slot_prim = t->upload(h.slot_prim, "primitive map");
     S.record_instance = t->upload(h.record_instance, "instance map");
     S.root = h.root;
-    S.geo_flags = (h.bounds_tame ? GEO_TAME : 0u) | (h.watertight ? GEO_WATERTIGHT : 0u);
+    S.geo_flags = (h.bounds_tame ? GEO_TAME : 0u) | (h.watertight ? GEO_WATERTIGHT : 0u) |
+                  (detail.walk_order == PFXH_WALK_LEFT_FIRST ? GEO_LEFT_FIRST : 0u);
     S.env_w = h.env_w;
     S.env_h = h.env_h;
     S.env_black = h.env_black ? 1u : 0u;
@@ -309,7 +310,7 @@ pfxh_status trace_check(const pfxh_scene* scene, const pfxh_detail* detail, uint
         return PFXH_REFUSED;
     }
     if (!projection_valid(detail->projection, detail->ortho_size[0], detail->ortho_size[1]) ||
-        detail->triangle_test > PFXH_TRI_WATERTIGHT) {
+        detail->triangle_test > PFXH_TRI_WATERTIGHT || detail->walk_order > PFXH_WALK_LEFT_FIRST) {
         put_err(err, err_len, "invalid trace detail");
         return PFXH_SETUP;
     }

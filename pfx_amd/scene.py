@@ -100,6 +100,7 @@ class Detail:
     ortho_size: tuple = (0.0, 0.0)  # orthographic view width, height
     emitter_sampling: bool = True  # False: emissive triangles only through BSDF sampling (gpu.rs:781)
     watertight: bool = False  # True: the watertight ray-triangle test instead of the reference's
+    left_first: bool = False  # True: closest hits walk the reference's left-first order (default nearest first)
 
     def to_c(self) -> N.DetailC:
         d = N.DetailC()
@@ -114,6 +115,7 @@ class Detail:
         d.ortho_size[:] = [float(v) for v in self.ortho_size]
         d.no_emitter_sampling = 0 if self.emitter_sampling else 1
         d.triangle_test = 1 if self.watertight else 0
+        d.walk_order = 1 if self.left_first else 0
         return d
 
 

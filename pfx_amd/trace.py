@@ -234,8 +234,13 @@ class Trace:
     def cast(self, origins, directions, limit: float = 1e20):
         """Closest hits of caller rays: (caller triangle ids, UINT32_MAX on a
         miss; distances, 0 on a miss)."""
-        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
-        d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(directions, np.float32)
+        if o.shape != d.shape or o.size % 3 != 0:
+            raise ValueError(f"origins {o.shape} and directions {d.shape} must be equal (n, 3) arrays")
+        o, d = o.reshape(-1, 3), d.reshape(-1, 3)
+        if len(o) >= 2**32:
+            raise ValueError(f"{len(o)} rays do not fit a uint32 count")
         ids = np.zeros(len(o), np.uint32)
         t = np.zeros(len(o), np.float32)
         err = N.err_buffer()

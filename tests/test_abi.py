@@ -6,6 +6,7 @@ import re
 from pathlib import Path
 
 import numpy as np
+import pytest
 
 from pfx_amd import _native as N
 
@@ -34,7 +35,7 @@ def test_struct_sizes_match_the_header():
     assert C.sizeof(N.Camera) == 48
     assert C.sizeof(N.Sun) == 28
     assert C.sizeof(N.Environment) == 16
-    assert C.sizeof(N.DetailC) == 64
+    assert C.sizeof(N.DetailC) == 68
     assert C.sizeof(N.AdaptiveC) == 16
     assert C.sizeof(N.ConvergedC) == 4 + 4 * 64 + 4 + 8 * 64 + 8
     assert C.sizeof(N.SceneC) == 7 * 16 + 16 + 48 + 28 + 4
@@ -128,3 +129,14 @@ def test_host_side_argument_errors(native_lib):
     f3 = (C.c_float * 3)()
     ids = (C.c_uint32 * 1)()
     assert lib.pfxh_trace_cast(None, f3, f3, 1, C.c_float(1.0), ids, f3, err, len(err)) == N.INVALID
+
+
+def test_cast_refuses_unequal_ray_arrays():
+    """Trace.cast checks that origins and directions are equal (n, 3)
+    arrays before the C call, which reads 3*count floats from each."""
+    from pfx_amd.trace import Trace
+
+    o = np.zeros((4, 3), np.float32)
+    for d in (np.zeros((3, 3), np.float32), np.zeros((4, 2), np.float32), np.zeros(11, np.float32)):
+        with pytest.raises(ValueError):
+            Trace.cast(object(), o, d)

@@ -1,0 +1,195 @@
+"""Device parity at the configs' own sizes and on the reference's pins:
+
+* the two-level test of levels.rs:591-681 cast on the device
+  (pfxh_trace_cast): 3,000 rays against brute force over the flattened
+  triangles, and, with the reference's left-first walk, id for id and
+  distance for distance against the oracle;
+* a render under a non-uniform HDR sky at equal spp (the sky slot's
+  sampling, pdf and balance-heuristic MIS: trace.wgsl:2420-2448, sky.rs
+  SKY_SAMPLING_WGSL);
+* the small plain-method still at its full size, 512x512 x 16 spp with 4
+  bounces (probe ids bit-exact, image within relMSE 1e-4);
+* the 3840x2160 terrain's primary hits (8.3M rays): bit-exact with the
+  left-first walk; with the default nearest-first walk the rays whose id
+  differs are counted and bounded;
+* batched stills against the oracle directly, and the 32-still uneven mix
+  (256^2..2048^2, 10k-500k triangles, 1-64 lights, 4/16/64 spp) rendered as
+  one batch, byte-identical to each still's own trace.
+"""
+import numpy as np
+import pytest
+
+import levels_case as L
+from oracle import Oracle
+from pfx_amd import Bounces, Detail, Trace, scenes
+from pfx_amd.batch import Batch, Still
+from test_gpu_batch import mix, same, single
+from test_gpu_parity import rel_mse, small_terrain
+
+pytestmark = pytest.mark.gpu
+
+MISS = 0xFFFFFFFF
+
+
+def test_two_level_pin_on_the_device():
+    scene, placed, origins, directions = L.case()
+    brute = np.array([L.triangle_hits(placed, o, d).min() for o, d in zip(origins, directions)])
+    for left in (False, True):
+        detail = Detail(left_first=left)
+        print(f"levels pin on the device left_first={left}", flush=True)
+        with Trace.new_detailed(scene, detail, 4, 4) as t:
+            ids, dist = t.cast(origins, directions)
+        hit = np.isfinite(brute)
+        assert np.array_equal(ids != MISS, hit)
+        gap = np.abs(dist[hit] - brute[hit])
+        assert np.all(gap <= 1e-4 * np.maximum(brute[hit], 1.0)), float(gap.max())
+        if left:
+            ref = Oracle(scene, detail, 4, 4)
+            rid, rt = ref.cast(origins, directions)
+            ref.close()
+            np.testing.assert_array_equal(ids, rid)
+            np.testing.assert_array_equal(dist, rt)
+
+
+def hdr_sky(w=64, h=32):
+    """A lat-long sky with a gradient and a small bright patch (radiance up
+    to 40): the sky CDF is far from uniform."""
+    v = (np.arange(h) + 0.5) / h
+    u = (np.arange(w) + 0.5) / w
+    vv, uu = np.meshgrid(v, u, indexing="ij")
+    sky = np.zeros((h, w, 4), np.float32)
+    up = np.clip(1.0 - vv * 1.6, 0.05, 1.0)
+    sky[..., 0] = 0.3 * up + 0.05
+    sky[..., 1] = 0.45 * up + 0.05
+    sky[..., 2] = 0.9 * up + 0.08
+    spot = np.exp(-(((uu - 0.3) / 0.03) ** 2 + ((vv - 0.25) / 0.04) ** 2))
+    sky[..., :3] += (40.0 * spot)[..., None] * np.array([1.0, 0.9, 0.7])
+    sky[..., 3] = 1.0
+    return sky
+
+
+@pytest.mark.parametrize("spp", [8])
+def test_sky_lit_render_matches_the_oracle(spp):
+    w, h = 96, 54
+    scene = small_terrain(w, h)
+    scene.sky = hdr_sky()
+    detail = Detail(bounces=Bounces.deep(6))
+    print(f"sky-lit terrain {w}x{h} spp={spp}", flush=True)
+    o = Oracle(scene, detail, w, h)
+    o.sample(spp, 11)
+    rc, ra, rn = o.readback()
+    with Trace.new_detailed(scene, detail, w, h) as t:
+        t.sample(spp, 11)
+        out = t.readback()
+    exact = float(np.mean(out.color == rc))
+    rel = rel_mse(out.color[..., :3], rc[..., :3])
+    print(f"  exact fraction {exact:.6f} relMSE {rel:.3e} mean {float(rc[..., :3].mean()):.4f}")
+    assert float(rc[..., :3].mean()) > 0.05  # the sky does light the scene
+    assert rel <= 1e-4
+    assert exact > 0.99
+    np.testing.assert_allclose(out.normal, rn, rtol=0, atol=1e-4)
+
+
+def test_small_plain_still_at_full_size():
+    """512x512, 16 spp, Bounces::deep(4), box of 12 triangles + 64
+    icospheres of 320 + the emissive ceiling, scene seed 1, trace seed 1."""
+    w = h = 512
+    scene = scenes.box_spheres(seed=1, width=w, height=h)
+    detail = Detail(bounces=Bounces.deep(4))
+    print("box_spheres 512x512 x 16 spp against the oracle", flush=True)
+    o = Oracle(scene, detail, w, h)
+    ref_probe = o.probe()
+    o.sample(16, 1)
+    rc, _, _ = o.readback()
+    with Trace.new_detailed(scene, detail, w, h) as t:
+        probe = t.probe()
+        t.sample(16, 1)
+        out = t.readback()
+    assert np.array_equal(probe.primitive, ref_probe["primitive"])
+    np.testing.assert_array_equal(probe.distance, ref_probe["distance"])
+    exact = float(np.mean(out.color == rc))
+    rel = rel_mse(out.color[..., :3], rc[..., :3])
+    print(f"  exact fraction {exact:.6f} relMSE {rel:.3e}")
+    assert rel <= 1e-4
+    assert exact > 0.99
+
+
+@pytest.mark.timeout(900)
+def test_terrain_4k_primary_hits():
+    """The 3840x2160 terrain (1,048,576 heightfield triangles, 256 instances
+    of 8 meshes, 128 lights): probe ids, instances and distances of all
+    8,294,400 primary rays. Left first they equal the oracle's everywhere;
+    nearest first the differing ids are counted (the reference's own answer
+    depends on its walk order only for hits within an ulp of a box face)."""
+    w, h = 3840, 2160
+    scene = scenes.terrain(seed=2, width=w, height=h)
+    print("4K terrain probe: oracle", flush=True)
+    ref = Oracle(scene, Detail(bounces=Bounces.deep(6)), w, h)
+    rp = ref.probe()
+    ref.close()
+    for left in (True, False):
+        print(f"4K terrain probe: device left_first={left}", flush=True)
+        with Trace.new_detailed(scene, Detail(bounces=Bounces.deep(6), left_first=left), w, h) as t:
+            got = t.probe()
+        differ = got.primitive != rp["primitive"]
+        print(f"  ids differ on {int(differ.sum())} of {w * h} rays")
+        assert np.array_equal(got.primitive == MISS, rp["primitive"] == MISS)
+        if left:
+            assert not differ.any()
+            assert np.array_equal(got.instance, rp["instance"])
+            np.testing.assert_array_equal(got.distance, rp["distance"])
+        else:
+            assert int(differ.sum()) <= (w * h) // 100_000
+            same_ = ~differ
+            np.testing.assert_array_equal(got.distance[same_], rp["distance"][same_])
+            a = got.distance[differ].view(np.int32).astype(np.int64)
+            b = rp["distance"][differ].view(np.int32).astype(np.int64)
+            assert np.all(np.abs(a - b) <= 4)
+
+
+def test_batched_stills_match_the_oracle():
+    """Each still of a batch against the oracle at its own seed and spp."""
+    detail = Detail(bounces=Bounces.deep(4))
+    stills = mix()
+    print(f"batch of {len(stills)} stills against the oracle", flush=True)
+    with Batch(stills, detail) as b:
+        b.render()
+        outs = [b.trace(i).readback() for i in range(len(stills))]
+    for k, (still, out) in enumerate(zip(stills, outs)):
+        o = Oracle(still.scene, detail, still.width, still.height)
+        o.sample(still.spp, still.seed)
+        rc, ra, _ = o.readback()
+        o.close()
+        exact = float(np.mean(out.color == rc))
+        rel = rel_mse(out.color[..., :3], rc[..., :3])
+        print(f"  still {k} {still.width}x{still.height} x {still.spp}: exact {exact:.6f} relMSE {rel:.3e}")
+        assert rel <= 1e-4
+        assert exact > 0.99
+
+
+@pytest.mark.timeout(900)
+def test_uneven_32_still_batch_equals_single_traces():
+    """The 32-still uneven mix at its own sizes, triangle counts, light
+    counts and spp, autotuned and graph-captured, against each still's own
+    trace byte for byte; the smallest still also against the oracle."""
+    detail = Detail(bounces=Bounces.deep(6))
+    print("32-still uneven mix: building scenes", flush=True)
+    stills = [Still(*s) for s in scenes.batch_mix(seed=3)]
+    print(f"  {sum(s.width * s.height for s in stills)} pixels, "
+          f"{sum(s.width * s.height * s.spp for s in stills)} paths", flush=True)
+    with Batch(stills, detail) as b:
+        b.tune()
+        b.render()
+        outs = [b.trace(i).readback() for i in range(len(stills))]
+    for k, (still, out) in enumerate(zip(stills, outs)):
+        print(f"  still {k}: {still.width}x{still.height} x {still.spp} spp", flush=True)
+        same(out, single(still, detail))
+    k = min(range(len(stills)), key=lambda i: stills[i].width * stills[i].height * stills[i].spp)
+    still = stills[k]
+    o = Oracle(still.scene, detail, still.width, still.height)
+    o.sample(still.spp, still.seed)
+    rc, _, _ = o.readback()
+    o.close()
+    rel = rel_mse(outs[k].color[..., :3], rc[..., :3])
+    print(f"  oracle on still {k}: relMSE {rel:.3e}")
+    assert rel <= 1e-4

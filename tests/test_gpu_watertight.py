@@ -180,3 +180,46 @@ def test_batch_of_watertight_stills_equals_single_traces():
         outs = [b.trace(i).readback() for i in range(len(stills))]
     for still, out in zip(stills, outs):
         same(out, single(still, detail))
+
+
+@pytest.mark.parametrize("instanced", [False, True])
+def test_left_first_walk_keeps_the_oracles_hit_on_every_edge_and_vertex_ray(instanced):
+    """With the reference's left-first walk (PFXH_WALK_LEFT_FIRST) the device
+    tests the reference's triangles in the reference's order, so on the
+    same 46,080 edge and vertex rays where the nearest-first walk keeps
+    another triangle for a few hundred, every id and every distance equals
+    the oracle's (trace.wgsl:999-1075, push order :1049-1054), with either
+    triangle test."""
+    scene, world = ico_scene(instanced=instanced)
+    targets = edge_targets(world)
+    rng = np.random.default_rng(5)
+    o = np.concatenate([np.tile(CENTRE, (len(targets), 1)),
+                        (CENTRE + rng.uniform(-0.8, 0.8, (len(targets), 3))).astype(np.float32)])
+    d = (np.concatenate([targets, targets]) - o).astype(np.float32)
+    for wt in (False, True):
+        print(f"left-first edge rays instanced={instanced} watertight={wt} n={len(o)}", flush=True)
+        detail = Detail(watertight=wt, left_first=True)
+        with Trace.new_detailed(scene, detail, 4, 4) as t:
+            ids, dist = t.cast(o, d)
+        rid, rt = oracle_cast(scene, detail, o, d)
+        print(f"  ids differ on {int(np.sum(ids != rid))} rays")
+        np.testing.assert_array_equal(ids, rid)
+        np.testing.assert_array_equal(dist, rt)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_left_first_images_match_the_oracle(case):
+    make, bounces, w, h = CASES[case]
+    scene = make(w, h)
+    detail = Detail(bounces=bounces, left_first=True)
+    print(f"left-first image parity {case} {w}x{h}", flush=True)
+    o = Oracle(scene, detail, w, h)
+    o.sample(8, 7)
+    rc, _, _ = o.readback()
+    with Trace.new_detailed(scene, detail, w, h) as t:
+        t.sample(8, 7)
+        out = t.readback()
+    exact = float(np.mean(out.color == rc))
+    print(f"  exact fraction {exact:.6f}")
+    assert rel_mse(out.color[..., :3], rc[..., :3]) <= 1e-4
+    assert exact > 0.99
