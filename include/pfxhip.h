@@ -398,6 +398,21 @@ pfxh_status pfxh_atrous(uint32_t width, uint32_t height, const float* color, con
                         const float* normal, uint32_t samples, float* out, char* err, size_t err_len);
 /* The same on a trace's current images and sample count, on the device. */
 pfxh_status pfxh_trace_atrous(pfxh_trace* trace, float* out, char* err, size_t err_len);
+/* pfxh_atrous on device memory, allocating nothing. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * color, albedo, normal and out are device pointers to width*height RGBA32F
+ * pixels; out must not alias an input. The passes are queued on `stream`
+ * (a hipStream_t; NULL is the null stream) and the call returns without
+ * waiting: out is ready once the stream has run them. The middle pass's
+ * image lives in one static device array per GPU of 2^21 pixels (32 MiB,
+ * held until the process ends): larger images are refused with
+ * PFXH_INVALID. Every call on a GPU shares that array, so calls that overlap
+ * on one GPU (other streams, other threads) race; serialise them. The
+ * result equals pfxh_atrous's bit for bit. */
+pfxh_status pfxh_atrous_noalloc(uint32_t width, uint32_t height, const float* color, const float* albedo,
+                                const float* normal, uint32_t samples, float* out, void* stream, char* err,
+                                size_t err_len);
 
 /* ---- the repo denoiser (crates/denoise: OIDN 2.5.1's RT filter) ------------ */
 

@@ -238,6 +238,7 @@ SIGNATURES = {
     "pfxh_checkpoint_read": (C.c_int, [_P, _Z, _S, _P, _P, _P, _U, _S, _Z]),
     "pfxh_trace_save": (C.c_int, [_P, _S, _P, _Z, C.POINTER(C.c_size_t), _S, _Z]),
     "pfxh_atrous": (C.c_int, [_U, _U, _P, _P, _P, _U, _P, _S, _Z]),
+    "pfxh_atrous_noalloc": (C.c_int, [_U, _U, _P, _P, _P, _U, _P, _P, _S, _Z]),
     "pfxh_trace_atrous": (C.c_int, [_P, _P, _S, _Z]),
     "pfxh_denoiser_new": (C.c_int, [_S, _Z, C.POINTER(_P), _S, _Z]),
     "pfxh_denoise": (C.c_int, [_P, _U, _U, _P, _P, _P, _P, _S, _Z]),

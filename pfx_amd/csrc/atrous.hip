@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "../../include/pfxhip.h"
@@ -76,6 +77,81 @@ hipError_t pfxh_atrous_device(hipStream_t s, float4* a, float4* b, const float4*
     }
     // three passes leave the result in b; bring it back to a
     return hipMemcpyAsync(a, src, (size_t)w * h * 16, hipMemcpyDeviceToDevice, s);
+}
+
+// ---- the allocation-free variant ------------------------------------------
+// The middle pass's image lives in a file-scope device array instead of
+// device memory the call allocates. The array's name is a host-side stand-in
+// in host code: its device address comes from hipGetSymbolAddress, once per
+// device, with that device current, and only that address is handed to the
+// kernels.
+constexpr uint64_t ATROUS_SCRATCH_PIXELS = 1ull << 21;  // 32 MiB: 1920x1080 fits
+__device__ __align__(256) float4 scratch[ATROUS_SCRATCH_PIXELS];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_lock;
+float4* scratch_at[MAX_DEVICES] = {};
+
+hipError_t scratch_of_current_device(float4** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> guard(scratch_lock);
+    if (!scratch_at[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_at[dev] = (float4*)ptr;
+    }
+    *out = scratch_at[dev];
+    return hipSuccess;
+}
+}  // namespace
+
+extern "C" pfxh_status pfxh_atrous_noalloc(uint32_t width, uint32_t height, const float* color, const float* albedo,
+                                           const float* normal, uint32_t samples, float* out, void* stream,
+                                           char* err, size_t err_len) {
+    if (!color || !albedo || !normal || !out) return PFXH_INVALID;
+    uint64_t len = (uint64_t)width * height;
+    if (len > ATROUS_SCRATCH_PIXELS) {
+        put_err(err, err_len, "image larger than the 2097152 pixels of the allocation-free a-trous filter");
+        return PFXH_INVALID;
+    }
+    if (samples == 0) {
+        put_err(err, err_len, "invalid denoise inputs");
+        return PFXH_INVALID;
+    }
+    if ((const float*)out == color || (const float*)out == albedo || (const float*)out == normal) {
+        put_err(err, err_len, "the output must not alias an input");
+        return PFXH_INVALID;
+    }
+    if (len == 0) return PFXH_OK;
+    float4* mid = nullptr;
+    hipError_t e = scratch_of_current_device(&mid);
+    if (e == hipSuccess) {
+        hipStream_t s = (hipStream_t)stream;
+        uint32_t blocks = (uint32_t)((len + 255u) / 256u);
+        float scale = 2.8f / sqrtf((float)samples);
+        const float4* a = (const float4*)albedo;
+        const float4* n = (const float4*)normal;
+        float4* o = (float4*)out;
+        // colour -> out (step 1), out -> scratch (step 2), scratch -> out (step 4)
+        hipLaunchKernelGGL(k_atrous, dim3(blocks), dim3(256), 0, s, (const float4*)color, o, a, n, width, height, 1,
+                           scale);
+        hipLaunchKernelGGL(k_atrous, dim3(blocks), dim3(256), 0, s, (const float4*)o, mid, a, n, width, height, 2,
+                           scale);
+        hipLaunchKernelGGL(k_atrous, dim3(blocks), dim3(256), 0, s, (const float4*)mid, o, a, n, width, height, 4,
+                           scale);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        put_err(err, err_len, std::string("a-trous filter: ") + hipGetErrorString(e));
+        return PFXH_DEVICE;
+    }
+    return PFXH_OK;
 }
 
 extern "C" pfxh_status pfxh_atrous(uint32_t width, uint32_t height, const float* color, const float* albedo,
