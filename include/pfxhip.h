@@ -80,8 +80,16 @@ typedef struct pfxh_material {
     float fresnel_power; /* <= 0 means 5 */
 } pfxh_material;
 
-/* lights::LocalLight (crates/trace/src/lights.rs:30-39), point and spot. */
-enum { PFXH_LIGHT_POINT = 0, PFXH_LIGHT_SPOT = 1 };
+/* lights::LocalLight (crates/trace/src/lights.rs:30-39) with its four
+ * LightShape kinds (:10-28): point, spot, and the rect and disc area
+ * lights. `position` is the centre of an area light; `colour * intensity`
+ * is a point or spot light's intensity and an area light's power, spread
+ * over its area (lights.rs:207-230). Area lights are never put in the light
+ * tree: each takes its own light slot, and camera and bounce rays that hit
+ * a shadowed one see its radiance (lights.wgsl:64-91). LocalLight::rect and
+ * ::disc (lights.rs:55-99) set radius 0, range FLT_MAX, shadow 1 and
+ * two_sided 0. */
+enum { PFXH_LIGHT_POINT = 0, PFXH_LIGHT_SPOT = 1, PFXH_LIGHT_RECT = 2, PFXH_LIGHT_DISC = 3 };
 typedef struct pfxh_light {
     float position[3];
     float colour[3];
@@ -89,10 +97,14 @@ typedef struct pfxh_light {
     float radius;
     float range;
     uint32_t shadow;
-    uint32_t kind;       /* PFXH_LIGHT_POINT or PFXH_LIGHT_SPOT */
-    float direction[3];  /* spot axis */
-    float inner_deg;
-    float outer_deg;
+    uint32_t kind;       /* PFXH_LIGHT_* */
+    float direction[3];  /* spot axis, or the disc's normal */
+    float inner_deg;     /* spot */
+    float outer_deg;     /* spot */
+    float half_u[3];     /* rect: half of each side; perpendicular; emits along half_u x half_v */
+    float half_v[3];
+    float disc_radius;   /* disc */
+    uint32_t two_sided;  /* rect, disc: emit from both faces */
 } pfxh_light;
 
 /* gpu::Camera (crates/trace/src/gpu.rs:172-178) */
@@ -238,6 +250,24 @@ typedef struct pfxh_stats {
 
 typedef struct pfxh_trace pfxh_trace;
 
+/* Device limits traces are built against, the counterpart of the wgpu
+ * limits Trace::new checks before it allocates (fits_buffer,
+ * crates/trace/src/gpu.rs:1258-1278): a trace whose allocation would pass
+ * one is refused with PFXH_REFUSED and the message "limit <name> of
+ * <needed> is beyond the device's <has>" (GpuShortfall::Limit,
+ * crates/gpu/src/floor.rs:349-351), name max_buffer_size (one allocation)
+ * or max_memory (everything the trace, or a batch, holds). 0 means the
+ * device's own limit (hipMalloc refuses what HBM cannot hold, also as
+ * PFXH_REFUSED). Process-wide; for callers that share a GPU or want a
+ * smaller budget, and for tests. */
+typedef struct pfxh_device_limits {
+    uint64_t max_buffer_size;
+    uint64_t max_memory;
+} pfxh_device_limits;
+/* NULL restores the device's own limits. */
+void pfxh_set_device_limits(const pfxh_device_limits* limits);
+void pfxh_get_device_limits(pfxh_device_limits* out);
+
 /* Trace::new_detailed (crates/trace/src/gpu.rs:1926-1934) */
 pfxh_status pfxh_trace_new(const pfxh_scene* scene, const pfxh_detail* detail, uint32_t width,
                            uint32_t height, pfxh_trace** out, char* err, size_t err_len);
@@ -270,6 +300,15 @@ pfxh_status pfxh_trace_set_lens(pfxh_trace* trace, float shift_x, float shift_y,
  * sample count */
 pfxh_status pfxh_trace_set_projection(pfxh_trace* trace, uint32_t kind, float width, float height,
                                       char* err, size_t err_len);
+/* Trace::set_sun_radius (gpu.rs:2903-2910): the sun's angular radius in
+ * degrees, 0 <= radius < 90 (0 is a point sun); resets the sample count */
+pfxh_status pfxh_trace_set_sun_radius(pfxh_trace* trace, float radius_deg, char* err, size_t err_len);
+/* Trace::set_environment (gpu.rs:2810-2815): a new lat-long sky (width 0 is
+ * black) and its sampling CDF; resets the sample count. A trace built
+ * under a black sky grows its shadow-ray buffers by one light slot when
+ * given a lit one; a batch still cannot (PFXH_INVALID). */
+pfxh_status pfxh_trace_set_environment(pfxh_trace* trace, const pfxh_environment* sky, char* err,
+                                       size_t err_len);
 /* Trace::samples (gpu.rs:3232-3234) */
 uint32_t pfxh_trace_samples(const pfxh_trace* trace);
 /* Trace::gpu_bytes (gpu.rs:2360) */

@@ -53,6 +53,17 @@ bool light_valid(const pfxh_light& l) {
         if (l.colour[k] < 0.0f) return false;
     if (l.intensity < 0.0f || l.radius < 0.0f || l.range < 0.0f) return false;
     if (l.kind == PFXH_LIGHT_POINT) return true;
+    if (l.kind == PFXH_LIGHT_RECT) {  // lights.rs:128-135
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(l.half_u[k]) || !std::isfinite(l.half_v[k])) return false;
+        float lu = pfx::length(load3(l.half_u)), lv = pfx::length(load3(l.half_v));
+        return lu > 1e-6f && lv > 1e-6f && fabsf(pfx::dot(load3(l.half_u), load3(l.half_v))) <= 1e-4f * lu * lv;
+    }
+    if (l.kind == PFXH_LIGHT_DISC) {  // lights.rs:136-141
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(l.direction[k])) return false;
+        return pfx::length(load3(l.direction)) > 1e-6f && std::isfinite(l.disc_radius) && l.disc_radius > 1e-6f;
+    }
     if (l.kind != PFXH_LIGHT_SPOT) return false;
     for (int k = 0; k < 3; ++k)
         if (!std::isfinite(l.direction[k])) return false;
@@ -85,11 +96,43 @@ float light_cone(const pfxh_light& l, const float toward[3]) {
     float t = pfx::fm_clamp(cd * scale + offset, 0.0f, 1.0f);
     return t * t;
 }
-// lights.rs:183-231 (point, spot)
+// lights.rs:146-152
+float light_area(const pfxh_light& l) {
+    if (l.kind == PFXH_LIGHT_RECT) return 4.0f * pfx::length(pfx::cross(load3(l.half_u), load3(l.half_v)));
+    if (l.kind == PFXH_LIGHT_DISC) return PI * l.disc_radius * l.disc_radius;
+    return 0.0f;
+}
+// lights.rs:183-231
 GLight light_pack(const pfxh_light& l) {
     GLight g;
     memset(&g, 0, sizeof g);
     float shadow = l.shadow ? 1.0f : 0.0f;
+    if (l.kind == PFXH_LIGHT_RECT || l.kind == PFXH_LIGHT_DISC) {  // :207-230
+        float area = light_area(l);
+        float sided = l.two_sided ? 1.0f : 0.0f;
+        for (int k = 0; k < 3; ++k) {
+            g.a[k] = l.position[k];
+            g.b[k] = (l.colour[k] * l.intensity) / area;
+        }
+        g.b[3] = FLT_MAX;
+        g.c[3] = sided;
+        g.d[3] = shadow;
+        if (l.kind == PFXH_LIGHT_RECT) {
+            g.a[3] = 2.0f;
+            for (int k = 0; k < 3; ++k) {
+                g.c[k] = l.half_u[k];
+                g.d[k] = l.half_v[k];
+            }
+        } else {
+            g.a[3] = 3.0f;
+            v3 nrm = rnormalize(load3(l.direction));
+            g.c[0] = nrm.x;
+            g.c[1] = nrm.y;
+            g.c[2] = nrm.z;
+            g.d[0] = l.disc_radius;
+        }
+        return g;
+    }
     for (int k = 0; k < 3; ++k) {
         g.a[k] = l.position[k];
         g.b[k] = l.colour[k] * l.intensity;

@@ -56,6 +56,7 @@ struct Scene {
     Env env;
     SkyCdf sky;
     bool lit = false;
+    bool area_lights = false;  // a looped row is a shadowed rect or disc light (light_hits can see it)
     // Frame (gpu.rs:2295-2316)
     v3 origin, forward, right, up;
     v3 sun_dir, sun_color;
@@ -515,9 +516,75 @@ static Candidate emitter_candidate(const Scene& sc, const m3& T, v3 p, v3 origin
     v4 ray = shadow_ray_to(origin, x, true);
     return Candidate{local, t3(e.e), e.e[3] / e.c[3] * distance2 / cosine, 0.0f, ray.xyz, ray.w};
 }
+// lights.wgsl:57-63: (unit normal, area) of a rect or disc light row
+static v4 area_frame(const GLight& light) {
+    if ((uint32_t)light.a[3] == 3u) return v4{t3(light.c), PI * light.d[0] * light.d[0]};
+    v3 spanned = pfx::cross(t3(light.c), t3(light.d)) * 4.0f;
+    float area = pfx::length(spanned);
+    return v4{spanned / area, area};
+}
+// lights.wgsl:64-91: radiance of the shadowed area lights among the looped
+// rows that the ray (o, d) meets before `limit`, MIS-weighted against light
+// sampling when the ray was BSDF-sampled with pdf previous_pdf
+static v3 light_hits(const Scene& sc, v3 o, v3 d, float limit, v3 previous_point, float previous_pdf) {
+    uint32_t count = sc.lights.looped;
+    v3 total = mk(0, 0, 0);
+    for (uint32_t i = 0; i < count; ++i) {
+        const GLight& light = sc.lights.rows[i];
+        uint32_t kind = (uint32_t)light.a[3];
+        if ((kind != 2u && kind != 3u) || light.d[3] < 0.5f) continue;
+        v4 plane = area_frame(light);
+        float facing = dot(d, plane.xyz);
+        if (fabsf(facing) < 1e-9f || (light.c[3] < 0.5f && facing >= 0.0f)) continue;
+        float t = dot(t3(light.a) - o, plane.xyz) / facing;
+        if (t <= 0.0001f || t >= limit) continue;
+        v3 x = o + d * t;
+        v3 q = x - t3(light.a);
+        if (kind == 3u) {
+            if (dot(q, q) > light.d[0] * light.d[0]) continue;
+        } else if (fabsf(dot(q, t3(light.c))) > dot(t3(light.c), t3(light.c)) ||
+                   fabsf(dot(q, t3(light.d))) > dot(t3(light.d), t3(light.d))) {
+            continue;
+        }
+        float weight = 1.0f;
+        if (previous_pdf > 0.0f) {
+            v3 to = x - previous_point;
+            float distance2 = dot(to, to);
+            float cosine = fabsf(dot(plane.xyz, to)) / sqrtf(fmaxf(distance2, 1e-30f));
+            weight = mis_power(previous_pdf, distance2 / (plane.w * fmaxf(cosine, 1e-7f)));
+        }
+        total = total + t3(light.b) * weight;
+    }
+    return total;
+}
 static Candidate light_candidate(const GLight& light, const m3& T, v3 p, v3 origin, v3 geometric,
-                                 uint32_t* state) {  // :145-200 (point, spot)
+                                 uint32_t* state) {  // :145-200
     bool shadow = light.d[3] > 0.5f;
+    uint32_t kind = (uint32_t)light.a[3];
+    if (kind == 2u || kind == 3u) {  // :149-170: a point on the rect or disc, area pdf in solid angle
+        float u1 = pfx::rnd(state);
+        float u2 = pfx::rnd(state);
+        v4 plane = area_frame(light);
+        v3 x = t3(light.a) + t3(light.c) * (2.0f * u1 - 1.0f) + t3(light.d) * (2.0f * u2 - 1.0f);
+        if (kind == 3u) {
+            m3 disc = pfx::basis(t3(light.c));
+            float r = light.d[0] * sqrtf(u1);
+            float phi = 2.0f * PI * u2;
+            x = t3(light.a) + disc.c0 * (r * fm_cos(phi)) + disc.c1 * (r * fm_sin(phi));
+        }
+        v3 to = x - p;
+        float distance2 = dot(to, to);
+        v3 direction = to / sqrtf(fmaxf(distance2, 1e-30f));
+        v3 local = pfx::tmul(T, direction);
+        if (local.z <= 0.0f) return candidate_none();
+        float facing = -dot(plane.xyz, direction);
+        float cosine = light.c[3] > 0.5f ? fabsf(facing) : facing;
+        if (cosine <= 1e-7f) return candidate_none();
+        float pdf = distance2 / (plane.w * cosine);
+        v4 ray = shadow_ray_to(origin, x, shadow);
+        if (!shadow) return Candidate{local, t3(light.b) / pdf, 0.0f, 0.0f, ray.xyz, ray.w};
+        return Candidate{local, t3(light.b), pdf, 0.0f, ray.xyz, ray.w};
+    }
     v3 to = t3(light.a) - p;
     float distance = pfx::length(to);
     float range = light.b[3];
@@ -834,7 +901,7 @@ static Result radiance(const Scene& sc, v3 o0, v3 d0, uint32_t* state, uint64_t*
         Hit hit = scene_hit(sc, o, d, 1e20f, false);
         *closest += 1;
         v3 origin_point = pfx::spawn_point(o, -d, o, 0.0f);
-        (void)origin_point;
+        if (sc.area_lights) result = result + throughput * light_hits(sc, o, d, hit.t, origin_point, previous_pdf);
         if (hit.t >= 1e19f) {
             float pdf = sky_pdf(sc, d);
             float weight = (bounce > 0u && previous_pdf > 0.0f) ? previous_pdf / (previous_pdf + pdf) : 1.0f;
@@ -1182,6 +1249,10 @@ static std::string prepare(Scene& sc, const pfxh_scene& in, const pfxh_detail& d
     sc.lights = tree_lay(in.lights, in.light_count);
     sc.emitters = build_emitters(cands, sc.lights.looped, sc.lights.tree);
     sc.lit = !cands.empty() || in.light_count > 0;
+    for (uint32_t i = 0; i < sc.lights.looped; ++i) {
+        uint32_t kind = (uint32_t)sc.lights.rows[i].a[3];
+        if ((kind == 2u || kind == 3u) && sc.lights.rows[i].d[3] >= 0.5f) sc.area_lights = true;
+    }
     sc.env.width = in.sky.width;
     sc.env.height = in.sky.height;
     if (in.sky.width > 0 && in.sky.height > 0)
@@ -1247,6 +1318,22 @@ void or_set_camera(void* h, const pfxh_camera* c) {
     sc->forward = t3(c->forward);
     sc->right = t3(c->right);
     sc->up = t3(c->up);
+    sc->samples = 0;
+}
+// Trace::set_sun_radius (gpu.rs:2903-2910), Trace::set_environment (:2810-2815)
+void or_set_sun_radius(void* h, float radius_deg) {
+    Scene* sc = (Scene*)h;
+    sc->sun_w = fm_cos(radius_deg * (3.14159265358979f / 180.0f));
+    sc->samples = 0;
+}
+void or_set_environment(void* h, const pfxh_environment* sky) {
+    Scene* sc = (Scene*)h;
+    sc->env.width = sky->width;
+    sc->env.height = sky->height;
+    sc->env.texels.clear();
+    if (sky->width > 0 && sky->height > 0)
+        sc->env.texels.assign(sky->texels, sky->texels + (size_t)sky->width * sky->height * 4);
+    sc->sky = sky_build(sc->env, 256, 128);
     sc->samples = 0;
 }
 int or_set_window(void* h, uint32_t fw, uint32_t fh, uint32_t x, uint32_t y) {

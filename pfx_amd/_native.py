@@ -44,6 +44,10 @@ LIGHT = np.dtype(
         ("direction", "<f4", (3,)),
         ("inner_deg", "<f4"),
         ("outer_deg", "<f4"),
+        ("half_u", "<f4", 3),
+        ("half_v", "<f4", 3),
+        ("disc_radius", "<f4"),
+        ("two_sided", "<u4"),
     ]
 )
 PROBE_HIT = np.dtype(
@@ -60,9 +64,9 @@ BVH_NODE = np.dtype(
     [("lo", "<f4", (4,)), ("hi", "<f4", (4,)), ("first", "<u4"), ("count", "<u4"), ("pad", "<u4", (2,))]
 )
 assert TRIANGLE.itemsize == 40 and TRIANGLE_DETAIL.itemsize == 76 and INSTANCE.itemsize == 72
-assert MATERIAL.itemsize == 52 and LIGHT.itemsize == 64 and PROBE_HIT.itemsize == 32 and BVH_NODE.itemsize == 48
+assert MATERIAL.itemsize == 52 and LIGHT.itemsize == 96 and PROBE_HIT.itemsize == 32 and BVH_NODE.itemsize == 48
 
-LIGHT_POINT, LIGHT_SPOT = 0, 1
+LIGHT_POINT, LIGHT_SPOT, LIGHT_RECT, LIGHT_DISC = 0, 1, 2, 3
 ACCUM_COMPENSATED, ACCUM_PLAIN, ACCUM_FP64 = 0, 1, 2
 PROJ_PERSPECTIVE, PROJ_ORTHOGRAPHIC, PROJ_EQUIRECT = 0, 1, 2
 
@@ -133,6 +137,10 @@ class DetailC(C.Structure):
         ("triangle_test", C.c_uint32),
         ("walk_order", C.c_uint32),
     ]
+
+
+class DeviceLimitsC(C.Structure):
+    _fields_ = [("max_buffer_size", C.c_uint64), ("max_memory", C.c_uint64)]
 
 
 class AdaptiveC(C.Structure):
@@ -234,6 +242,10 @@ SIGNATURES = {
     "pfxh_trace_set_sun": (C.c_int, [_P, _P, _S, _Z]),
     "pfxh_trace_set_lens": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_float, _S, _Z]),
     "pfxh_trace_set_projection": (C.c_int, [_P, C.c_uint32, C.c_float, C.c_float, _S, _Z]),
+    "pfxh_trace_set_sun_radius": (C.c_int, [_P, C.c_float, _S, _Z]),
+    "pfxh_set_device_limits": (None, [_P]),
+    "pfxh_get_device_limits": (None, [_P]),
+    "pfxh_trace_set_environment": (C.c_int, [_P, _P, _S, _Z]),
     "pfxh_checkpoint_write": (C.c_int, [_P, _S, _P, _Z, C.POINTER(C.c_size_t), _S, _Z]),
     "pfxh_checkpoint_read": (C.c_int, [_P, _Z, _S, _P, _P, _P, _U, _S, _Z]),
     "pfxh_trace_save": (C.c_int, [_P, _S, _P, _Z, C.POINTER(C.c_size_t), _S, _Z]),

@@ -7,7 +7,7 @@
 //               [--bounces B] [--seed S] [--denoise none|repo|atrous|learned]
 //               [--weights FILE] [--out PREFIX]
 //
-// SCENE is a PFXSCN01 file (pfx_amd.scene.save_scene). The repo denoiser
+// SCENE is a PFXSCN02 file (pfx_amd.scene.save_scene). The repo denoiser
 // reads OIDN's TZA weights and the learned one a PFXLNET1 file; both default
 // to the files beside the library (weights/).
 #include <stdint.h>
@@ -37,14 +37,14 @@ struct SceneFile {
     pfxh_scene scene{};
 };
 
-// PFXSCN01: magic, 9 u32 counts, camera (12 f32), sun (7 f32), the arrays
+// PFXSCN02: magic, 9 u32 counts, camera (12 f32), sun (7 f32), the arrays
 // as the pfxh_* structs lay them out, the sky texels.
 SceneFile load_scene(const std::string& path) {
     SceneFile s;
     s.bytes = read_file(path);
     const uint8_t* p = s.bytes.data();
     size_t len = s.bytes.size(), at = 120;
-    if (len < 120 || memcmp(p, "PFXSCN01", 8) != 0) throw std::runtime_error(path + ": not a PFXSCN01 scene");
+    if (len < 120 || memcmp(p, "PFXSCN02", 8) != 0) throw std::runtime_error(path + ": not a PFXSCN02 scene");
     uint32_t n[9];
     memcpy(n, p + 8, sizeof n);
     float cam[12], sun[7];

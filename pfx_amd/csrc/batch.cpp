@@ -45,8 +45,11 @@ struct pfxh_batch {
     uint64_t host_stats[2] = {0, 0};
     uint64_t paths = 0;
 
+    uint64_t held = 0;  // bytes of the batch's own allocations
     void* alloc(size_t bytes, const char* what) {
         void* p = nullptr;
+        std::string short_ = limit_shortfall(std::max<size_t>(bytes, 16), held);
+        if (!short_.empty()) throw DeviceError{PFXH_REFUSED, short_ + " (the batch's " + what + ")"};
         hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 16));
         if (e != hipSuccess) {
             (void)hipGetLastError();
@@ -54,6 +57,7 @@ struct pfxh_batch {
                                                 " bytes on the GPU: " + hipGetErrorString(e)};
         }
         allocations.push_back(p);
+        held += std::max<size_t>(bytes, 16);
         return p;
     }
     BatchDev dev() const { return BatchDev{scenes, accums, waves, wave}; }
@@ -162,6 +166,10 @@ void render(pfxh_batch* b, const std::vector<uint32_t>& count, const pfxh_batch_
     hipStream_t s = b->stream;
     for (pfxh_trace* st : b->stills)  // trace streams may hold work on a still's buffers
         HIP_TRY(hipStreamSynchronize(st->stream));
+    // the scene table again: a lent trace's setters (camera, sky, sun) change its SceneDev
+    std::vector<SceneDev> rows;
+    for (pfxh_trace* st : b->stills) rows.push_back(st->S);
+    HIP_TRY(hipMemcpy(b->scenes, rows.data(), rows.size() * sizeof(SceneDev), hipMemcpyHostToDevice));
     for (uint32_t i = 0; i < b->stills.size(); ++i) {
         pfxh_trace* st = b->stills[i];
         if (st->samples == 0 && count[i] > 0) {

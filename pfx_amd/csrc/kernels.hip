@@ -807,10 +807,85 @@ DEV Cand emitter_candidate(const SceneDev& S, const m3& T, v3 p, v3 origin, uint
     r.roughness = 0.0f;
     return r;
 }
+// lights.wgsl:57-63: (unit normal, area) of a rect or disc light row
+DEV float4 area_frame(float4 la, float4 lc, float4 ld) {
+    if ((uint32_t)la.w == 3u) return f4(f3(lc), PI * ld.x * ld.x);
+    v3 spanned = pfx::cross(f3(lc), f3(ld)) * 4.0f;
+    float area = pfx::length(spanned);
+    return f4(spanned / area, area);
+}
+// lights.wgsl:64-91: radiance of the shadowed rect and disc lights among the
+// looped rows that the ray (o, d) meets before `limit`, MIS-weighted
+// against light sampling when the ray was BSDF-sampled with previous_pdf.
+DEV v3 light_hits(const SceneDev& S, v3 o, v3 d, float limit, v3 previous_point, float previous_pdf) {
+    v3 total = mk(0, 0, 0);
+    for (uint32_t i = 0; i < S.looped; ++i) {
+        float4 la = row(S.lights, i, 0), lb = row(S.lights, i, 1), lc = row(S.lights, i, 2),
+               ld = row(S.lights, i, 3);
+        uint32_t kind = (uint32_t)la.w;
+        if ((kind != 2u && kind != 3u) || ld.w < 0.5f) continue;
+        float4 plane = area_frame(la, lc, ld);
+        v3 pn = f3(plane);
+        float facing = dot(d, pn);
+        if (fabsf(facing) < 1e-9f || (lc.w < 0.5f && facing >= 0.0f)) continue;
+        float t = dot(f3(la) - o, pn) / facing;
+        if (t <= 0.0001f || t >= limit) continue;
+        v3 x = o + d * t;
+        v3 q = x - f3(la);
+        if (kind == 3u) {
+            if (dot(q, q) > ld.x * ld.x) continue;
+        } else if (fabsf(dot(q, f3(lc))) > dot(f3(lc), f3(lc)) || fabsf(dot(q, f3(ld))) > dot(f3(ld), f3(ld))) {
+            continue;
+        }
+        float weight = 1.0f;
+        if (previous_pdf > 0.0f) {
+            v3 to = x - previous_point;
+            float distance2 = dot(to, to);
+            float cosine = fabsf(dot(pn, to)) / sqrtf(fmaxf(distance2, 1e-30f));
+            weight = mis_power(previous_pdf, distance2 / (plane.w * fmaxf(cosine, 1e-7f)));
+        }
+        total = total + f3(lb) * weight;
+    }
+    return total;
+}
 DEV Cand light_candidate(const SceneDev& S, uint32_t li, const m3& T, v3 p, v3 origin, v3 geometric,
-                         uint32_t* state) {  // :145-200 (point, spot)
+                         uint32_t* state) {  // :145-200
     float4 la = row(S.lights, li, 0), lb = row(S.lights, li, 1), lc = row(S.lights, li, 2), ld = row(S.lights, li, 3);
     bool shadow = ld.w > 0.5f;
+    uint32_t kind = (uint32_t)la.w;
+    if (kind == 2u || kind == 3u) {  // :149-170: a point on the rect or disc, its area pdf in solid angle
+        float u1 = pfx::rnd(state);
+        float u2 = pfx::rnd(state);
+        float4 plane = area_frame(la, lc, ld);
+        v3 x = f3(la) + f3(lc) * (2.0f * u1 - 1.0f) + f3(ld) * (2.0f * u2 - 1.0f);
+        if (kind == 3u) {
+            m3 disc = basis(f3(lc));
+            float r = ld.x * sqrtf(u1);
+            float phi = 2.0f * PI * u2;
+            x = f3(la) + disc.c0 * (r * fm_cos(phi)) + disc.c1 * (r * fm_sin(phi));
+        }
+        v3 to = x - p;
+        float distance2 = dot(to, to);
+        v3 direction = to / sqrtf(fmaxf(distance2, 1e-30f));
+        v3 local = pfx::tmul(T, direction);
+        if (local.z <= 0.0f) return none();
+        float facing = -dot(f3(plane), direction);
+        float cosine = lc.w > 0.5f ? fabsf(facing) : facing;
+        if (cosine <= 1e-7f) return none();
+        float pdf = distance2 / (plane.w * cosine);
+        Cand r;
+        shadow_to(origin, x, shadow, r.direction, r.limit);
+        r.wi = local;
+        r.roughness = 0.0f;
+        if (!shadow) {
+            r.scale = f3(lb) / pdf;
+            r.pdf = 0.0f;
+        } else {
+            r.scale = f3(lb);
+            r.pdf = pdf;
+        }
+        return r;
+    }
     v3 to = f3(la) - p;
     float distance = pfx::length(to);
     float range = lb.w;
@@ -1283,6 +1358,7 @@ DEV void shade_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t bo
     uint32_t state = fbits(t4.w);
     float t = hv.x;
     bool done = false;
+    if (S.area) result = result + throughput * light_hits(S, o, d, t, pfx::spawn_point(o, -d, o, 0.0f), previous_pdf);
     if (t >= 1e19f) {
         if (!S.env_black) {
             float pdf = sky_pdf(S, d);
@@ -1551,7 +1627,10 @@ __global__ __launch_bounds__(256) void k_hits_batch(BatchDev B, WaveDev W, const
     uint32_t n = W.counters[C_N];
     uint32_t chunk = (n + gridDim.x - 1u) / gridDim.x;
     uint32_t lo = min(n, blockIdx.x * chunk), hi = min(n, lo + chunk);
-    auto keep = [&](uint32_t i) { return !(W.hit[i].x >= 1e19f) || !B.scenes[W.sid[i]].env_black; };
+    auto keep = [&](uint32_t i) {
+        const SceneDev& S = B.scenes[W.sid[i]];
+        return !(W.hit[i].x >= 1e19f) || !S.env_black || S.area;
+    };
     compact_chunk(lo, hi, queue, keep, W.counters + C_HITS, hitq);
 }
 

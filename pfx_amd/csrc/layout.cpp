@@ -594,9 +594,41 @@ void spot_terms(float inner_deg, float outer_deg, float* scale, float* offset) {
     *scale = 1.0f / pfx::maxf(inner - outer, 0.001f);
     *offset = -outer * *scale;
 }
+// LocalLight::area (lights.rs:146-152)
+float light_area(const pfxh_light& l) {
+    if (l.kind == PFXH_LIGHT_RECT) return 4.0f * pfx::length(pfx::cross(ld3(l.half_u), ld3(l.half_v)));
+    if (l.kind == PFXH_LIGHT_DISC) return 3.14159265358979f * l.disc_radius * l.disc_radius;
+    return 0.0f;
+}
 // lights.rs:183-231, 16 floats: a, b, c, d
 void pack_light(const pfxh_light& l, float* g) {
     memset(g, 0, 64);
+    if (l.kind == PFXH_LIGHT_RECT || l.kind == PFXH_LIGHT_DISC) {
+        // a = centre + kind, b = power / area + FLT_MAX, c = half_u or the
+        // normal + two-sidedness, d = half_v or the radius + shadow
+        float area = light_area(l);
+        for (int k = 0; k < 3; ++k) {
+            g[k] = l.position[k];
+            g[4 + k] = (l.colour[k] * l.intensity) / area;
+        }
+        g[3] = l.kind == PFXH_LIGHT_RECT ? 2.0f : 3.0f;
+        g[7] = FLT_MAX;
+        g[11] = l.two_sided ? 1.0f : 0.0f;
+        g[15] = l.shadow ? 1.0f : 0.0f;
+        if (l.kind == PFXH_LIGHT_RECT) {
+            for (int k = 0; k < 3; ++k) {
+                g[8 + k] = l.half_u[k];
+                g[12 + k] = l.half_v[k];
+            }
+        } else {
+            v3 nrm = rnormalize(ld3(l.direction));
+            g[8] = nrm.x;
+            g[9] = nrm.y;
+            g[10] = nrm.z;
+            g[12] = l.disc_radius;
+        }
+        return;
+    }
     for (int k = 0; k < 3; ++k) {
         g[k] = l.position[k];
         g[4 + k] = l.colour[k] * l.intensity;
@@ -1127,6 +1159,14 @@ bool light_valid(const pfxh_light& l) {  // lights.rs:101-142
         if (l.colour[k] < 0.0f) return false;
     if (l.intensity < 0.0f || l.radius < 0.0f || l.range < 0.0f) return false;
     if (l.kind == PFXH_LIGHT_POINT) return true;
+    if (l.kind == PFXH_LIGHT_RECT) {
+        if (!finite_n(l.half_u, 3) || !finite_n(l.half_v, 3)) return false;
+        float lu = pfx::length(ld3(l.half_u)), lv = pfx::length(ld3(l.half_v));
+        return lu > 1e-6f && lv > 1e-6f && fabsf(pfx::dot(ld3(l.half_u), ld3(l.half_v))) <= 1e-4f * lu * lv;
+    }
+    if (l.kind == PFXH_LIGHT_DISC)
+        return finite_n(l.direction, 3) && pfx::length(ld3(l.direction)) > 1e-6f && std::isfinite(l.disc_radius) &&
+               l.disc_radius > 1e-6f;
     if (l.kind != PFXH_LIGHT_SPOT) return false;
     if (!finite_n(l.direction, 3)) return false;
     return pfx::length(ld3(l.direction)) > 1e-6f && std::isfinite(l.inner_deg) && std::isfinite(l.outer_deg) &&
@@ -1247,6 +1287,54 @@ void build_emitters(const std::vector<float>& tris12, const std::vector<uint32_t
     if (n) cdf[n - 1] = 1.0f;
 }
 
+// The environment texture and its sampling CDF, 256 x 128 (sky.rs:207-272,
+// gpu.rs:1795); also what Trace::set_environment rebuilds (gpu.rs:2810-2815).
+std::string lay_sky(const pfxh_environment& sky, HostScene& out) {
+    out.env_w = sky.width;
+    out.env_h = sky.height;
+    out.env.clear();
+    out.env_black = true;
+    if (out.env_w) {
+        out.env.assign(sky.texels, sky.texels + (size_t)out.env_w * out.env_h * 4);
+        for (size_t i = 0; i < out.env.size(); ++i) {
+            if (!std::isfinite(out.env[i])) return "the sky has non-finite texels";
+            if ((i & 3) != 3 && out.env[i] != 0.0f) out.env_black = false;
+        }
+    } else {
+        out.env.assign(4, 0.0f);
+    }
+    const uint32_t CW = 256, CH = 128;
+    out.sky_rows.assign(CH, 0.0f);
+    out.sky_columns.assign((size_t)CW * CH, 0.0f);
+    double total = 0.0;
+    for (uint32_t row = 0; row < CH; ++row) {
+        float theta = PI_F * ((float)row + 0.5f) / (float)CH;
+        float sine = fm_sin(theta);
+        double row_total = 0.0;
+        for (uint32_t col = 0; col < CW; ++col) {
+            float phi = TAU_F * ((float)col + 0.5f) / (float)CW;
+            v3 d = pfx::mk(fm_sin(theta) * fm_cos(phi), fm_cos(theta), fm_sin(theta) * fm_sin(phi));
+            v3 c = env_lookup(out.env, out.env_w, out.env_h, d);
+            float c3[3] = {c.x, c.y, c.z};
+            row_total += (double)pfx::maxf(lum(c3), 0.0f) * (double)sine;
+            out.sky_columns[(size_t)row * CW + col] = (float)row_total;
+        }
+        if (row_total > 0.0) {
+            for (uint32_t col = 0; col < CW; ++col) out.sky_columns[(size_t)row * CW + col] /= (float)row_total;
+        } else {
+            for (uint32_t col = 0; col < CW; ++col) out.sky_columns[(size_t)row * CW + col] = (float)(col + 1) / (float)CW;
+        }
+        total += row_total;
+        out.sky_rows[row] = (float)total;
+    }
+    if (total > 0.0) {
+        for (auto& v : out.sky_rows) v /= (float)total;
+    } else {
+        for (uint32_t i = 0; i < CH; ++i) out.sky_rows[i] = (float)(i + 1) / (float)CH;
+    }
+    return "";
+}
+
 std::string build_scene(const pfxh_scene& sc, HostScene& out, bool emitter_sampling, bool watertight) {
     if (sc.material_count == 0) return "invalid trace scene detail";
     for (uint32_t i = 0; i < sc.light_count; ++i)
@@ -1329,49 +1417,8 @@ std::string build_scene(const pfxh_scene& sc, HostScene& out, bool emitter_sampl
     out.lit = out.emitter_count > 0 || sc.light_count > 0;
     if (out.lights.empty()) out.lights.assign(16, 0.0f);
     if (out.emitter_cdf.empty()) out.emitter_cdf.assign(1, 1.0f);
-    // sky CDF, 256 x 128 (sky.rs:207-272, gpu.rs:1795)
-    out.env_w = sc.sky.width;
-    out.env_h = sc.sky.height;
-    out.env.clear();
-    out.env_black = true;
-    if (out.env_w) {
-        out.env.assign(sc.sky.texels, sc.sky.texels + (size_t)out.env_w * out.env_h * 4);
-        for (size_t i = 0; i < out.env.size(); ++i) {
-            if (!std::isfinite(out.env[i])) return "the sky has non-finite texels";
-            if ((i & 3) != 3 && out.env[i] != 0.0f) out.env_black = false;
-        }
-    } else {
-        out.env.assign(4, 0.0f);
-    }
-    const uint32_t CW = 256, CH = 128;
-    out.sky_rows.assign(CH, 0.0f);
-    out.sky_columns.assign((size_t)CW * CH, 0.0f);
-    double total = 0.0;
-    for (uint32_t row = 0; row < CH; ++row) {
-        float theta = PI_F * ((float)row + 0.5f) / (float)CH;
-        float sine = fm_sin(theta);
-        double row_total = 0.0;
-        for (uint32_t col = 0; col < CW; ++col) {
-            float phi = TAU_F * ((float)col + 0.5f) / (float)CW;
-            v3 d = pfx::mk(fm_sin(theta) * fm_cos(phi), fm_cos(theta), fm_sin(theta) * fm_sin(phi));
-            v3 c = env_lookup(out.env, out.env_w, out.env_h, d);
-            float c3[3] = {c.x, c.y, c.z};
-            row_total += (double)pfx::maxf(lum(c3), 0.0f) * (double)sine;
-            out.sky_columns[(size_t)row * CW + col] = (float)row_total;
-        }
-        if (row_total > 0.0) {
-            for (uint32_t col = 0; col < CW; ++col) out.sky_columns[(size_t)row * CW + col] /= (float)row_total;
-        } else {
-            for (uint32_t col = 0; col < CW; ++col) out.sky_columns[(size_t)row * CW + col] = (float)(col + 1) / (float)CW;
-        }
-        total += row_total;
-        out.sky_rows[row] = (float)total;
-    }
-    if (total > 0.0) {
-        for (auto& v : out.sky_rows) v /= (float)total;
-    } else {
-        for (uint32_t i = 0; i < CH; ++i) out.sky_rows[i] = (float)(i + 1) / (float)CH;
-    }
+    std::string e_sky = lay_sky(sc.sky, out);
+    if (!e_sky.empty()) return e_sky;
     // device trees: world BLAS, mesh BLASes, then the top level
     out.nodes.clear();
     Collapse conv(out.nodes);

@@ -61,6 +61,9 @@ struct HostScene {
 std::string build_scene(const pfxh_scene& scene, HostScene& out, bool emitter_sampling = true,
                         bool watertight = false);
 std::string lay_levels(const pfxh_scene& scene, RefLevels& out, std::vector<uint32_t>* slot_prim);
+// The environment texels, black flag and sky sampling CDF of `out` from
+// `sky` (a scene's sky, or a Trace::set_environment argument).
+std::string lay_sky(const pfxh_environment& sky, HostScene& out);
 
 // tree::lay (crates/trace/src/tree.rs:220-243)
 // Rust's slice::select_nth_unstable_by as the light tree uses it, over

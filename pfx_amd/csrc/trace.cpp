@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -104,6 +105,11 @@ void trace_setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& deta
     S.env_black = h.env_black ? 1u : 0u;
     S.lit = h.lit ? 1u : 0u;
     S.looped = h.looped;
+    S.area = 0u;
+    for (uint32_t i = 0; i < h.looped; ++i) {
+        uint32_t kind = (uint32_t)h.lights[16 * i + 3];
+        if ((kind == 2u || kind == 3u) && h.lights[16 * i + 15] >= 0.5f) S.area = 1u;
+    }
     S.tree = h.tree;
     S.emitter_count = h.emitter_count;
     S.light_slots = h.lit ? 1u + h.looped + (h.tree > 0 ? 1u : 0u) : 0u;
@@ -246,7 +252,7 @@ void sample(pfxh_trace* t, uint32_t count, uint32_t seed) {
             // under a black sky only the paths that hit need shading
             const uint32_t* shade_queue = queue;
             const uint32_t* shade_count = nullptr;
-            if (t->S.env_black && t->qh) {
+            if (t->S.env_black && !t->S.area && t->qh) {
                 launch_zero(s, W.counters + C_HITS, 1);
                 launch_hits(s, W, queue, n, t->qh);
                 shade_queue = t->qh;
@@ -321,7 +327,23 @@ pfxh_status trace_check(const pfxh_scene* scene, const pfxh_detail* detail, uint
     return PFXH_OK;
 }
 
+namespace {
+std::atomic<uint64_t> g_max_buffer{0}, g_max_memory{0};
+}
+uint64_t limit_max_buffer() { return g_max_buffer.load(); }
+uint64_t limit_max_memory() { return g_max_memory.load(); }
+
 extern "C" {
+
+void pfxh_set_device_limits(const pfxh_device_limits* limits) {
+    g_max_buffer.store(limits ? limits->max_buffer_size : 0);
+    g_max_memory.store(limits ? limits->max_memory : 0);
+}
+void pfxh_get_device_limits(pfxh_device_limits* out) {
+    if (!out) return;
+    out->max_buffer_size = g_max_buffer.load();
+    out->max_memory = g_max_memory.load();
+}
 
 const char* pfxh_version(void) { return "pfxhip 0.1.0 (gfx950 wavefront tracer)"; }
 
@@ -482,6 +504,75 @@ pfxh_status pfxh_trace_set_sun(pfxh_trace* t, const pfxh_sun* sun, char* err, si
     t->samples = 0;
     t->probe_fresh = false;
     return PFXH_OK;
+}
+
+pfxh_status pfxh_trace_set_sun_radius(pfxh_trace* t, float radius_deg, char* err, size_t err_len) {
+    if (!t) return PFXH_INVALID;
+    if (!std::isfinite(radius_deg) || !(radius_deg >= 0.0f && radius_deg < 90.0f)) {
+        put_err(err, err_len, "invalid sun radius");
+        return PFXH_INVALID;
+    }
+    t->S.sun_w = pfx::fm_cos(radius_deg * (3.14159265358979f / 180.0f));
+    t->samples = 0;
+    return PFXH_OK;
+}
+
+pfxh_status pfxh_trace_set_environment(pfxh_trace* t, const pfxh_environment* sky, char* err, size_t err_len) {
+    if (!t || !sky) return PFXH_INVALID;
+    if (sky->width && (!sky->height || !sky->texels)) {
+        put_err(err, err_len, "invalid environment");
+        return PFXH_INVALID;
+    }
+    HostScene next;
+    std::string e = lay_sky(*sky, next);
+    if (!e.empty()) {
+        put_err(err, err_len, e);
+        return PFXH_INVALID;
+    }
+    const HostScene& h = t->host;
+    uint32_t needed = (t->S.sun_intensity > 0.0f ? 1u : 0u) + (next.env_black ? 0u : 1u) +
+                      (h.lit ? ((h.emitter_count > 0 ? 1u : 0u) + h.looped + (h.tree > 0 ? 1u : 0u)) : 0u);
+    if (needed > t->S.entries && (t->wave_shared || needed > 15u)) {
+        put_err(err, err_len, "a batch still built under a black sky cannot take a lit one");
+        return PFXH_INVALID;
+    }
+    return guarded(err, err_len, [&] {
+        HIP_TRY(hipStreamSynchronize(t->stream));
+        SceneDev& S = t->S;
+        if (needed > S.entries && t->W.capacity) {
+            // one more light slot per path vertex: the wave's entry buffers grow
+            size_t cap = t->W.capacity, old = std::max<uint32_t>(S.entries, 1u);
+            WaveDev& W = t->W;
+            t->release(W.e_o, cap * old * 16);
+            t->release(W.e_d, cap * old * 16);
+            t->release(W.e_c, cap * old * 16);
+            t->release(W.q_sray, cap * old * 4);
+            W.e_o = W.e_d = W.e_c = nullptr;
+            W.q_sray = nullptr;
+            W.e_o = (float4*)t->alloc(cap * needed * 16, "shadow ray origins");
+            W.e_d = (float4*)t->alloc(cap * needed * 16, "shadow ray directions");
+            W.e_c = (float4*)t->alloc(cap * needed * 16, "shadow ray contributions");
+            W.q_sray = (uint32_t*)t->alloc(cap * needed * 4, "shadow ray queue");
+        }
+        if (needed > S.entries) S.entries = needed;
+        t->release(S.sky_rows, t->host.sky_rows.size() * 4);
+        t->release(S.sky_columns, t->host.sky_columns.size() * 4);
+        t->release(S.env, t->host.env.size() * 4);
+        S.sky_rows = S.sky_columns = S.env = nullptr;
+        t->host.env = std::move(next.env);
+        t->host.env_w = next.env_w;
+        t->host.env_h = next.env_h;
+        t->host.env_black = next.env_black;
+        t->host.sky_rows = std::move(next.sky_rows);
+        t->host.sky_columns = std::move(next.sky_columns);
+        S.sky_rows = t->upload(t->host.sky_rows, "sky rows");
+        S.sky_columns = t->upload(t->host.sky_columns, "sky columns");
+        S.env = t->upload(t->host.env, "sky texels");
+        S.env_w = t->host.env_w;
+        S.env_h = t->host.env_h;
+        S.env_black = t->host.env_black ? 1u : 0u;
+        t->samples = 0;
+    });
 }
 
 pfxh_status pfxh_trace_set_lens(pfxh_trace* t, float sx, float sy, float aperture, float focus, char* err,

@@ -32,6 +32,24 @@ struct DeviceError {
         if (e_ != hipSuccess) throw DeviceError{PFXH_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
     } while (0)
 
+// Process-wide device limits (pfxh_set_device_limits); 0 means the
+// device's own.
+uint64_t limit_max_buffer();
+uint64_t limit_max_memory();
+// The refusal for one allocation of `bytes` on top of `held` bytes already
+// held, or an empty string when it fits the limits. Worded as the
+// reference's GpuShortfall::Limit (crates/gpu/src/floor.rs:349-351).
+inline std::string limit_shortfall(uint64_t bytes, uint64_t held) {
+    uint64_t buffer = limit_max_buffer(), memory = limit_max_memory();
+    if (buffer && bytes > buffer)
+        return "limit max_buffer_size of " + std::to_string(bytes) + " is beyond the device's " +
+               std::to_string(buffer);
+    if (memory && held + bytes > memory)
+        return "limit max_memory of " + std::to_string(held + bytes) + " is beyond the device's " +
+               std::to_string(memory);
+    return "";
+}
+
 struct pfxh_trace {
     HostScene host;
     hipStream_t stream = nullptr;
@@ -63,6 +81,8 @@ struct pfxh_trace {
     void* alloc(size_t bytes, const char* what) {
         void* p = nullptr;
         if (bytes == 0) bytes = 16;
+        std::string short_ = limit_shortfall(bytes, gpu_bytes);
+        if (!short_.empty()) throw DeviceError{PFXH_REFUSED, short_ + " (the tracer's " + what + ")"};
         hipError_t e = hipMalloc(&p, bytes);
         if (e != hipSuccess) {
             (void)hipGetLastError();
@@ -72,6 +92,14 @@ struct pfxh_trace {
         allocations.push_back(p);
         gpu_bytes += bytes;
         return p;
+    }
+    // Free one allocation made by alloc (after the stream has drained).
+    void release(const void* p, size_t bytes) {
+        auto it = std::find(allocations.begin(), allocations.end(), p);
+        if (it == allocations.end()) return;
+        (void)hipFree(*it);
+        allocations.erase(it);
+        gpu_bytes -= std::min<uint64_t>(gpu_bytes, bytes == 0 ? 16 : bytes);
     }
     template <class T>
     T* upload(const std::vector<T>& v, const char* what) {

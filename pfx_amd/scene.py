@@ -191,11 +191,11 @@ class Scene:
         return s, keep
 
 
-SCENE_MAGIC = b"PFXSCN01"
+SCENE_MAGIC = b"PFXSCN02"
 
 
 def save_scene(scene: "Scene", path) -> None:
-    """A scene file for non-Python callers (pfxh-render): b"PFXSCN01", nine
+    """A scene file for non-Python callers (pfxh-render): b"PFXSCN02", nine
     little-endian u32 counts (triangles, world details, mesh triangles,
     meshes, instances, materials, lights, sky width, sky height), the camera
     (12 f32) and sun (7 f32), then the arrays in that order as the C
@@ -219,7 +219,7 @@ def save_scene(scene: "Scene", path) -> None:
 def load_scene(path) -> "Scene":
     data = memoryview(open(path, "rb").read())
     if bytes(data[:8]) != SCENE_MAGIC:
-        raise ValueError(f"{path}: not a PFXSCN01 scene file")
+        raise ValueError(f"{path}: not a PFXSCN02 scene file")
     counts = np.frombuffer(data[8:44], "<u4")
     cam = np.frombuffer(data[44:92], "<f4")
     sun = np.frombuffer(data[92:120], "<f4")
@@ -277,4 +277,29 @@ def spot_light(position, colour, intensity, range_, direction, inner_deg, outer_
     l["direction"] = direction
     l["inner_deg"] = inner_deg
     l["outer_deg"] = outer_deg
+    return l
+
+
+F32_MAX = float(np.finfo(np.float32).max)
+
+
+def rect_light(centre, half_u, half_v, colour, intensity, two_sided=False, shadow=True) -> np.ndarray:
+    """LocalLight::rect (crates/trace/src/lights.rs:55-76): a rectangle of
+    sides 2*half_u and 2*half_v (perpendicular) emitting `colour *
+    intensity` watts spread over its area, along half_u x half_v."""
+    l = point_light(centre, colour, intensity, F32_MAX, 0.0, shadow)
+    l["kind"] = N.LIGHT_RECT
+    l["half_u"] = half_u
+    l["half_v"] = half_v
+    l["two_sided"] = 1 if two_sided else 0
+    return l
+
+
+def disc_light(centre, normal, radius, colour, intensity, two_sided=False, shadow=True) -> np.ndarray:
+    """LocalLight::disc (crates/trace/src/lights.rs:78-99)."""
+    l = point_light(centre, colour, intensity, F32_MAX, 0.0, shadow)
+    l["kind"] = N.LIGHT_DISC
+    l["direction"] = normal
+    l["disc_radius"] = radius
+    l["two_sided"] = 1 if two_sided else 0
     return l

@@ -155,6 +155,26 @@ class Trace:
         s = sun.to_c()
         self._check(N.lib().pfxh_trace_set_sun(self._handle(), C.byref(s), err, len(err)), err)
 
+    def set_sun_radius(self, radius_deg: float) -> None:
+        """Trace::set_sun_radius (gpu.rs:2903-2910); resets the sample count."""
+        err = N.err_buffer()
+        self._check(N.lib().pfxh_trace_set_sun_radius(self._handle(), float(radius_deg), err, len(err)), err)
+
+    def set_environment(self, sky) -> None:
+        """Trace::set_environment (gpu.rs:2810-2815): an (H, W, 4) float32
+        lat-long sky, or None for black; resets the sample count."""
+        err = N.err_buffer()
+        if sky is None or not np.asarray(sky).size:
+            env = N.Environment(0, 0, None)
+            keep = None
+        else:
+            keep = np.ascontiguousarray(sky, np.float32)
+            if keep.ndim != 3 or keep.shape[2] != 4:
+                raise ValueError(f"a sky is an (H, W, 4) array, not {keep.shape}")
+            env = N.Environment(keep.shape[1], keep.shape[0], keep.ctypes.data)
+        self._check(N.lib().pfxh_trace_set_environment(self._handle(), C.byref(env), err, len(err)), err)
+        del keep
+
     def set_lens(self, shift=(0.0, 0.0), aperture: float = 0.0, focus_distance: float = 0.0) -> None:
         err = N.err_buffer()
         st = N.lib().pfxh_trace_set_lens(self._handle(), shift[0], shift[1], aperture, focus_distance, err, len(err))

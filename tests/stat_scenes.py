@@ -182,3 +182,103 @@ def lit_floor_expected(scene, lights, size, sub=16) -> np.ndarray:
         val = cos * light_falloff(lights, k, dist) * light_cone(lights, k, -to)
         total += val[..., None] * np.asarray(lights[k]["colour"], np.float64) * base
     return total.reshape(h, sub, w, sub, 3).mean(axis=(1, 3))
+
+
+# ---- direct light with the rect light of gpu.rs:4443-4560, plus a disc ----
+RECT = dict(centre=(0.9, 1.0, -0.4), half_u=(0.3, 0.0, 0.0), half_v=(0.0, 0.0, 0.2), colour=(0.8, 0.9, 1.0),
+            intensity=2.0)
+DISC = dict(centre=(-0.6, 0.9, -1.0), normal=(0.0, -1.0, 0.0), radius=0.25, colour=(1.0, 0.8, 0.6), intensity=1.5)
+
+
+def lit_floor_area() -> tuple:
+    """The reference's `local_lights_match_the_live_direct_term` lights on
+    the Lambertian floor: the point light of radius 0.1, the spot light and
+    the one-sided rect light facing down (gpu.rs:4491-4509), plus a
+    one-sided disc light facing down that casts no shadow (its light is
+    sampled without MIS or a visibility test). The analytic occluder sphere
+    of the reference's scene is left out (no analytic shapes here). The
+    camera looks down on the backs of the area lights, which do not show."""
+    from pfx_amd.scene import disc_light, rect_light
+
+    floor = quad((-4.0, 0.0, -4.0), (4.0, 0.0, -4.0), (4.0, 0.0, 4.0), (-4.0, 0.0, 4.0))
+    mats = material(LIT_FLOOR_BASE, roughness=0.6, specular=0.0)
+    lights = np.concatenate([
+        point_light((-0.8, 1.2, 0.3), (1.0, 0.9, 0.8), 3.0, 8.0, radius=0.1),
+        spot_light((1.2, 1.5, 1.2), (0.9, 1.0, 0.8), 4.0, 10.0, (0.0, -1.0, 0.0), 15.0, 25.0),
+        rect_light(RECT["centre"], RECT["half_u"], RECT["half_v"], RECT["colour"], RECT["intensity"]),
+        disc_light(DISC["centre"], DISC["normal"], DISC["radius"], DISC["colour"], DISC["intensity"], shadow=False),
+    ])
+    cam = Camera(origin=(0.0, 4.0, 0.0), forward=(0.0, -1.0, 0.0), right=(0.6, 0.0, 0.0), up=(0.0, 0.0, -0.6))
+    scene = Scene(triangles=triangles(floor, 0), materials=mats, camera=cam, sun=NO_SUN, lights=lights)
+    return scene, lights
+
+
+def _floor_points(scene, size, sub):
+    w, h = size
+    cam = scene.camera
+    o = np.asarray(cam.origin, np.float64)
+    fwd, right, up = (np.asarray(v, np.float64) for v in (cam.forward, cam.right, cam.up))
+    s = (np.arange(sub) + 0.5) / sub
+    px = (np.arange(w)[:, None] + s[None, :]).ravel()
+    py = (np.arange(h)[:, None] + s[None, :]).ravel()
+    d = fwd[None, None] + right[None, None] * (px / w * 2.0 - 1.0)[None, :, None] \
+        + up[None, None] * (-(py / h * 2.0 - 1.0))[:, None, None]
+    d = d / np.linalg.norm(d, axis=-1, keepdims=True)
+    return o + d * (-o[1] / d[..., 1:2])
+
+
+def _area_term(p, samples, cos_light, cell_area, radiance):
+    """Sum over emitter cells of L cos(floor) cos(light) dA / d^2 at floor
+    points p (the floor's normal is +y)."""
+    total = np.zeros(p.shape[:-1] + (3,))
+    for x, c, a in zip(samples, cos_light, cell_area):
+        to = x - p
+        d2 = np.einsum("...k,...k->...", to, to)
+        l = to / np.sqrt(d2)[..., None]
+        cf = np.clip(l[..., 1], 0.0, None)
+        cl = np.clip(-(l @ c), 0.0, None)
+        total += (cf * cl * a / d2)[..., None] * radiance
+    return total
+
+
+def lit_floor_area_expected(scene, lights, size, sub=4, cells=48) -> np.ndarray:
+    """Each pixel's expected colour: base / pi times the point and spot
+    terms (as lit_floor_expected) plus the rect and disc integrated over a
+    cells x cells grid of their surfaces (the reference's own 48 x 48
+    midpoint rule for the rect, gpu.rs:4590-4608; polar cells for the
+    disc), averaged over sub x sub points of each pixel."""
+    w, h = size
+    p = _floor_points(scene, size, sub)
+    base = np.asarray(LIT_FLOOR_BASE) / np.pi
+    total = np.zeros(p.shape)
+    for k in range(2):
+        to = np.asarray(lights[k]["position"], np.float64) - p
+        dist = np.linalg.norm(to, axis=-1)
+        cos = np.clip(to[..., 1] / dist, 0.0, None)
+        val = cos * light_falloff(lights, k, dist) * light_cone(lights, k, -to)
+        total += val[..., None] * np.asarray(lights[k]["colour"], np.float64)
+    # rect: area 4 |u x v|, radiance colour * intensity / area, normal u x v / |u x v|
+    hu, hv = np.asarray(RECT["half_u"]), np.asarray(RECT["half_v"])
+    n = np.cross(hu, hv)
+    area = 4.0 * np.linalg.norm(n)
+    n = n / np.linalg.norm(n)
+    g = (np.arange(cells) + 0.5) / cells * 2.0 - 1.0
+    pts = [np.asarray(RECT["centre"]) + hu * u + hv * v for v in g for u in g]
+    total += _area_term(p, pts, [n] * len(pts), [area / cells**2] * len(pts),
+                        np.asarray(RECT["colour"]) * RECT["intensity"] / area)
+    # disc: polar midpoint cells
+    r0, nd = DISC["radius"], np.asarray(DISC["normal"], np.float64)
+    nd = nd / np.linalg.norm(nd)
+    e1 = np.array([1.0, 0.0, 0.0])
+    e2 = np.cross(nd, e1)
+    area = np.pi * r0 * r0
+    rs = (np.arange(cells) + 0.5) / cells * r0
+    phis = (np.arange(2 * cells) + 0.5) / (2 * cells) * 2.0 * np.pi
+    pts, cells_a = [], []
+    for r in rs:
+        for f in phis:
+            pts.append(np.asarray(DISC["centre"]) + r * (np.cos(f) * e1 + np.sin(f) * e2))
+            cells_a.append(r * (r0 / cells) * (2.0 * np.pi / (2 * cells)))
+    total += _area_term(p, pts, [nd] * len(pts), cells_a, np.asarray(DISC["colour"]) * DISC["intensity"] / area)
+    total = total * base
+    return total.reshape(h, sub, w, sub, 3).mean(axis=(1, 3))

@@ -140,3 +140,13 @@ def test_cast_refuses_unequal_ray_arrays():
     for d in (np.zeros((3, 3), np.float32), np.zeros((4, 2), np.float32), np.zeros(11, np.float32)):
         with pytest.raises(ValueError):
             Trace.cast(object(), o, d)
+
+
+def test_device_limits_round_trip(native_lib):
+    got = N.DeviceLimitsC()
+    native_lib.pfxh_set_device_limits(C.byref(N.DeviceLimitsC(123, 456)))
+    native_lib.pfxh_get_device_limits(C.byref(got))
+    assert (got.max_buffer_size, got.max_memory) == (123, 456)
+    native_lib.pfxh_set_device_limits(None)
+    native_lib.pfxh_get_device_limits(C.byref(got))
+    assert (got.max_buffer_size, got.max_memory) == (0, 0)

@@ -193,3 +193,98 @@ def test_uneven_32_still_batch_equals_single_traces():
     rel = rel_mse(outs[k].color[..., :3], rc[..., :3])
     print(f"  oracle on still {k}: relMSE {rel:.3e}")
     assert rel <= 1e-4
+
+
+def test_set_environment_and_sun_radius_match_the_oracle():
+    """Trace::set_environment (gpu.rs:2810-2815) on a trace built under a
+    black sky (its shadow-ray buffers grow by the sky's light slot), then
+    Trace::set_sun_radius (:2903-2910) with a sun: each render equals the
+    oracle's after the same setters, and each setter resets the count."""
+    from pfx_amd import Sun
+
+    w, h = 96, 54
+    scene = small_terrain(w, h)
+    scene.sun = Sun(direction=(0.3, 1.0, 0.2), color=(1.0, 0.95, 0.9), intensity=2.0)
+    detail = Detail(bounces=Bounces.deep(6))
+    o = Oracle(scene, detail, w, h)
+    with Trace.new_detailed(scene, detail, w, h) as t:
+        for step, (sky, radius) in enumerate([(hdr_sky(), None), (hdr_sky(), 3.0), (None, 0.5)]):
+            print(f"setters step {step}: sky {'lit' if sky is not None else 'black'} radius {radius}", flush=True)
+            t.sample(2, 1)
+            t.set_environment(sky)
+            o.set_environment(sky)
+            if radius is not None:
+                t.set_sun_radius(radius)
+                o.set_sun_radius(radius)
+            assert t.samples() == 0
+            t.sample(4, 21 + step)
+            o.sample(4, 21 + step)
+            out = t.readback()
+            rc, _, _ = o.readback()
+            exact = float(np.mean(out.color == rc))
+            rel = rel_mse(out.color[..., :3], rc[..., :3])
+            print(f"  exact fraction {exact:.6f} relMSE {rel:.3e}")
+            assert rel <= 1e-4 and exact > 0.99
+        with pytest.raises(RuntimeError):
+            t.set_sun_radius(90.0)
+    o.close()
+
+
+def test_batch_still_setters_reach_the_batch():
+    """A setter on a batch still's lent trace (here set_camera) is what the
+    next batch render uses: the still equals its own trace with that
+    camera."""
+    from pfx_amd import Camera
+
+    detail = Detail(bounces=Bounces.deep(4))
+    stills = mix()[:3]
+    cam = Camera.look((2.0, 6.0, -9.0), (0.0, 0.0, 0.0), 50.0, stills[0].width / stills[0].height)
+    with Batch(stills, detail) as b:
+        b.trace(0).set_camera(cam)
+        b.render()
+        out = b.trace(0).readback()
+    with Trace.new_detailed(stills[0].scene, detail, stills[0].width, stills[0].height) as t:
+        t.set_camera(cam)
+        t.sample(stills[0].spp, stills[0].seed)
+        ref = t.readback()
+    same(out, ref)
+
+
+def test_device_limits_refuse_with_named_shortfalls():
+    """gpu.rs:5138-5200 on this device: under a 380 MB buffer limit (the
+    reference's test uses wgpu's 128 MiB binding limit; here the largest
+    buffer of a small trace is its 352 MB of spilled traversal stacks, sized
+    for every thread the GPU holds) a 64x64 trace builds and samples, and a
+    3840x2160 one is refused (PFXH_REFUSED) naming the limit, the
+    48-byte-per-pixel accumulator it needed and the limit it met; a memory
+    limit is named the same way; the device's own limits come back
+    afterwards."""
+    import ctypes as C
+
+    from pfx_amd import _native as N
+    from pfx_amd.trace import TraceError
+
+    lib = N.lib()
+    scene = scenes.box_spheres(width=64, height=64)
+    try:
+        lib.pfxh_set_device_limits(C.byref(N.DeviceLimitsC(380_000_000, 0)))
+        with Trace.new_detailed(scene, Detail(bounces=Bounces.deep(4)), 64, 64) as t:
+            t.sample(2, 3)
+            assert t.samples() == 2
+        with pytest.raises(TraceError) as info:
+            Trace.new_detailed(scene, Detail(bounces=Bounces.deep(4)), 3840, 2160)
+        assert info.value.refused
+        msg = str(info.value)
+        print(f"  refused: {msg}")
+        assert msg.startswith(f"limit max_buffer_size of {3840 * 2160 * 48} is beyond the device's 380000000")
+        lib.pfxh_set_device_limits(C.byref(N.DeviceLimitsC(0, 64 << 20)))
+        with pytest.raises(TraceError) as info:
+            Trace.new_detailed(scene, Detail(bounces=Bounces.deep(4)), 64, 64)
+        assert info.value.refused and str(info.value).startswith("limit max_memory of ")
+    finally:
+        lib.pfxh_set_device_limits(None)
+    got = N.DeviceLimitsC()
+    lib.pfxh_get_device_limits(C.byref(got))
+    assert got.max_buffer_size == 0 and got.max_memory == 0
+    with Trace.new_detailed(scene, Detail(bounces=Bounces.deep(4)), 3840, 2160) as t:
+        assert t.gpu_bytes() > 3840 * 2160 * 48

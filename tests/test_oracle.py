@@ -11,7 +11,7 @@ import pytest
 from oracle import oracle as O
 from pfx_amd import _native as N
 from pfx_amd import scenes
-from pfx_amd.scene import point_light, spot_light
+from pfx_amd.scene import disc_light, point_light, rect_light, spot_light
 
 
 def lcg17():
@@ -201,6 +201,58 @@ def test_lights_validate_and_pack():
     assert 0.0 < e < 1.0
     wide = spot_light((0.0, 2.0, 0.0), (1.0, 0.5, 0.25), 4.0, 10.0, (0.0, -1.0, 0.0), 40.0, 30.0)
     assert not lib.or_light_valid(wide.ctypes.data)
+    # the rect light of the same test: area 0.5, radiance 2 / 0.5 = 4 per channel
+    rect = rect_light((0.0, 2.0, 0.0), (0.5, 0.0, 0.0), (0.0, 0.0, 0.25), (1.0, 1.0, 1.0), 2.0)
+    assert lib.or_light_valid(rect.ctypes.data)
+    packed = _pack(rect)
+    assert list(packed[1, :3]) == [4.0, 4.0, 4.0]
+    skew = rect.copy()
+    skew["half_v"] = (0.5, 0.0, 0.25)
+    assert not lib.or_light_valid(skew.ctypes.data)
+
+
+def test_area_lights_pack_and_validate():
+    """lights.rs:101-152, 207-230 for rect and disc lights: kind in a.w,
+    power over area in b.xyz with b.w = f32::MAX, half_u or the unit normal
+    with the two-sidedness in c, half_v or the radius with the shadow flag
+    in d; the validity rules of both shapes; product rows == oracle rows."""
+    lib = O.lib()
+    rect = rect_light((1.0, 2.0, 3.0), (0.3, 0.0, 0.0), (0.0, 0.0, 0.2), (0.8, 0.9, 1.0), 2.0, two_sided=True)
+    packed = _pack(rect)
+    area = np.float32(4.0) * np.float32(0.3 * 0.2)
+    assert packed[0, 3] == 2.0 and list(packed[0, :3]) == [1.0, 2.0, 3.0]
+    np.testing.assert_allclose(packed[1, :3], np.float32([0.8, 0.9, 1.0]) * np.float32(2.0) / area, rtol=1e-6)
+    assert packed[1, 3] == np.finfo(np.float32).max
+    assert list(packed[2]) == [np.float32(0.3), 0.0, 0.0, 1.0] and list(packed[3]) == [0.0, 0.0, np.float32(0.2), 1.0]
+    disc = disc_light((0.0, 1.0, 0.0), (0.0, -2.0, 0.0), 0.25, (1.0, 1.0, 1.0), np.pi * 0.0625, shadow=False)
+    packed = _pack(disc)
+    assert packed[0, 3] == 3.0
+    np.testing.assert_allclose(packed[1, :3], [1.0, 1.0, 1.0], rtol=1e-6)  # power pi r^2 over area pi r^2
+    assert list(packed[2]) == [0.0, -1.0, 0.0, 0.0] and list(packed[3]) == [0.25, 0.0, 0.0, 0.0]
+    for bad in (dict(disc_radius=0.0), dict(direction=(0.0, 0.0, 0.0)), dict(disc_radius=np.nan)):
+        d = disc.copy()
+        for k, v in bad.items():
+            d[k] = v
+        assert not lib.or_light_valid(d.ctypes.data)
+    flat = rect.copy()
+    flat["half_u"] = (0.0, 0.0, 0.0)
+    assert not lib.or_light_valid(flat.ctypes.data)
+    # area lights never go to the tree: with five point lights and two
+    # area lights the rows are [area lights][treed in tree order][nodes]
+    lights = np.concatenate([point_light((k, 3.0, 0.0), (1, 1, 1), 5.0, 20.0) for k in range(3)] + [rect, disc] +
+                            [point_light((k, 3.0, 2.0), (1, 1, 1), 5.0, 20.0) for k in range(2)])
+    rows = np.zeros((3 * len(lights), 16), np.float32)
+    looped, tree = C.c_uint32(), C.c_uint32()
+    n = lib.or_lights(lights.ctypes.data, len(lights), rows.ctypes.data, C.byref(looped), C.byref(tree))
+    assert looped.value == 2 and tree.value == 5
+    assert rows[0, 3] == 2.0 and rows[1, 3] == 3.0
+    prod = np.zeros_like(rows)
+    pn, pl, pt = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    err = C.create_string_buffer(256)
+    assert N.lib().pfxh_layout_lights(lights.ctypes.data, len(lights), prod.ctypes.data, C.byref(pn), C.byref(pl),
+                                      C.byref(pt), err, 256) == 0, err.value
+    assert pn.value == n and pl.value == 2 and pt.value == 5
+    assert prod[:n].tobytes() == rows[:n].tobytes()
 
 
 def street(count):

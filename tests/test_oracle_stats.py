@@ -128,3 +128,24 @@ def test_point_and_spot_lights_match_the_direct_term():
     e, g = blocks(expected), blocks(img)
     bright = e > 0.05 * e.max()
     assert np.all(np.abs(g[bright] - e[bright]) <= 0.03 * e[bright])
+
+
+def test_area_lights_match_the_direct_term():
+    """gpu.rs:4443-4560 with its rect light (and a disc that casts no
+    shadow): the floor lit by a point, a spot, the rect and the disc, direct
+    light only, converges to the closed form; the rect and disc are sampled
+    by NEE with MIS against BSDF rays that hit them (lights.wgsl:64-91,
+    149-170), the unshadowed disc by NEE alone."""
+    scene, lights = S.lit_floor_area()
+    size = (32, 32)
+    img = render(scene, Detail(bounces=Bounces(0, 0, 0, 0)), size, 1024, 13)[0][..., :3].astype(np.float64)
+    expected = S.lit_floor_area_expected(scene, lights, size)
+    ratio = img.mean() / expected.mean()
+    print(f"image mean / direct term {ratio:.4f}")
+    assert abs(ratio - 1.0) < 0.01
+    blocks = lambda a: a.reshape(8, 4, 8, 4, 3).mean(axis=(1, 3))
+    e, g = blocks(expected), blocks(img)
+    bright = e > 0.05 * e.max()
+    worst = float(np.max(np.abs(g[bright] - e[bright]) / e[bright]))
+    print(f"  worst bright block {worst:.4f}")
+    assert worst <= 0.03
