@@ -335,6 +335,77 @@ void pfxh_trace_set_counting(pfxh_trace* trace, int on);
 /* Drop the trace and its device memory. NULL is allowed. */
 void pfxh_trace_free(pfxh_trace* trace);
 
+/* ---- staging (crates/trace/src/stage.rs) ---------------------------------
+ * Stage::build (stage.rs:210-385): indexed meshes and their placements in,
+ * a scene and detail out, for callers that hold meshes the way a loaded
+ * scene does. Content images, text, motion and clip planes are not staged
+ * (those layers are not built here). */
+
+/* stage::Mesh (stage.rs:27-36) without uv sets: 3 floats per vertex of
+ * positions and normals, an optional alpha per vertex (NULL: opaque), and
+ * 3 indices per triangle. */
+typedef struct pfxh_stage_mesh {
+    const float* positions;
+    const float* normals;
+    const float* alpha;
+    uint32_t vertex_count;
+    const uint32_t* indices;
+    uint32_t index_count;
+} pfxh_stage_mesh;
+
+/* stage::Placement (stage.rs:150-195). Placement::new sets casts_shadow 1,
+ * shadow_only 0, two_sided 1 and alpha_cutoff 0.5 (ALPHA_CUTOFF). */
+typedef struct pfxh_placement {
+    uint32_t mesh;
+    uint32_t material;
+    float model[16]; /* column-major 4x4, affine */
+    uint32_t casts_shadow;
+    uint32_t shadow_only;
+    uint32_t two_sided;
+    float alpha_cutoff; /* corners below it are cut away (stage.rs:770-822) */
+} pfxh_placement;
+
+/* stage::Stage (stage.rs:196-203), plus the lights the trace's detail
+ * takes. projection is a PFXH_PROJ_* value, ortho_size its view size; the
+ * lens is detail::Lens (aperture, focus distance > 0). */
+typedef struct pfxh_stage {
+    const pfxh_stage_mesh* meshes;
+    uint32_t mesh_count;
+    const pfxh_placement* placements;
+    uint32_t placement_count;
+    const pfxh_material* materials;
+    uint32_t material_count;
+    const pfxh_light* lights;
+    uint32_t light_count;
+    pfxh_environment sky;
+    pfxh_sun sun;
+    pfxh_camera camera;
+    uint32_t projection;
+    float ortho_size[2];
+    float lens_shift[2];
+    float aperture;
+    float focus_distance;
+} pfxh_stage;
+
+typedef struct pfxh_staged pfxh_staged;
+
+/* Stage::build: PFXH_SETUP with the reference's message for a stage with
+ * no material, an invalid projection or lens, a mesh without vertices or
+ * indices or with an index outside its vertices, a placement naming a
+ * missing mesh or material, or an alpha cutoff outside [0, 1]. Placements
+ * of one mesh, cutoff and material share one mesh of the staged scene. */
+pfxh_status pfxh_stage_build(const pfxh_stage* stage, pfxh_staged** out, char* err, size_t err_len);
+/* Staged::scene: valid until pfxh_staged_free; no world triangles. */
+const pfxh_scene* pfxh_staged_scene(const pfxh_staged* staged);
+/* Staged::detail: Detail::default() (Bounces::REFERENCE 16/4/8/12) with the
+ * stage's lens and projection. */
+pfxh_status pfxh_staged_detail(const pfxh_staged* staged, pfxh_detail* out);
+/* Staged::trace (stage.rs:383-387): pfxh_trace_new on the staged scene
+ * with `detail` (NULL: the staged detail). */
+pfxh_status pfxh_staged_trace(const pfxh_staged* staged, const pfxh_detail* detail, uint32_t width,
+                              uint32_t height, pfxh_trace** out, char* err, size_t err_len);
+void pfxh_staged_free(pfxh_staged* staged);
+
 /* ---- host-only scene layout (no device needed) -------------------------- */
 
 /* bvh::Node (crates/trace/src/bvh.rs:5-13), 48 bytes */
@@ -590,6 +661,16 @@ pfxh_status pfxh_learner_params(pfxh_learner* learner, float* out, uint64_t coun
 /* Gradient of the loss at the last pfxh_learner_gradient or training step. */
 pfxh_status pfxh_learner_grads(pfxh_learner* learner, float* out, uint64_t count, char* err,
                                size_t err_len);
+/* The training loss: PFXH_LEARN_LOSS_L1_LOG (the default), the mean over
+ * pixels and colour channels of |y - log(1 + target)| with y the network's
+ * log radiance; or PFXH_LEARN_LOSS_REL_L2, the mean of
+ * (x - target)^2 / (x^2 + 0.01) with x = expm1(y) and the denominator held
+ * fixed in the gradient (the relative L2 of Noise2Noise, Lehtinen et al.
+ * 2018: its minimiser over independent noisy targets is their mean, so
+ * targets may be noisy renders), y capped at 40. */
+#define PFXH_LEARN_LOSS_L1_LOG 0u
+#define PFXH_LEARN_LOSS_REL_L2 1u
+pfxh_status pfxh_learner_set_loss(pfxh_learner* learner, uint32_t mode);
 /* Clear Adam's moments and step count. */
 void pfxh_learner_reset_optimizer(pfxh_learner* learner);
 /* Make room on the GPU for `capacity` training crops of side x side pixels

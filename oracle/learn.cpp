@@ -13,7 +13,9 @@
 //   relu, pool 2x2 max, the gradient going to the first maximum in
 //              row-major order
 //   loss       mean over pixels and the three colour channels of
-//              |y - log(1 + target)|
+//              |y - log(1 + target)| (mode 0), or of (x - target)^2 /
+//              (x^2 + 0.01) with x = expm1(min(y, 40)) and the denominator
+//              held fixed in the gradient (mode 1, relative L2)
 //   backward   the chain rule through the above, by direct loops
 #include <math.h>
 #include <stdint.h>
@@ -177,6 +179,9 @@ void or_learn_layer(uint32_t k, int32_t* out /* in0, in1, cin, cout, level, up, 
 }
 
 // Loss (returned) and, when `grad` is not null, its gradient for `batch`
+int g_loss_mode = 0;
+void or_learn_set_loss(int mode) { g_loss_mode = mode; }
+
 // crops of h x w pixels (multiples of 16) stored one after another.
 double or_learn_grad(const double* params, uint32_t batch, uint32_t h, uint32_t w, const float* color,
                      const float* albedo, const float* normal, const float* depth, const float* target,
@@ -194,9 +199,17 @@ double or_learn_grad(const double* params, uint32_t batch, uint32_t h, uint32_t 
     double scale = 1.0 / (3.0 * (double)px);
     for (size_t p = 0; p < px; ++p)
         for (int k = 0; k < 3; ++k) {
-            double d = y.v[p * 3 + k] - log1p(fmax((double)target[4 * p + k], 0.0));
-            loss += fabs(d);
-            g[16].v[p * 3 + k] = d > 0.0 ? scale : (d < 0.0 ? -scale : 0.0);
+            double t = fmax((double)target[4 * p + k], 0.0);
+            if (g_loss_mode == 1) {
+                double v = fmin(y.v[p * 3 + k], 40.0);
+                double x = expm1(v), den = x * x + 0.01, d = x - t;
+                loss += d * d / den;
+                g[16].v[p * 3 + k] = y.v[p * 3 + k] > 40.0 ? 0.0 : scale * (2.0 * d / den) * (x + 1.0);
+            } else {
+                double d = y.v[p * 3 + k] - log1p(t);
+                loss += fabs(d);
+                g[16].v[p * 3 + k] = d > 0.0 ? scale : (d < 0.0 ? -scale : 0.0);
+            }
         }
     loss *= scale;
     if (!grad) return loss;

@@ -40,6 +40,7 @@ def lib() -> C.CDLL:
             "or_learn_param_count": (C.c_uint64, []),
             "or_learn_layer": (None, [_U, _P]),
             "or_learn_grad": (C.c_double, [_P, _U, _U, _U, _P, _P, _P, _P, _P, _P]),
+            "or_learn_set_loss": (None, [C.c_int]),
             "or_learn_forward": (None, [_P, _U, _U, _P, _P, _P, _P, _P]),
             "or_scene_free": (None, [_P]),
             "or_set_camera": (None, [_P, _P]),
@@ -229,8 +230,10 @@ def _f32(a):
     return None if a is None else np.ascontiguousarray(a, np.float32)
 
 
-def learn_grad(params, color, albedo, normal, depth, target, want_grad: bool = True):
-    """Loss and gradient for crops shaped (batch, h, w, 4) (depth (batch, h, w))."""
+def learn_grad(params, color, albedo, normal, depth, target, want_grad: bool = True, loss: str = "l1_log"):
+    """Loss and gradient for crops shaped (batch, h, w, 4) (depth (batch, h, w));
+    loss 'l1_log' or 'rel_l2'."""
+    lib().or_learn_set_loss({"l1_log": 0, "rel_l2": 1}[loss])
     p = np.ascontiguousarray(params, np.float64)
     color, albedo, normal, depth, target = (_f32(a) for a in (color, albedo, normal, depth, target))
     b, h, w = color.shape[:3]

@@ -244,6 +244,12 @@ SIGNATURES = {
     "pfxh_trace_set_projection": (C.c_int, [_P, C.c_uint32, C.c_float, C.c_float, _S, _Z]),
     "pfxh_trace_set_sun_radius": (C.c_int, [_P, C.c_float, _S, _Z]),
     "pfxh_set_device_limits": (None, [_P]),
+    "pfxh_learner_set_loss": (C.c_int, [_P, _U]),
+    "pfxh_stage_build": (C.c_int, [_P, C.POINTER(_P), _S, _Z]),
+    "pfxh_staged_scene": (_P, [_P]),
+    "pfxh_staged_detail": (C.c_int, [_P, _P]),
+    "pfxh_staged_trace": (C.c_int, [_P, _P, _U, _U, C.POINTER(_P), _S, _Z]),
+    "pfxh_staged_free": (None, [_P]),
     "pfxh_get_device_limits": (None, [_P]),
     "pfxh_trace_set_environment": (C.c_int, [_P, _P, _S, _Z]),
     "pfxh_checkpoint_write": (C.c_int, [_P, _S, _P, _Z, C.POINTER(C.c_size_t), _S, _Z]),

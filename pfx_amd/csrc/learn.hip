@@ -14,8 +14,9 @@
 //                           over pixels on the matrix cores (BF16 or FP32
 //                           operands), partial sums per pixel chunk, then
 //                           k_dw_sum / k_db_sum
-//   k_learn_in / k_learn_out / k_loss   input encoding, decoding, L1 loss on
-//                           log radiance and its gradient
+//   k_learn_in / k_learn_out / k_loss   input encoding, decoding, the loss (L1 on
+//                           log radiance, or relative L2 of linear radiance)
+//                           and its gradient
 //   k_adam                  Adam (Kingma & Ba 2015) with bias correction
 #include <hip/hip_runtime.h>
 
@@ -360,20 +361,35 @@ __global__ void k_learn_in(const float4* color, const float4* albedo, const floa
     float4* o = reinterpret_cast<float4*>(x + q * 16);
     for (int k = 0; k < 4; ++k) o[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
 }
-// L1 between the network's log radiance (channels 0..2 of y, 16 per pixel)
-// and log(1 + target); writes the gradient of the mean and per-block sums
-__global__ void k_loss(const float* y, const float4* target, size_t n, float scale, float* dy, float* sums) {
+// The loss between the network's log radiance (channels 0..2 of y, 16 per
+// pixel) and the target; writes the gradient of the mean and per-block
+// sums. Mode 0: L1 between y and log(1 + target). Mode 1: the relative L2
+// of linear radiance, (x - target)^2 / (x^2 + 0.01) with x = expm1(y) and
+// the denominator held fixed (Lehtinen et al., Noise2Noise, 2018), whose
+// minimiser over noisy targets is their mean; y is capped at 40 so x^2
+// stays finite in FP32.
+__global__ void k_loss(const float* y, const float4* target, size_t n, float scale, uint32_t mode, float* dy,
+                       float* sums) {
     __shared__ float red[256];
     size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
     float l = 0.0f;
     if (p < n) {
         float4 t = target[p];
-        float tv[3] = {log1pf(fmaxf(t.x, 0.0f)), log1pf(fmaxf(t.y, 0.0f)), log1pf(fmaxf(t.z, 0.0f))};
+        float tl[3] = {fmaxf(t.x, 0.0f), fmaxf(t.y, 0.0f), fmaxf(t.z, 0.0f)};
         float g[16] = {};
         for (int k = 0; k < 3; ++k) {
-            float dlt = y[p * 16 + k] - tv[k];
-            l += fabsf(dlt);
-            g[k] = dlt > 0.0f ? scale : (dlt < 0.0f ? -scale : 0.0f);
+            if (mode == 1u) {
+                float v = fminf(y[p * 16 + k], 40.0f);
+                float x = expm1f(v);
+                float den = x * x + 0.01f;
+                float dlt = x - tl[k];
+                l += dlt * dlt / den;
+                g[k] = y[p * 16 + k] > 40.0f ? 0.0f : scale * (2.0f * dlt / den) * (x + 1.0f);
+            } else {
+                float dlt = y[p * 16 + k] - log1pf(tl[k]);
+                l += fabsf(dlt);
+                g[k] = dlt > 0.0f ? scale : (dlt < 0.0f ? -scale : 0.0f);
+            }
         }
         float4* o = reinterpret_cast<float4*>(dy + p * 16);
         for (int k = 0; k < 4; ++k) o[k] = make_float4(g[4 * k], g[4 * k + 1], g[4 * k + 2], g[4 * k + 3]);
@@ -420,9 +436,9 @@ void learn_in(hipStream_t s, const float4* c, const float4* a, const float4* nm,
     hipLaunchKernelGGL(k_learn_in, dim3(blocks((size_t)w16 * h16, 256)), dim3(256), 0, s, c, a, nm, d, ds, w, h, w16,
                        h16, x);
 }
-uint32_t loss(hipStream_t s, const float* y, const float4* t, size_t n, float* dy, float* sums) {
+uint32_t loss(hipStream_t s, const float* y, const float4* t, size_t n, uint32_t mode, float* dy, float* sums) {
     uint32_t nb = blocks(n, 256);
-    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, y, t, n, 1.0f / (3.0f * (float)n), dy, sums);
+    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, y, t, n, 1.0f / (3.0f * (float)n), mode, dy, sums);
     return nb;
 }
 void learn_out(hipStream_t s, const float* y, uint32_t w16, const float4* color, uint32_t w, uint32_t h, float4* out) {

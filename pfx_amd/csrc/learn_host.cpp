@@ -131,6 +131,7 @@ struct pfxh_learner {
     float *master = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
     std::vector<float*> wf, wt;  // packed for k_conv: forward, transposed
     int adam_t = 0;
+    uint32_t loss_mode = 0;  // PFXH_LEARN_LOSS_*
     // training set: encoded inputs [crop][side][side][16], targets [crop][side][side]
     uint32_t side = 0, capacity = 0, crops = 0;
     float* inputs = nullptr;
@@ -363,7 +364,7 @@ struct pfxh_learner {
         const size_t px = (size_t)f.batch * f.h * f.w;
         pfxl::gather(stream, inputs, targets, (size_t)side * side, side, picks_dev, f.batch, f.t[0], f.target);
         forward(f);
-        uint32_t nb = pfxl::loss(stream, f.t[TENSORS - 1], f.target, px, f.dt[TENSORS - 1], f.sums);
+        uint32_t nb = pfxl::loss(stream, f.t[TENSORS - 1], f.target, px, loss_mode, f.dt[TENSORS - 1], f.sums);
         pfxl::loss_total(stream, f.sums, nb, px, f.losses + slot);
         backward(f);
         hip_check(hipGetLastError(), "training launch");
@@ -488,6 +489,12 @@ pfxh_status pfxh_learner_grads(pfxh_learner* l, float* out, uint64_t count, char
         if (count != canonical_count()) throw Failure{PFXH_INVALID, "gradient buffer has the wrong length"};
         l->download(l->grad, out);
     });
+}
+
+pfxh_status pfxh_learner_set_loss(pfxh_learner* l, uint32_t mode) {
+    if (!l || mode > PFXH_LEARN_LOSS_REL_L2) return PFXH_INVALID;
+    l->loss_mode = mode;
+    return PFXH_OK;
 }
 
 void pfxh_learner_reset_optimizer(pfxh_learner* l) {

@@ -123,6 +123,13 @@ class Learner:
     def reset_optimizer(self) -> None:
         N.lib().pfxh_learner_reset_optimizer(self._h)
 
+    def set_loss(self, mode: str) -> None:
+        """'l1_log' (L1 on log radiance, the default) or 'rel_l2' (relative
+        L2 of linear radiance, which tolerates noisy targets)."""
+        code = {"l1_log": 0, "rel_l2": 1}[mode]
+        if N.lib().pfxh_learner_set_loss(self._h, code) != N.OK:
+            raise ValueError(mode)
+
     def reserve(self, side: int, capacity: int) -> None:
         err = N.err_buffer()
         self._check(N.lib().pfxh_learner_reserve(self._h, side, capacity, err, len(err)), err)

@@ -9,7 +9,7 @@ sequence, a high-spp target; both are cut into square crops and go to the
 device-resident training set. Training then runs Adam on the GPU through
 `Learner.train`.
 
-    python -m pfx_amd.train --crops 1600 --out weights.pfxl [--init weights.pfxl]
+    python -m pfx_amd.train --crops 1600 --out weights.pfxl [--init weights.pfxl] [--loss rel_l2]
 """
 from __future__ import annotations
 
@@ -128,6 +128,8 @@ def main(argv=None) -> int:
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--lr-final", type=float, default=1e-4, help="cosine decay to this learning rate")
+    ap.add_argument("--loss", default="l1_log", choices=["l1_log", "rel_l2"],
+                    help="L1 on log radiance, or relative L2 of linear radiance (noisy targets allowed)")
     ap.add_argument("--init", type=Path, help="start from these weights (else He init)")
     ap.add_argument("--out", type=Path, required=True)
     ap.add_argument("--gen-seconds", type=float, default=0, help="stop generating after this long (0: no limit)")
@@ -138,6 +140,7 @@ def main(argv=None) -> int:
     t0 = time.perf_counter()
     params = Lm.load_params(args.init) if args.init else Lm.init_params(args.seed)
     with Lm.Learner(params=params) as learner:
+        learner.set_loss(args.loss)
         deadline = t0 + args.gen_seconds if args.gen_seconds else None
         if args.synthetic:
             n = synthetic(learner, args.crops, args.side, args.seed)

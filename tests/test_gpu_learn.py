@@ -59,18 +59,21 @@ def filled(learner, data, side):
     return learner
 
 
-@pytest.mark.parametrize("fp32,tol,total", [(True, 1e-3, 1e-3), (False, 0.15, 0.05)])
-def test_gradient_matches_fp64_oracle(fp32, tol, total):
-    print(f"learn gradient fp32={fp32}", flush=True)
+@pytest.mark.parametrize("fp32,tol,total,loss", [(True, 1e-3, 1e-3, "l1_log"), (False, 0.15, 0.05, "l1_log"),
+                                                  (True, 1e-3, 1e-3, "rel_l2")])
+def test_gradient_matches_fp64_oracle(fp32, tol, total, loss):
+    print(f"learn gradient fp32={fp32} loss={loss}", flush=True)
     rng = np.random.default_rng(21)
     data = crops(rng, 2, 32, 32)
     p = params_for_test(21)
     picks = [0 << 3 | 0, 1 << 3 | 5]
     with filled(Lm.Learner(fp32=fp32, params=p), data, 32) as L:
-        loss = L.gradient(picks)
+        L.set_loss(loss)
+        loss_value = L.gradient(picks)
         g = L.grads()
     ref_data = [np.stack([a[0], symmetry(a[1], 5)]) for a in data]
-    ref_loss, ref_g = O.learn_grad(p.astype(np.float64), *ref_data)
+    ref_loss, ref_g = O.learn_grad(p.astype(np.float64), *ref_data, loss=loss)
+    loss = loss_value
     assert loss == pytest.approx(ref_loss, rel=tol * 1e-2 if fp32 else 1e-2)
     for k, (lo, hi) in enumerate(layer_slices()):
         gap = np.linalg.norm(g[lo:hi] - ref_g[lo:hi])

@@ -288,3 +288,40 @@ def test_device_limits_refuse_with_named_shortfalls():
     assert got.max_buffer_size == 0 and got.max_memory == 0
     with Trace.new_detailed(scene, Detail(bounces=Bounces.deep(4)), 3840, 2160) as t:
         assert t.gpu_bytes() > 3840 * 2160 * 48
+
+
+def test_staged_scene_traces_like_the_oracle():
+    """Staged::trace (stage.rs:383-387): a staged scene of alpha-cut
+    icospheres in two materials and three placements, traced through
+    pfxh_staged_trace's scene and detail, equals the oracle at equal spp."""
+    from pfx_amd import Camera, point_light
+    from pfx_amd.scene import material
+    from pfx_amd.scenes import icosphere
+    from pfx_amd.stage import IDENTITY, Placement, StageMesh, stage
+
+    ico = icosphere(2).astype(np.float32)
+    pos = ico.reshape(-1, 3)
+    alpha = np.clip(pos[:, 1] * 2.0 + 0.6, 0.0, 1.0).astype(np.float32)  # the bottom is cut away
+    mesh = StageMesh(pos, pos, np.arange(len(pos), dtype=np.uint32), alpha)
+    moved, lifted = IDENTITY.copy(), IDENTITY.copy()
+    moved[3] = [2.5, 0.0, 0.0, 1.0]
+    lifted[3] = [-2.5, 0.5, 1.0, 1.0]
+    mats = np.concatenate([material((0.8, 0.3, 0.2)), material((0.7, 0.7, 0.75), roughness=0.2, metalness=1.0)])
+    lights = np.concatenate([point_light((0.0, 4.0, -3.0), (1, 1, 1), 40.0, 30.0),
+                             point_light((3.0, 2.0, -2.0), (1, 0.8, 0.6), 10.0, 30.0)])
+    w, h = 96, 64
+    staged = stage([mesh], [Placement(0, IDENTITY, 0), Placement(0, moved, 1), Placement(0, lifted, 0, two_sided=False)],
+                   mats, lights=lights, sky=hdr_sky(),
+                   camera=Camera.look((0.0, 1.5, -7.0), (0.0, 0.0, 0.0), 50.0, w / h))
+    detail = staged.detail
+    detail.bounces = Bounces.deep(4)
+    o = Oracle(staged.scene, detail, w, h)
+    o.sample(8, 3)
+    rc, _, _ = o.readback()
+    o.close()
+    with Trace.new_detailed(staged.scene, detail, w, h) as t:
+        t.sample(8, 3)
+        out = t.readback()
+    rel = rel_mse(out.color[..., :3], rc[..., :3])
+    print(f"  staged scene relMSE {rel:.3e}, exact {float(np.mean(out.color == rc)):.6f}")
+    assert rel <= 1e-4 and float(np.mean(out.color == rc)) > 0.99

@@ -115,6 +115,32 @@ def test_oracle_loss_is_mean_l1_of_log_radiance():
     assert loss == pytest.approx(want, rel=1e-12)
 
 
+def test_oracle_relative_l2_loss_and_its_gradient_in_closed_form():
+    """The relative-L2 loss (mode 'rel_l2'): with only the last layer's
+    bias set, y is that bias at every pixel, x = expm1(y), and the loss is
+    the mean of (x - t)^2 / (x^2 + 0.01); its gradient with the denominator
+    held fixed is, for the last bias of channel k, the mean over pixels of
+    2 (x - t) / (x^2 + 0.01) (x + 1) / 3 per channel. A wrong sign, a missing
+    (x + 1) or a denominator taken from the target fails one of these; the
+    other layers' backward pass is the one the finite differences pin."""
+    rng = np.random.default_rng(6)
+    data = list(crops(rng, 2, 16, 16))
+    p = np.zeros(O.learn_param_count())
+    bias = np.array([0.25, 1.5, -0.5])
+    p[layer_slices()[15][1]:] = bias
+    loss, g = O.learn_grad(p, *data, loss="rel_l2")
+    t = np.maximum(data[4][..., :3].astype(np.float64), 0.0)
+    x = np.expm1(bias)
+    den = x * x + 0.01
+    want = np.mean((x - t) ** 2 / den)
+    assert loss == pytest.approx(want, rel=1e-12)
+    want_g = np.mean(2.0 * (x - t) / den * (x + 1.0), axis=(0, 1, 2)) / 3.0
+    np.testing.assert_allclose(g[layer_slices()[15][1]:], want_g, rtol=1e-10)
+    # the L1 mode is untouched by the switch
+    loss1, _ = O.learn_grad(p, *data, want_grad=False)
+    assert loss1 == pytest.approx(np.mean(np.abs(bias - np.log1p(t))), rel=1e-12)
+
+
 def test_weights_file_round_trip(tmp_path, native_lib):
     p = Lm.init_params(7)
     assert p.dtype == np.float32 and len(p) == Lm.param_count()
