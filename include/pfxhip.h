@@ -184,8 +184,16 @@ enum { PFXH_TRI_MOLLER_TRUMBORE = 0, PFXH_TRI_WATERTIGHT = 1 };
  * ray. Any-hit (shadow) walks are left first in both modes. */
 enum { PFXH_WALK_NEAREST = 0, PFXH_WALK_LEFT_FIRST = 1 };
 
+/* Sampler (crates/trace/src/sobol.rs:8-31). PFXH_SAMPLER_PCG is
+ * `Sampler::Pcg`, one PCG stream per path (trace.wgsl:423-441), what the
+ * configs and PFX_TRACE_SAMPLER=pcg use. PFXH_SAMPLER_SOBOL is
+ * `Sampler::Sobol { samples }`, the reference's default: 256 scrambled 4-D
+ * patterns of sequence_size(samples) points (a power of two from 256 to
+ * 8192), looked up per pixel, sample and dimension (sobol.wgsl). */
+enum { PFXH_SAMPLER_PCG = 0, PFXH_SAMPLER_SOBOL = 1 };
+
 /* The parts of detail::Detail (crates/trace/src/detail.rs:374-398) the
- * path uses. The sampler is PCG (`Sampler::Pcg`, sobol.rs:13-23). */
+ * path uses. */
 typedef struct pfxh_detail {
     pfxh_bounces bounces;
     float lens_shift[2];
@@ -200,6 +208,8 @@ typedef struct pfxh_detail {
                                    * Trace::build with emitter_sampling = false (gpu.rs:781) */
     uint32_t triangle_test;       /* PFXH_TRI_*; 0 is the reference's test */
     uint32_t walk_order;          /* PFXH_WALK_*; 0 is nearest first */
+    uint32_t sampler;             /* PFXH_SAMPLER_*; 0 is PCG */
+    uint32_t sobol_samples;       /* Sampler::Sobol { samples }: sizes the patterns (256..8192 points) */
 } pfxh_detail;
 
 /* adaptive::Adaptive (crates/trace/src/adaptive.rs:9-26) */
@@ -445,6 +455,12 @@ pfxh_status pfxh_layout_lights(const pfxh_light* lights, uint32_t count, float* 
  * median-of-medians fallback alone. index < len. */
 pfxh_status pfxh_layout_select(float* keys, uint32_t* ids, size_t len, size_t index, int mode,
                                size_t* result);
+
+/* sobol::table (crates/trace/src/sobol.rs:137-150) for Sampler::Sobol
+ * { samples }: *length = sequence_size(samples) points per pattern
+ * (sobol.rs:33-39); with table non-NULL, the 256 patterns, 4 floats a
+ * point, pattern-major (256 * length * 4 floats). */
+pfxh_status pfxh_layout_sobol(uint32_t samples, float* table, uint32_t* length);
 
 /* lights::build_emitters (crates/trace/src/lights.rs:391-440) for the
  * emissive triangles of a scene: entries (16 floats each, `GpuEmitter`,

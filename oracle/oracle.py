@@ -55,6 +55,8 @@ def lib() -> C.CDLL:
             "or_sample_adaptive": (C.c_int, [_P, _P, _U, _P]),
             "or_tree_pick": (None, [_P, _P, _P, _F, _P]),
             "or_tree_pick_many": (None, [_P, _P, _P, _P, _U, _P]),
+            "or_sobol_pattern": (None, [_P, _U, _U]),
+            "or_sobol_point": (None, [_P, _U, _U, _U, _U, _P]),
             "or_light_meta": (_U, [_P, C.POINTER(_U), C.POINTER(_U)]),
             "or_trace_counts": (None, [_P, _P, _P, _P]),
             "or_cast": (None, [_P, _P, _P, _U, C.c_int, _F, _P, _P]),

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "oracle.h"
+#include "../pfx_amd/csrc/sobol.h"
 
 namespace oracle {
 
@@ -57,6 +58,8 @@ struct Scene {
     SkyCdf sky;
     bool lit = false;
     bool area_lights = false;  // a looped row is a shadowed rect or disc light (light_hits can see it)
+    std::vector<float> sobol;  // Sobol patterns (empty: PCG)
+    uint32_t sobol_len = 0;
     // Frame (gpu.rs:2295-2316)
     v3 origin, forward, right, up;
     v3 sun_dir, sun_color;
@@ -341,6 +344,27 @@ static bool occluded(const Scene& sc, v3 o, v3 d, float limit, Counter* ctr = nu
 }
 
 // ---- BRDF (crates/materials/src/wgsl/brdf.wgsl) ------------------------------
+// A path's random numbers (trace.wgsl:423-446): PCG draws in order, or with
+// the Sobol sampler the components of one dimension's point at a time
+// (sobol.wgsl:44-55; the lookup shared with the product in sobol.h, the
+// patterns built by this file's own generate_4d).
+struct Rng {
+    uint32_t state = 0;         // PCG state, or the Sobol pixel seed (rng.x)
+    uint32_t index = 0;         // Sobol sample index (rng.y)
+    uint32_t base = 0;          // rng.z
+    const float* table = nullptr;
+    uint32_t length = 0;
+    pfx::Sobol4 u{};
+    void at(uint32_t dim) {
+        if (table) u = pfx::sobol_point(table, length, state, index, base + dim);
+    }
+    float x() { return table ? u.x : pfx::rnd(&state); }
+    float y() { return table ? u.y : pfx::rnd(&state); }
+    float z() { return table ? u.z : pfx::rnd(&state); }
+    float w() { return table ? u.w : pfx::rnd(&state); }
+};
+static uint32_t light_dim(uint32_t k) { return k == 0u ? 1u : (k < 13u ? 3u + k : k << 16); }  // :442-446
+
 static v3 mix3(v3 a, v3 b, float t) { return a * (1.0f - t) + b * t; }
 static v3 schlick(v3 f0, float cos_theta, float power) {  // brdf.wgsl:41-44
     float k = fm_pow(1.0f - fm_clamp(cos_theta, 0.0f, 1.0f), power);
@@ -422,17 +446,18 @@ struct BsdfSample {
     uint32_t lobe;
 };
 static v3 reflect(v3 e1, v3 e2) { return e1 - e2 * (2.0f * dot(e2, e1)); }
-static BsdfSample sample_bsdf(const Mat& s, v3 wo, uint32_t* state) {  // trace.wgsl:1716-1773
+static BsdfSample sample_bsdf(const Mat& s, v3 wo, Rng* state) {  // trace.wgsl:1716-1773
     float transmission = 0.0f;
-    if (pfx::rnd(state) < transmission) { /* never: no transmission */ }
+    state->at(3u);
+    if (state->w() < transmission) { /* never: no transmission */ }
     float cw = coat_weight(s, wo);
     float sw = (1.0f - cw) * fm_clamp(s.metalness + s.specular, 0.0f, 0.9f);
-    float pick = pfx::rnd(state);
+    float pick = state->z();
     if (pick < cw + sw) {
         bool coat = pick < cw;
         float alpha = alpha_rough(coat ? s.clearcoat_roughness : s.roughness);
-        float u1 = pfx::rnd(state);
-        float u2 = pfx::rnd(state);
+        float u1 = state->x();
+        float u2 = state->y();
         float phi = 2.0f * PI * u1;
         float cos_theta = sqrtf((1.0f - u2) / (1.0f + (alpha * alpha - 1.0f) * u2));
         float st = sqrtf(fmaxf(0.0f, 1.0f - cos_theta * cos_theta));
@@ -440,8 +465,8 @@ static BsdfSample sample_bsdf(const Mat& s, v3 wo, uint32_t* state) {  // trace.
         v3 wi = reflect(-wo, h);
         return BsdfSample{wi, bsdf_pdf(s, wo, wi), 1u};
     }
-    float u1 = pfx::rnd(state);
-    float u2 = pfx::rnd(state);
+    float u1 = state->x();
+    float u2 = state->y();
     float phi = 2.0f * PI * u1;
     float r = sqrtf(u2);
     v3 wi = mk(fm_cos(phi) * r, fm_sin(phi) * r, sqrtf(1.0f - u2));
@@ -497,11 +522,11 @@ static v4 shadow_ray_to(v3 origin, v3 aim, bool shadow) {  // lights.wgsl:95-102
     if (limit <= 0.0f) return v4{mk(0, 0, 1), 0.0f};
     return v4{ray / distance, limit};
 }
-static Candidate emitter_candidate(const Scene& sc, const m3& T, v3 p, v3 origin, uint32_t* state) {  // :103-144
+static Candidate emitter_candidate(const Scene& sc, const m3& T, v3 p, v3 origin, Rng* state) {  // :103-144
     uint32_t count = (uint32_t)sc.emitters.entries[0].a[0];
-    const GEmitter& e = sc.emitters.entries[find_emitter(sc, count, pfx::rnd(state)) + 1];
-    float r1 = sqrtf(pfx::rnd(state));
-    float r2 = pfx::rnd(state);
+    const GEmitter& e = sc.emitters.entries[find_emitter(sc, count, state->z()) + 1];
+    float r1 = sqrtf(state->x());
+    float r2 = state->y();
     v3 a = t3(e.a), b = t3(e.b), c = t3(e.c);
     v3 x = a * (1.0f - r1) + b * (r1 * (1.0f - r2)) + c * (r1 * r2);
     v3 to = x - p;
@@ -558,12 +583,12 @@ static v3 light_hits(const Scene& sc, v3 o, v3 d, float limit, v3 previous_point
     return total;
 }
 static Candidate light_candidate(const GLight& light, const m3& T, v3 p, v3 origin, v3 geometric,
-                                 uint32_t* state) {  // :145-200
+                                 Rng* state) {  // :145-200
     bool shadow = light.d[3] > 0.5f;
     uint32_t kind = (uint32_t)light.a[3];
     if (kind == 2u || kind == 3u) {  // :149-170: a point on the rect or disc, area pdf in solid angle
-        float u1 = pfx::rnd(state);
-        float u2 = pfx::rnd(state);
+        float u1 = state->x();
+        float u2 = state->y();
         v4 plane = area_frame(light);
         v3 x = t3(light.a) + t3(light.c) * (2.0f * u1 - 1.0f) + t3(light.d) * (2.0f * u2 - 1.0f);
         if (kind == 3u) {
@@ -602,8 +627,8 @@ static Candidate light_candidate(const GLight& light, const m3& T, v3 p, v3 orig
     v3 aim = t3(light.a);
     if (shadow && radius > 0.0f) {
         m3 disc = pfx::basis(l);
-        float r = radius * sqrtf(pfx::rnd(state));
-        float phi = 2.0f * PI * pfx::rnd(state);
+        float r = radius * sqrtf(state->x());
+        float phi = 2.0f * PI * state->y();
         aim = aim + (disc.c0 * (r * fm_cos(phi)) + disc.c1 * (r * fm_sin(phi)));
     }
     float ratio = distance / range;
@@ -806,8 +831,8 @@ F2 tree_pick(const Scene& sc, v3 p, v3 n, float start) {  // :350-386
         pdf *= at == left ? left_probability : 1.0f - left_probability;
     }
 }
-static Candidate tree_light(const Scene& sc, const m3& T, v3 p, v3 origin, v3 geometric, uint32_t* state) {  // :387-393
-    F2 picked = tree_pick(sc, p, T.c2, pfx::rnd(state));
+static Candidate tree_light(const Scene& sc, const m3& T, v3 p, v3 origin, v3 geometric, Rng* state) {  // :387-393
+    F2 picked = tree_pick(sc, p, T.c2, state->z());
     if (!(picked.y > 0.0f)) return candidate_none();
     Candidate c = light_candidate(sc.lights.rows[(uint32_t)picked.x], T, p, origin, geometric, state);
     c.scale = c.scale / picked.y;
@@ -818,7 +843,7 @@ static uint32_t light_slots(const Scene& sc) {  // :400-402
     return 1u + (uint32_t)sc.emitters.entries[0].a[1] + (sc.emitters.entries[0].a[3] > 0.5f ? 1u : 0u);
 }
 static Candidate direct_light(const Scene& sc, uint32_t slot, const m3& T, v3 p, v3 origin, v3 geometric,
-                              uint32_t* state) {  // :403-409
+                              Rng* state) {  // :403-409
     uint32_t looped = (uint32_t)sc.emitters.entries[0].a[1];
     if (slot > looped) return tree_light(sc, T, p, origin, geometric, state);
     if (slot > 0u) return light_candidate(sc.lights.rows[slot - 1u], T, p, origin, geometric, state);
@@ -886,7 +911,7 @@ struct Result {
     float albedo[4];
     float normal[4];
 };
-static Result radiance(const Scene& sc, v3 o0, v3 d0, uint32_t* state, uint64_t* closest, uint64_t* shadows) {
+static Result radiance(const Scene& sc, v3 o0, v3 d0, Rng* state, uint64_t* closest, uint64_t* shadows) {
     Result res;
     memset(&res, 0, sizeof res);
     v3 o = o0, d = d0;
@@ -898,6 +923,7 @@ static Result radiance(const Scene& sc, v3 o0, v3 d0, uint32_t* state, uint64_t*
     const pfxh_bounces& B = sc.bounces;
     uint32_t slots = light_slots(sc);
     for (uint32_t bounce = 0; bounce <= B.total + 1u; ++bounce) {
+        state->base = 16u * (bounce + 1u);  // trace.wgsl:2170
         Hit hit = scene_hit(sc, o, d, 1e20f, false);
         *closest += 1;
         v3 origin_point = pfx::spawn_point(o, -d, o, 0.0f);
@@ -934,6 +960,7 @@ static Result radiance(const Scene& sc, v3 o0, v3 d0, uint32_t* state, uint64_t*
         v3 wo = pfx::tmul(T, -d);
         v3 origin = pfx::spawn_point(p, n, o, hit.t);
         for (uint32_t k = 0; k < 2u + slots; ++k) {
+            state->at(light_dim(k));
             v3 local = mk(0, 0, 1), direction = mk(0, 0, 1), strength = mk(0, 0, 0);
             float limit = 1e19f, pdf = 0.0f;
             Mat lit = s;
@@ -941,9 +968,9 @@ static Result radiance(const Scene& sc, v3 o0, v3 d0, uint32_t* state, uint64_t*
                 if (sc.sun_intensity <= 0.0f) continue;
                 direction = normalize(sc.sun_dir);
                 if (sc.sun_w < 1.0f) {
-                    float height = 1.0f - pfx::rnd(state) * (1.0f - sc.sun_w);
+                    float height = 1.0f - state->x() * (1.0f - sc.sun_w);
                     float radius = sqrtf(fmaxf(0.0f, 1.0f - height * height));
-                    float phi = 2.0f * PI * pfx::rnd(state);
+                    float phi = 2.0f * PI * state->y();
                     m3 orient = pfx::basis(direction);
                     direction = normalize(pfx::mul(orient, mk(radius * fm_cos(phi), radius * fm_sin(phi), height)));
                 }
@@ -951,8 +978,8 @@ static Result radiance(const Scene& sc, v3 o0, v3 d0, uint32_t* state, uint64_t*
                 if (local.z <= 0.0f) continue;
                 strength = sc.sun_color * (sc.sun_intensity * 1.0f);
             } else if (k == 1u) {
-                float rx = pfx::rnd(state);
-                float ry = pfx::rnd(state);
+                float rx = state->x();
+                float ry = state->y();
                 SkySample light = sky_sample(sc, rx, ry);
                 direction = light.direction;
                 local = pfx::tmul(T, direction);
@@ -998,7 +1025,8 @@ static Result radiance(const Scene& sc, v3 o0, v3 d0, uint32_t* state, uint64_t*
         o = pfx::spawn_point(p, d, o, hit.t);
         if (bounce >= 3u) {
             float keep = fm_clamp(fmaxf(throughput.x, fmaxf(throughput.y, throughput.z)), 0.05f, 0.95f);
-            if (pfx::rnd(state) > keep) break;
+            state->at(0u);
+            if (state->x() > keep) break;
             throughput = throughput / keep;
         }
     }
@@ -1007,7 +1035,7 @@ static Result radiance(const Scene& sc, v3 o0, v3 d0, uint32_t* state, uint64_t*
 }
 
 // ---- main: camera rays and accumulation (trace.wgsl:2503-2579) ----------------
-static void camera_ray(const Scene& sc, uint32_t px, uint32_t py, float jx, float jy, v3* o, v3* d, uint32_t* state,
+static void camera_ray(const Scene& sc, uint32_t px, uint32_t py, float jx, float jy, v3* o, v3* d, Rng* state,
                        bool centre) {
     float ndcx = ((float)px + jx) / (float)sc.window[0] * 2.0f - 1.0f;
     float ndcy = ((float)py + jy) / (float)sc.window[1] * 2.0f - 1.0f;
@@ -1026,8 +1054,9 @@ static void camera_ray(const Scene& sc, uint32_t px, uint32_t py, float jx, floa
         direction = normalize(normalize(sc.right) * local.x + normalize(sc.up) * local.y - normalize(sc.forward) * local.z);
     } else if (!centre && sc.lens[2] > 0.0f) {
         v3 focus = origin + direction * (sc.lens[3] / dot(direction, normalize(sc.forward)));
-        float radius = sqrtf(pfx::rnd(state)) * sc.lens[2];
-        float phi = 2.0f * PI * pfx::rnd(state);
+        state->at(1u);
+        float radius = sqrtf(state->x()) * sc.lens[2];
+        float phi = 2.0f * PI * state->y();
         origin = origin + (normalize(sc.right) * (radius * fm_cos(phi)) + normalize(sc.up) * (radius * fm_sin(phi)));
         direction = normalize(focus - origin);
     }
@@ -1062,9 +1091,17 @@ static void sample_pixel(Scene& sc, uint32_t x, uint32_t y, uint32_t count, uint
     uint32_t px = x + sc.window[2], py = y + sc.window[3];
     uint32_t stream = py * sc.window[0] + px;
     for (uint32_t sample = 0; sample < count; ++sample) {
-        uint32_t state = pfx::pcg(stream * 1973u + pfx::pcg(seed * 9277u + (acc + sample) * 7919u + 26699u));
-        float jx = pfx::rnd(&state);
-        float jy = pfx::rnd(&state);
+        Rng state;
+        state.state = pfx::pcg(stream * 1973u + pfx::pcg(seed * 9277u + (acc + sample) * 7919u + 26699u));
+        if (!sc.sobol.empty()) {  // trace.wgsl:2522
+            state.state = pfx::hash_iqnt2d(px, py) ^ seed;
+            state.index = acc + sample;
+            state.table = sc.sobol.data();
+            state.length = sc.sobol_len;
+        }
+        state.at(0u);
+        float jx = state.x();
+        float jy = state.y();
         v3 o, d;
         camera_ray(sc, px, py, jx, jy, &o, &d, &state, false);
         Result r = radiance(sc, o, d, &state, closest, shadows);
@@ -1202,6 +1239,64 @@ static uint64_t converge(Scene& sc, const pfxh_adaptive& a, bool mark) {
     return running;
 }
 
+// ---- Sobol patterns (sobol.rs:40-150) ----------------------------------------
+static const uint32_t XOR_TABLE[4][32] = {  // sobol.rs:41-70
+    {0},
+    {0x00000000, 0x00000001, 0x00000001, 0x00000007, 0x00000001, 0x00000013, 0x00000015, 0x0000007f,
+     0x00000001, 0x00000103, 0x00000105, 0x0000070f, 0x00000111, 0x00001333, 0x00001555, 0x00007fff,
+     0x00000001, 0x00010003, 0x00010005, 0x0007000f, 0x00010011, 0x00130033, 0x00150055, 0x007f00ff,
+     0x00010101, 0x01030303, 0x01050505, 0x070f0f0f, 0x01111111, 0x13333333, 0x15555555, 0x7fffffff},
+    {0x00000000, 0x00000001, 0x00000003, 0x00000001, 0x00000005, 0x0000001f, 0x0000002b, 0x0000003d,
+     0x00000011, 0x00000133, 0x00000377, 0x00000199, 0x00000445, 0x00001ccf, 0x00002ddb, 0x0000366d,
+     0x00000101, 0x00010303, 0x00030707, 0x00010909, 0x00051515, 0x001f3f3f, 0x002b6b6b, 0x003dbdbd,
+     0x00101011, 0x01303033, 0x03707077, 0x01909099, 0x04515145, 0x1cf3f3cf, 0x2db6b6db, 0x36dbdb6d},
+    {0x00000000, 0x00000001, 0x00000000, 0x00000003, 0x0000000d, 0x0000000c, 0x00000005, 0x0000004f,
+     0x00000014, 0x000000e7, 0x00000329, 0x0000039c, 0x00000011, 0x00001033, 0x00000044, 0x000030bb,
+     0x0000d1cd, 0x0000c2ec, 0x00005415, 0x0004fc3f, 0x00015054, 0x000e5c97, 0x0032e5b9, 0x0039725c,
+     0x00000101, 0x01000303, 0x00000404, 0x03000b0b, 0x0d001d1d, 0x0c002c2c, 0x05004545, 0x4f00cfcf},
+};
+static uint32_t hash_hp_uint(uint32_t i) {  // sobol.rs:72-79
+    i ^= i >> 16;
+    i *= 0x21f0aaadu;
+    i ^= i >> 15;
+    i *= 0xd35a2d97u;
+    i ^= i >> 15;
+    return i ^ 0xe6fe3bebu;
+}
+static void generate_4d(float* points, uint32_t size, uint32_t seed) {  // sobol.rs:106-135
+    uint32_t s = 0x44605a73u;
+    s ^= s << 19;
+    uint32_t rng = hash_hp_uint(seed ^ s);
+    const float inv = 1.0f / 4294967808.0f;
+    auto next = [&]() {
+        float value = (float)hash_hp_uint(rng) * inv;
+        rng += 1u;
+        return value;
+    };
+    for (int a = 0; a < 4; ++a) points[a] = next();
+    uint32_t log_n = 0, n = 1;
+    while (n < size) {
+        float strata = (float)(n * 2u);
+        for (uint32_t i = 0; i < n && n + i < size; ++i)
+            for (int axis = 0; axis < 4; ++axis) {
+                uint32_t occupied = (uint32_t)(points[4 * (i ^ XOR_TABLE[axis][log_n]) + axis] * strata);
+                points[4 * (n + i) + axis] = ((float)(occupied ^ 1u) + next()) / strata;
+            }
+        log_n += 1;
+        n *= 2u;
+    }
+}
+extern "C" {
+void or_sobol_pattern(float* points, uint32_t size, uint32_t seed) { generate_4d(points, size, seed); }
+void or_sobol_point(const float* table, uint32_t length, uint32_t seed, uint32_t index, uint32_t dim, float* out) {
+    pfx::Sobol4 r = pfx::sobol_point(table, length, seed, index, dim);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    out[3] = r.w;
+}
+}
+
 static std::string prepare(Scene& sc, const pfxh_scene& in, const pfxh_detail& detail, uint32_t w, uint32_t h) {
     if (in.material_count == 0) return "invalid trace scene detail";
     for (uint32_t i = 0; i < in.light_count; ++i)
@@ -1280,6 +1375,12 @@ static std::string prepare(Scene& sc, const pfxh_scene& in, const pfxh_detail& d
     sc.bounces = detail.bounces;
     sc.accumulate = detail.accumulate;
     sc.watertight = detail.triangle_test == PFXH_TRI_WATERTIGHT;
+    if (detail.sampler == PFXH_SAMPLER_SOBOL) {
+        sc.sobol_len = pfx::sobol_length(detail.sobol_samples);
+        sc.sobol.assign((size_t)sc.sobol_len * pfx::SOBOL_PATTERNS * 4, 0.0f);
+        for (uint32_t p = 0; p < pfx::SOBOL_PATTERNS; ++p)
+            generate_4d(&sc.sobol[4 * (size_t)p * sc.sobol_len], sc.sobol_len, p);
+    }
     sc.sum.assign((size_t)w * h * 12, 0.0f);
     sc.comp.assign((size_t)w * h * 12, 0.0f);
     sc.sum64.assign((size_t)w * h * 12, 0.0);

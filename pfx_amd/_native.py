@@ -136,6 +136,8 @@ class DetailC(C.Structure):
         ("no_emitter_sampling", C.c_uint32),
         ("triangle_test", C.c_uint32),
         ("walk_order", C.c_uint32),
+        ("sampler", C.c_uint32),
+        ("sobol_samples", C.c_uint32),
     ]
 
 
@@ -246,6 +248,7 @@ SIGNATURES = {
     "pfxh_set_device_limits": (None, [_P]),
     "pfxh_learner_set_loss": (C.c_int, [_P, _U]),
     "pfxh_stage_build": (C.c_int, [_P, C.POINTER(_P), _S, _Z]),
+    "pfxh_layout_sobol": (C.c_int, [_U, _P, C.POINTER(_U)]),
     "pfxh_staged_scene": (_P, [_P]),
     "pfxh_staged_detail": (C.c_int, [_P, _P]),
     "pfxh_staged_trace": (C.c_int, [_P, _P, _U, _U, C.POINTER(_P), _S, _Z]),

@@ -20,7 +20,8 @@ CSRC = HERE / "csrc"
 OUT = HERE / "_build"
 LIB = HERE / "libpfxhip.so"
 SOURCES = ["kernels.hip", "layout.cpp", "trace.cpp", "checkpoint.cpp", "denoise.hip", "denoise_host.cpp", "batch.cpp", "atrous.hip",
-           "learn.hip", "learn_host.cpp", "stage.cpp"]
+           "learn.hip", "learn_host.cpp", "stage.cpp",
+           "sobol_table.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950",

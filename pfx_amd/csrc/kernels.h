@@ -33,6 +33,8 @@ struct SceneDev {
     RootEntry root;
     uint32_t env_w, env_h, env_black, lit;
     uint32_t area;  // a looped light row is a shadowed rect or disc light: rays can hit it (light_hits)
+    const float* sobol_table;  // the Sobol sampler's patterns (4 floats a point), or null for PCG
+    uint32_t sobol_len;        // points per pattern
     uint32_t looped, tree, emitter_count, light_slots;
     uint32_t entries;  // NEE entries a vertex may queue
     uint32_t geo_flags;    // GEO_TAME | GEO_WATERTIGHT | GEO_LEFT_FIRST

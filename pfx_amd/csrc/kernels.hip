@@ -23,6 +23,7 @@
 
 #include "../../include/pfxhip.h"
 #include "kernels.h"
+#include "sobol.h"
 #include "vec3.h"
 
 namespace pfxh {
@@ -748,6 +749,29 @@ DEV float bsdf_pdf(const Mat& s, v3 wo, v3 wi) {  // trace.wgsl:1774-1788
 DEV v3 reflect(v3 e1, v3 e2) { return e1 - e2 * (2.0f * dot(e2, e1)); }
 
 // ---- lights (crates/trace/src/lights.wgsl) --------------------------------------
+// A path's random numbers (trace.wgsl:423-446, sobol.wgsl:44-55): with the
+// PCG sampler (`table` null) every draw is the next PCG value, in the
+// reference's order; with the Sobol sampler `at(d)` takes the point of
+// dimension base + d for the path's pixel seed and sample index, and a
+// draw is one of its components (rnd(state, u.x) and its kind).
+struct Draw {
+    uint32_t state;      // PCG state, or the Sobol pixel seed (rng.x)
+    uint32_t index;      // Sobol sample index (rng.y)
+    uint32_t base;       // Sobol dimension of the bounce (rng.z = 16 (bounce + 1))
+    const float* table;  // the Sobol patterns, or null for PCG
+    uint32_t length;
+    pfx::Sobol4 u;
+    DEV void at(uint32_t dim) {
+        if (table) u = pfx::sobol_point(table, length, state, index, base + dim);
+    }
+    DEV float x() { return table ? u.x : pfx::rnd(&state); }
+    DEV float y() { return table ? u.y : pfx::rnd(&state); }
+    DEV float z() { return table ? u.z : pfx::rnd(&state); }
+    DEV float w() { return table ? u.w : pfx::rnd(&state); }
+};
+// The Sobol dimension of NEE slot k (trace.wgsl:442-446)
+DEV uint32_t light_dim(uint32_t k) { return k == 0u ? 1u : (k < 13u ? 3u + k : k << 16); }
+
 struct Cand {
     v3 wi, scale;
     float pdf, roughness;
@@ -772,9 +796,9 @@ DEV void shadow_to(v3 origin, v3 aim, bool shadow, v3& dir, float& limit) {  // 
     dir = ray / distance;
     limit = l;
 }
-DEV Cand emitter_candidate(const SceneDev& S, const m3& T, v3 p, v3 origin, uint32_t* state) {  // :103-144
+DEV Cand emitter_candidate(const SceneDev& S, const m3& T, v3 p, v3 origin, Draw& dr) {  // :103-144
     uint32_t count = (uint32_t)S.emitters[0].x;
-    float uz = pfx::rnd(state);
+    float uz = dr.z();
     uint32_t low = 0, high = count;
     while (low < high) {
         uint32_t mid = (low + high) / 2u;
@@ -786,8 +810,8 @@ DEV Cand emitter_candidate(const SceneDev& S, const m3& T, v3 p, v3 origin, uint
     uint32_t index = (low < count - 1u ? low : count - 1u) + 1u;
     float4 ea = row(S.emitters, index, 0), eb = row(S.emitters, index, 1), ec = row(S.emitters, index, 2),
            ee = row(S.emitters, index, 3);
-    float r1 = sqrtf(pfx::rnd(state));
-    float r2 = pfx::rnd(state);
+    float r1 = sqrtf(dr.x());
+    float r2 = dr.y();
     v3 a = f3(ea), b = f3(eb), c = f3(ec);
     v3 x = a * (1.0f - r1) + b * (r1 * (1.0f - r2)) + c * (r1 * r2);
     v3 to = x - p;
@@ -849,13 +873,13 @@ DEV v3 light_hits(const SceneDev& S, v3 o, v3 d, float limit, v3 previous_point,
     return total;
 }
 DEV Cand light_candidate(const SceneDev& S, uint32_t li, const m3& T, v3 p, v3 origin, v3 geometric,
-                         uint32_t* state) {  // :145-200
+                         Draw& dr) {  // :145-200
     float4 la = row(S.lights, li, 0), lb = row(S.lights, li, 1), lc = row(S.lights, li, 2), ld = row(S.lights, li, 3);
     bool shadow = ld.w > 0.5f;
     uint32_t kind = (uint32_t)la.w;
     if (kind == 2u || kind == 3u) {  // :149-170: a point on the rect or disc, its area pdf in solid angle
-        float u1 = pfx::rnd(state);
-        float u2 = pfx::rnd(state);
+        float u1 = dr.x();
+        float u2 = dr.y();
         float4 plane = area_frame(la, lc, ld);
         v3 x = f3(la) + f3(lc) * (2.0f * u1 - 1.0f) + f3(ld) * (2.0f * u2 - 1.0f);
         if (kind == 3u) {
@@ -903,8 +927,8 @@ DEV Cand light_candidate(const SceneDev& S, uint32_t li, const m3& T, v3 p, v3 o
     v3 aim = f3(la);
     if (shadow && radius > 0.0f) {
         m3 disc = basis(l);
-        float r = radius * sqrtf(pfx::rnd(state));
-        float phi = 2.0f * PI * pfx::rnd(state);
+        float r = radius * sqrtf(dr.x());
+        float phi = 2.0f * PI * dr.y();
         aim = aim + (disc.c0 * (r * fm_cos(phi)) + disc.c1 * (r * fm_sin(phi)));
     }
     float ratio = distance / range;
@@ -1104,18 +1128,18 @@ DEV float2 tree_pick(const SceneDev& S, v3 p, v3 n, float start) {  // :350-386
     }
 }
 DEV Cand direct_light(const SceneDev& S, uint32_t slot, const m3& T, v3 p, v3 origin, v3 geometric,
-                      uint32_t* state) {  // :387-409
+                      Draw& dr) {  // :387-409
     uint32_t looped = (uint32_t)S.emitters[0].y;
     if (slot > looped) {
-        float2 picked = tree_pick(S, p, T.c2, pfx::rnd(state));
+        float2 picked = tree_pick(S, p, T.c2, dr.z());
         if (!(picked.y > 0.0f)) return none();
-        Cand c = light_candidate(S, (uint32_t)picked.x, T, p, origin, geometric, state);
+        Cand c = light_candidate(S, (uint32_t)picked.x, T, p, origin, geometric, dr);
         c.scale = c.scale / picked.y;
         return c;
     }
-    if (slot > 0u) return light_candidate(S, slot - 1u, T, p, origin, geometric, state);
+    if (slot > 0u) return light_candidate(S, slot - 1u, T, p, origin, geometric, dr);
     if (S.emitters[0].x < 0.5f) return none();
-    return emitter_candidate(S, T, p, origin, state);
+    return emitter_candidate(S, T, p, origin, dr);
 }
 
 // ---- sky (crates/trace/src/sky.rs SKY_SAMPLING_WGSL) ---------------------------
@@ -1210,9 +1234,15 @@ DEV void camera_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t p
     uint32_t y = pix / S.width, x = pix - y * S.width;
     uint32_t px = x + S.window[2], py = y + S.window[3];
     uint32_t stream = py * S.window[0] + px;
-    uint32_t state = pfx::pcg(stream * 1973u + pfx::pcg(seed * 9277u + sample * 7919u + 26699u));
-    float jx = pfx::rnd(&state);
-    float jy = pfx::rnd(&state);
+    Draw dr{pfx::pcg(stream * 1973u + pfx::pcg(seed * 9277u + sample * 7919u + 26699u)), 0u, 0u, S.sobol_table,
+            S.sobol_len, {}};
+    if (S.sobol_table) {  // trace.wgsl:2522
+        dr.state = pfx::hash_iqnt2d(px, py) ^ seed;
+        dr.index = sample;
+    }
+    dr.at(0u);
+    float jx = dr.x();
+    float jy = dr.y();
     float ndcx = ((float)px + jx) / (float)S.window[0] * 2.0f - 1.0f;
     float ndcy = ((float)py + jy) / (float)S.window[1] * 2.0f - 1.0f;
     v3 fwd = mk(S.forward[0], S.forward[1], S.forward[2]), right = mk(S.right[0], S.right[1], S.right[2]),
@@ -1223,15 +1253,16 @@ DEV void camera_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t p
         project(S, ndcx, ndcy, fwd, right, up, origin, direction);
     } else if (S.lens[2] > 0.0f) {
         v3 focus = origin + direction * (S.lens[3] / dot(direction, normalize(fwd)));
-        float radius = sqrtf(pfx::rnd(&state)) * S.lens[2];
-        float phi = 2.0f * PI * pfx::rnd(&state);
+        dr.at(1u);
+        float radius = sqrtf(dr.x()) * S.lens[2];
+        float phi = 2.0f * PI * dr.y();
         origin = origin + (normalize(right) * (radius * fm_cos(phi)) + normalize(up) * (radius * fm_sin(phi)));
         direction = normalize(focus - origin);
     }
     W.o[i] = f4(origin, 0.0f);
     W.d[i] = f4(direction, ubits(0u));
-    W.thr[i] = make_float4(1.0f, 1.0f, 1.0f, ubits(state));
-    W.res[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    W.thr[i] = make_float4(1.0f, 1.0f, 1.0f, ubits(dr.state));
+    W.res[i] = make_float4(0.0f, 0.0f, 0.0f, ubits(dr.index));
     W.alb[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     W.nrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
@@ -1355,7 +1386,7 @@ DEV void shade_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t bo
     v3 o = f3(o4), d = f3(d4), throughput = f3(t4), result = f3(r4);
     float previous_pdf = o4.w;
     uint32_t lobes = fbits(d4.w);
-    uint32_t state = fbits(t4.w);
+    Draw dr{fbits(t4.w), fbits(r4.w), 16u * (bounce + 1u), S.sobol_table, S.sobol_len, {}};
     float t = hv.x;
     bool done = false;
     if (S.area) result = result + throughput * light_hits(S, o, d, t, pfx::spawn_point(o, -d, o, 0.0f), previous_pdf);
@@ -1411,13 +1442,14 @@ DEV void shade_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t bo
                 v3 local = mk(0, 0, 1), direction = mk(0, 0, 1), strength = mk(0, 0, 0);
                 float limit = 1e19f, pdf = 0.0f;
                 Mat lit = s;
+                dr.at(light_dim(kk));
                 if (kk == 0u) {
                     if (S.sun_intensity <= 0.0f) continue;
                     direction = normalize(mk(S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]));
                     if (S.sun_w < 1.0f) {
-                        float height = 1.0f - pfx::rnd(&state) * (1.0f - S.sun_w);
+                        float height = 1.0f - dr.x() * (1.0f - S.sun_w);
                         float radius = sqrtf(fmaxf(0.0f, 1.0f - height * height));
-                        float phi = 2.0f * PI * pfx::rnd(&state);
+                        float phi = 2.0f * PI * dr.y();
                         m3 orient = basis(direction);
                         direction =
                             normalize(pfx::mul(orient, mk(radius * fm_cos(phi), radius * fm_sin(phi), height)));
@@ -1426,8 +1458,8 @@ DEV void shade_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t bo
                     if (local.z <= 0.0f) continue;
                     strength = mk(S.sun_color[0], S.sun_color[1], S.sun_color[2]) * (S.sun_intensity * 1.0f);
                 } else if (kk == 1u) {
-                    float rx = pfx::rnd(&state);
-                    float ry = pfx::rnd(&state);
+                    float rx = dr.x();
+                    float ry = dr.y();
                     if (S.env_black) continue;  // strength 0: the term adds nothing
                     float spdf;
                     sky_sample(S, rx, ry, direction, spdf);
@@ -1437,7 +1469,7 @@ DEV void shade_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t bo
                     pdf = spdf;
                     if (!(pdf > 0.0f)) continue;
                 } else {
-                    Cand c = direct_light(S, kk - 2u, T, p, origin, geometric, &state);
+                    Cand c = direct_light(S, kk - 2u, T, p, origin, geometric, dr);
                     if (c.limit < 0.0f) continue;
                     local = c.wi;
                     direction = c.direction;
@@ -1467,17 +1499,18 @@ DEV void shade_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t bo
                 }
             }
             // sample_bsdf (trace.wgsl:1716-1773), diffuse + GGX + coat
-            (void)pfx::rnd(&state);  // transmission pick: never below zero transmission
+            dr.at(3u);
+            (void)dr.w();  // transmission pick: never below zero transmission
             float cw = coat_weight(s, wo);
             float sw = (1.0f - cw) * fm_clamp(s.metalness + s.specular, 0.0f, 0.9f);
-            float pick = pfx::rnd(&state);
+            float pick = dr.z();
             v3 wi;
             uint32_t lobe;
             if (pick < cw + sw) {
                 bool coat = pick < cw;
                 float alpha = alpha_rough(coat ? s.clearcoat_roughness : s.roughness);
-                float u1 = pfx::rnd(&state);
-                float u2 = pfx::rnd(&state);
+                float u1 = dr.x();
+                float u2 = dr.y();
                 float phi = 2.0f * PI * u1;
                 float cos_theta = sqrtf((1.0f - u2) / (1.0f + (alpha * alpha - 1.0f) * u2));
                 float stt = sqrtf(fmaxf(0.0f, 1.0f - cos_theta * cos_theta));
@@ -1485,8 +1518,8 @@ DEV void shade_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t bo
                 wi = reflect(-wo, h);
                 lobe = 1u;
             } else {
-                float u1 = pfx::rnd(&state);
-                float u2 = pfx::rnd(&state);
+                float u1 = dr.x();
+                float u2 = dr.y();
                 float phi = 2.0f * PI * u1;
                 float r = sqrtf(u2);
                 wi = mk(fm_cos(phi) * r, fm_sin(phi) * r, sqrtf(1.0f - u2));
@@ -1507,7 +1540,8 @@ DEV void shade_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t bo
                 o = pfx::spawn_point(p, d, o, t);
                 if (bounce >= 3u) {
                     float keep = fm_clamp(fmaxf(throughput.x, fmaxf(throughput.y, throughput.z)), 0.05f, 0.95f);
-                    if (pfx::rnd(&state) > keep)
+                    dr.at(0u);
+                    if (dr.x() > keep)
                         done = true;
                     else
                         throughput = throughput / keep;
@@ -1518,8 +1552,8 @@ DEV void shade_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t bo
     cont = !done;
     W.o[i] = f4(o, previous_pdf);
     W.d[i] = f4(d, ubits(lobes));
-    W.thr[i] = f4(throughput, ubits(state));
-    W.res[i] = f4(result, 0.0f);
+    W.thr[i] = f4(throughput, ubits(dr.state));
+    W.res[i] = f4(result, ubits(dr.index));
 }
 
 // Queue position k of the bounce (every thread of the block calls it).
@@ -1710,12 +1744,13 @@ __global__ __launch_bounds__(256) void k_fold(WaveDev W, uint32_t n) {
     for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < n; k += gridDim.x * 256u) {
         uint32_t q = W.q_sh[k];
         uint32_t i = q & 0x0fffffffu, m = q >> 28;
-        v3 result = f3(W.res[i]);
+        float4 r4 = W.res[i];
+        v3 result = f3(r4);
         for (uint32_t e = 0; e < m; ++e) {
             float4 c = W.e_c[(size_t)e * W.capacity + i];
             if (c.w == 0.0f) result = result + f3(c);
         }
-        W.res[i] = f4(result, 0.0f);
+        W.res[i] = f4(result, r4.w);  // w: the Sobol sample index
     }
 }
 

@@ -28,6 +28,7 @@ pfxh_status pfxh_learned_denoise_device(pfxh_learner* l, uint32_t width, uint32_
 #include "learn.h"
 hipError_t pfxh_atrous_device(hipStream_t s, float4* a, float4* b, const float4* albedo, const float4* normal,
                               uint32_t w, uint32_t h, uint32_t samples);
+#include "sobol.h"
 #include "vec3.h"
 
 // Idle lanes out of 64 at which a traversal wave fetches new rays; a tuning
@@ -105,6 +106,13 @@ void trace_setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& deta
     S.env_black = h.env_black ? 1u : 0u;
     S.lit = h.lit ? 1u : 0u;
     S.looped = h.looped;
+    if (detail.sampler == PFXH_SAMPLER_SOBOL) {
+        uint32_t len = pfx::sobol_length(detail.sobol_samples);
+        std::vector<float> table((size_t)len * pfx::SOBOL_PATTERNS * 4);
+        pfx::sobol_table(len, table.data());
+        S.sobol_table = t->upload(table, "sobol patterns");
+        S.sobol_len = len;
+    }
     S.area = 0u;
     for (uint32_t i = 0; i < h.looped; ++i) {
         uint32_t kind = (uint32_t)h.lights[16 * i + 3];
@@ -316,7 +324,8 @@ pfxh_status trace_check(const pfxh_scene* scene, const pfxh_detail* detail, uint
         return PFXH_REFUSED;
     }
     if (!projection_valid(detail->projection, detail->ortho_size[0], detail->ortho_size[1]) ||
-        detail->triangle_test > PFXH_TRI_WATERTIGHT || detail->walk_order > PFXH_WALK_LEFT_FIRST) {
+        detail->triangle_test > PFXH_TRI_WATERTIGHT || detail->walk_order > PFXH_WALK_LEFT_FIRST ||
+        detail->sampler > PFXH_SAMPLER_SOBOL) {
         put_err(err, err_len, "invalid trace detail");
         return PFXH_SETUP;
     }
@@ -995,6 +1004,13 @@ pfxh_status pfxh_layout_lights(const pfxh_light* lights, uint32_t count, float* 
 pfxh_status pfxh_layout_select(float* keys, uint32_t* ids, size_t len, size_t index, int mode, size_t* result) {
     if ((len && (!keys || !ids)) || index >= len || mode < 0 || mode > 2 || !result) return PFXH_INVALID;
     *result = select_keys(keys, ids, len, index, mode);
+    return PFXH_OK;
+}
+
+pfxh_status pfxh_layout_sobol(uint32_t samples, float* table, uint32_t* length) {
+    if (!length) return PFXH_INVALID;
+    *length = pfx::sobol_length(samples);
+    if (table) pfx::sobol_table(*length, table);
     return PFXH_OK;
 }
 

@@ -101,6 +101,8 @@ class Detail:
     emitter_sampling: bool = True  # False: emissive triangles only through BSDF sampling (gpu.rs:781)
     watertight: bool = False  # True: the watertight ray-triangle test instead of the reference's
     left_first: bool = False  # True: closest hits walk the reference's left-first order (default nearest first)
+    sampler: str = "pcg"  # "pcg" (Sampler::Pcg) or "sobol" (Sampler::Sobol { samples })
+    sobol_samples: int = 256
 
     def to_c(self) -> N.DetailC:
         d = N.DetailC()
@@ -116,6 +118,8 @@ class Detail:
         d.no_emitter_sampling = 0 if self.emitter_sampling else 1
         d.triangle_test = 1 if self.watertight else 0
         d.walk_order = 1 if self.left_first else 0
+        d.sampler = {"pcg": 0, "sobol": 1}[self.sampler]
+        d.sobol_samples = int(self.sobol_samples)
         return d
 
 

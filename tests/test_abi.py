@@ -35,7 +35,7 @@ def test_struct_sizes_match_the_header():
     assert C.sizeof(N.Camera) == 48
     assert C.sizeof(N.Sun) == 28
     assert C.sizeof(N.Environment) == 16
-    assert C.sizeof(N.DetailC) == 68
+    assert C.sizeof(N.DetailC) == 76
     assert C.sizeof(N.AdaptiveC) == 16
     assert C.sizeof(N.ConvergedC) == 4 + 4 * 64 + 4 + 8 * 64 + 8
     assert C.sizeof(N.SceneC) == 7 * 16 + 16 + 48 + 28 + 4

@@ -1,0 +1,210 @@
+"""The Sobol sampler (crates/trace/src/sobol.rs, sobol.wgsl), pinned three
+ways: a plain-Python restatement of the pattern construction and of the
+per-dimension lookup agrees with the oracle's bit for bit (and the product's
+table with both); the reference's own tests hold (the table is the same
+bytes every time and in [0, 1), pattern 37 is generate_4d(37), every
+power-of-two prefix is stratified in each axis, sequence_size's known
+answers, sobol.rs:155-207); and what the scrambles promise holds (the
+pattern shuffle is a permutation of the 256 patterns, the Owen scramble of
+the sample index permutes every aligned block of 2^k indices)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pfx_amd import _native as N
+
+f32 = np.float32
+M = 0xFFFFFFFF
+
+XORS = [[0] * 32,
+        [0x00000000, 0x00000001, 0x00000001, 0x00000007, 0x00000001, 0x00000013, 0x00000015, 0x0000007f,
+         0x00000001, 0x00000103, 0x00000105, 0x0000070f, 0x00000111, 0x00001333, 0x00001555, 0x00007fff,
+         0x00000001, 0x00010003, 0x00010005, 0x0007000f, 0x00010011, 0x00130033, 0x00150055, 0x007f00ff,
+         0x00010101, 0x01030303, 0x01050505, 0x070f0f0f, 0x01111111, 0x13333333, 0x15555555, 0x7fffffff],
+        [0x00000000, 0x00000001, 0x00000003, 0x00000001, 0x00000005, 0x0000001f, 0x0000002b, 0x0000003d,
+         0x00000011, 0x00000133, 0x00000377, 0x00000199, 0x00000445, 0x00001ccf, 0x00002ddb, 0x0000366d,
+         0x00000101, 0x00010303, 0x00030707, 0x00010909, 0x00051515, 0x001f3f3f, 0x002b6b6b, 0x003dbdbd,
+         0x00101011, 0x01303033, 0x03707077, 0x01909099, 0x04515145, 0x1cf3f3cf, 0x2db6b6db, 0x36dbdb6d],
+        [0x00000000, 0x00000001, 0x00000000, 0x00000003, 0x0000000d, 0x0000000c, 0x00000005, 0x0000004f,
+         0x00000014, 0x000000e7, 0x00000329, 0x0000039c, 0x00000011, 0x00001033, 0x00000044, 0x000030bb,
+         0x0000d1cd, 0x0000c2ec, 0x00005415, 0x0004fc3f, 0x00015054, 0x000e5c97, 0x0032e5b9, 0x0039725c,
+         0x00000101, 0x01000303, 0x00000404, 0x03000b0b, 0x0d001d1d, 0x0c002c2c, 0x05004545, 0x4f00cfcf]]
+
+
+def hash_hp(i):  # sobol.rs:72-79
+    i ^= i >> 16
+    i = (i * 0x21F0AAAD) & M
+    i ^= i >> 15
+    i = (i * 0xD35A2D97) & M
+    i ^= i >> 15
+    return i ^ 0xE6FE3BEB
+
+
+def generate_4d(size, seed):  # sobol.rs:106-135
+    s = 0x44605A73
+    s = (s ^ (s << 19)) & M
+    rng = [hash_hp(seed ^ s)]
+    inv = f32(1.0) / f32(4294967808.0)
+
+    def nxt():
+        v = f32(hash_hp(rng[0])) * inv
+        rng[0] = (rng[0] + 1) & M
+        return v
+
+    pts = np.zeros((size, 4), f32)
+    pts[0] = [nxt() for _ in range(4)]
+    log_n, n = 0, 1
+    while n < size:
+        strata = f32(n * 2)
+        i = 0
+        while i < n and n + i < size:
+            for axis in range(4):
+                occupied = int(pts[i ^ XORS[axis][log_n], axis] * strata)
+                pts[n + i, axis] = (f32(occupied ^ 1) + nxt()) / strata
+            i += 1
+        log_n += 1
+        n *= 2
+    return pts
+
+
+def rev(v):
+    return int(f"{v:032b}"[::-1], 2)
+
+
+def owen(n, seed):  # sobol.wgsl:2-9
+    n ^= (n * 0x3D20ADEA) & M
+    n = (n + seed) & M
+    n = (n * ((seed >> 16) | 1)) & M
+    n ^= (n * 0x05526C56) & M
+    n ^= (n * 0x53A22864) & M
+    return n
+
+
+def scramble(i, seed):
+    return rev(owen(rev(i), seed))
+
+
+def wang(i, seed):  # :13-19
+    i = (i ^ 61) ^ seed
+    i = (i + (i << 3)) & M
+    i ^= i >> 4
+    return (i * 0x27D4EB2D) & M
+
+
+def shuffle(i, length, seed):  # :20-43
+    i %= length
+    mask = (1 << (32 - (32 - (length - 1).bit_length()))) - 1
+    while True:
+        i ^= seed
+        i = (i * 0xE170893D) & M
+        i ^= seed >> 16
+        i ^= (i & mask) >> 4
+        i ^= seed >> 8
+        i = (i * 0x0929EB3F) & M
+        i ^= seed >> 23
+        i ^= (i & mask) >> 1
+        i = (i * (1 | (seed >> 27))) & M
+        i = (i * 0x6935FA69) & M
+        i ^= (i & mask) >> 11
+        i = (i * 0x74DCB303) & M
+        i ^= (i & mask) >> 2
+        i = (i * 0x9E501CC3) & M
+        i ^= (i & mask) >> 2
+        i = (i * 0xC860A3DF) & M
+        i &= mask
+        i ^= i >> 5
+        if i < length:
+            return i
+
+
+def sobol_point(table, length, seed, index, dim):  # :44-55
+    pattern = shuffle(dim, 256, seed)
+    mask = length - 1
+    shuffled = scramble(index, wang(dim, seed))
+    sample = (index & ~mask & M) | (shuffled & mask)
+    at = ((pattern * length + sample) & M) % (length * 256)
+    shift = [f32(wang(dim, seed ^ x)) * f32(2.3283064e-10) for x in (0, 0xCA0E1151, 0xBF604C5A, 0x99634D1D)]
+    out = []
+    for k in range(4):
+        v = table[at, k] + shift[k]
+        out.append(min(f32(v - f32(np.floor(v))), f32(0.99999994)))
+    return np.array(out, f32)
+
+
+def oracle_pattern(size, seed):
+    out = np.zeros((size, 4), f32)
+    O.lib().or_sobol_pattern(out.ctypes.data, size, seed)
+    return out
+
+
+def product_table(samples):
+    n = C.c_uint32()
+    assert N.lib().pfxh_layout_sobol(samples, None, C.byref(n)) == 0
+    t = np.zeros((256 * n.value, 4), f32)
+    assert N.lib().pfxh_layout_sobol(samples, t.ctypes.data, C.byref(n)) == 0
+    return t, n.value
+
+
+def test_sequence_size_known_answers():
+    """sobol.rs:196-206"""
+    for samples, want in ((1, 256), (300, 512), (4096, 4096), (2**32 - 1, 8192), (0, 256)):
+        n = C.c_uint32()
+        assert N.lib().pfxh_layout_sobol(samples, None, C.byref(n)) == 0
+        assert n.value == want
+
+
+@pytest.mark.parametrize("size,seed", [(256, 0), (256, 37), (1024, 5), (512, 255)])
+def test_patterns_equal_the_restatement(size, seed):
+    np.testing.assert_array_equal(oracle_pattern(size, seed), generate_4d(size, seed))
+
+
+def test_the_table_is_the_same_bytes_every_time():
+    """sobol.rs:158-172: 256 x 256 points in [0, 1), deterministic, row 37
+    is generate_4d(37); the product's table equals the oracle's patterns."""
+    a, n = product_table(256)
+    b, _ = product_table(256)
+    assert a.shape == (256 * 256, 4) and a.tobytes() == b.tobytes()
+    assert np.all((a >= 0.0) & (a < 1.0))
+    np.testing.assert_array_equal(a[37 * 256:38 * 256], oracle_pattern(256, 37))
+    for p in (0, 1, 128, 255):
+        np.testing.assert_array_equal(a[p * 256:(p + 1) * 256], oracle_pattern(256, p))
+
+
+def test_every_power_of_two_prefix_is_stratified_in_each_axis():
+    """sobol.rs:174-188"""
+    pts = oracle_pattern(1024, 5)
+    for n in (2, 16, 128, 1024):
+        for axis in range(4):
+            cells = (pts[:n, axis] * f32(n)).astype(np.int64)
+            assert len(set(cells.tolist())) == n, (n, axis)
+
+
+def test_lookup_equals_the_restatement():
+    table, n = product_table(256)
+    rng = np.random.default_rng(9)
+    out = np.zeros(4, f32)
+    for _ in range(600):
+        seed = int(rng.integers(0, 2**32))
+        index = int(rng.integers(0, 5000))
+        dim = int(rng.choice([0, 1, 3, 4, 5, 16, 17, 19, 21, 32, 48, 13 << 16]))
+        O.lib().or_sobol_point(table.ctypes.data, n, seed, index, dim, out.ctypes.data)
+        np.testing.assert_array_equal(out, sobol_point(table, n, seed, index, dim))
+        assert np.all((out >= 0) & (out < 1))
+
+
+def test_the_scrambles_are_permutations():
+    """The pattern shuffle maps the 256 dimensions of a seed one to one onto
+    the 256 patterns (cycling every 256 dimensions), and the Owen scramble
+    permutes every aligned block of 2^k sample indices (so the first
+    2^k samples of a pixel read 2^k distinct points of the pattern)."""
+    rng = np.random.default_rng(10)
+    for _ in range(8):
+        seed = int(rng.integers(0, 2**32))
+        assert sorted(shuffle(d, 256, seed) for d in range(256)) == list(range(256))
+        for k in (1, 4, 8):
+            block = 1 << k
+            base = int(rng.integers(0, 1000)) * block
+            got = sorted(scramble(i, seed) & (block - 1) for i in range(base, base + block))
+            assert got == list(range(block))
