@@ -62,9 +62,35 @@ REF_NODE_BYTES, REF_PRIM_BYTES = 48, 96
 CHECK_THRESHOLD, CHECK_TILE = 1.0, 32
 # samples per pixel of the fixed passes that compare the two triangle tests
 FIXED_SPP = 8
-# HBM bytes per closest-hit ray measured with rocprofv3 counters (FETCH_SIZE
-# and WRITE_SIZE of k_extend, calibrated on a copy kernel); absent -> null
-TRAFFIC_FILE = ROOT / "profiles" / "k_extend_traffic.json"
+# Hardware counters of the closest-hit kernel on this workload, from
+# rocprofv3 passes (HBM bytes per ray from FETCH_SIZE and WRITE_SIZE
+# calibrated on a copy kernel, VALU instructions, lane utilisation), keyed
+# to a hash of the kernel sources: a file made from other sources is stale
+# and its figures are left out (null).
+COUNTER_FILE = ROOT / "profiles" / "k_extend_counters.json"
+KERNEL_SOURCES = ["kernels.hip", "kernels.h", "device_types.h", "vec3.h", "fmath.h", "sobol.h"]
+CLOCK_GHZ, CUS = 2.4, 256  # MI355X peak engine clock and compute units
+# without packed FP32 (this build turns it off) a CU issues at most one
+# 64-lane VALU instruction per cycle
+VALU_ISSUE_PEAK = 1.0
+
+
+def kernel_hash() -> str:
+    import hashlib
+
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        h.update((ROOT / "pfx_amd" / "csrc" / name).read_bytes())
+    return h.hexdigest()[:16]
+
+
+def counters() -> dict | None:
+    """The counter file's figures when they were measured on these kernel
+    sources, else None."""
+    if not COUNTER_FILE.exists():
+        return None
+    c = json.loads(COUNTER_FILE.read_text())
+    return c if c.get("kernel_sources_sha16") == kernel_hash() else None
 
 
 def log(msg: str) -> None:
@@ -118,6 +144,7 @@ def cpu_baseline(scene, seconds_target: float = 15.0) -> dict:
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
+        "label": "repo CPU oracle",
         "sample": f"rows {y0}-{y0 + rows} of the 3840x2160 frame at {spp} spp (fixed), {rays} rays in {dt:.1f} s",
     }
 
@@ -369,10 +396,22 @@ def main() -> int:
     # achievable HBM bandwidth on this GPU: the library's streaming copy of 1 GiB
     achievable = hbm_probe(1 << 30, 5)
     log(f"streaming copy: {achievable:.0f} GB/s (read + write)")
-    traffic = None
-    if TRAFFIC_FILE.exists():
-        measured = json.loads(TRAFFIC_FILE.read_text())
+    measured = counters()
+    traffic, valu, limiter = None, None, None
+    if measured:
         traffic = round(measured["hbm_bytes_per_ray"] * rays_per_launch)
+        launch_s = avg_launch_ms * 1e-3
+        hbm_util = traffic / launch_s / (HBM_PEAK_GBS * 1e9)
+        issue = measured["valu_wave_instr_per_ray"] * rays_per_launch / (launch_s * CLOCK_GHZ * 1e9 * CUS)
+        valu = {"wave_instr_per_ray": measured["valu_wave_instr_per_ray"],
+                "issue_per_cu_cycle": round(issue, 3),
+                "issue_frac": round(issue / VALU_ISSUE_PEAK, 3),
+                "lane_utilisation": measured["valu_lane_utilisation"],
+                "l2_hit_rate": measured.get("l2_hit_rate")}
+        # the measured resource nearest its ceiling bounds the kernel
+        limiter = "valu-issue" if issue / VALU_ISSUE_PEAK > hbm_util else "hbm"
+    else:
+        log(f"{COUNTER_FILE.name} missing or measured on other kernel sources: traffic and VALU figures omitted")
     result = {
         "metric": "Mrays/s; 4K ms to relMSE≤0.01: 4spp+learned denoise vs plain 64spp",
         "value": round(mrays, 2),
@@ -398,7 +437,11 @@ def main() -> int:
             "shadow_rays": s["shadow_rays"],
         },
         "roofline": {
+            # the roofline family: traversal does no matrix work, so its
+            # roofline is HBM bandwidth; what the counters show binds the
+            # kernel is `limiter`
             "bound": "hbm",
+            "limiter": limiter,
             "kernel": "k_extend (closest hit)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
@@ -409,6 +452,7 @@ def main() -> int:
             "achievable_probe": "best of 5 device-to-device streaming copies of 1 GiB at each of 2, 4 and 8 "
                                 "blocks per CU (pfxh_hbm_probe, nontemporal float4), read + write bytes over time",
             "traffic": traffic,
+            "valu": valu,
             "bytes_per_ray": round(bytes_per_ray, 1),
             "reference_layout_bytes_per_ray": round(ref_bytes_per_ray, 1),
             "reference_layout_gbs": round(ref_bytes_per_ray * rays_per_launch / (avg_launch_ms * 1e-3) / 1e9, 1),

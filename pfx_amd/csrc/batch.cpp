@@ -33,6 +33,7 @@ struct pfxh_batch {
     uint32_t *qa = nullptr, *qb = nullptr, *qh = nullptr, *active = nullptr;
     uint32_t capacity = 0, max_pixels = 0, total_bounces = 0, cus = 256;
     bool left_first = false;  // closest-hit walks in the reference's left-first order
+    bool sobol = false;       // the stills use the Sobol sampler
     SceneDev* scenes = nullptr;
     AccumDev* accums = nullptr;
     WavePlan* waves = nullptr;
@@ -146,7 +147,7 @@ void record_wave(pfxh_batch* b, const pfxh_batch_tuning& t) {
         launch_roll(s, B, Wb, 1, bounce);
         // only paths that hit, or whose still has a sky, need shading
         launch_hits_batch(s, B, Wb, queue, b->qh, flat_grid);
-        launch_shade_batch(s, B, Wb, b->qh, bounce, shade_grid, Wb.counters + C_HITS);
+        launch_shade_batch(s, B, Wb, b->qh, bounce, shade_grid, Wb.counters + C_HITS, b->sobol);
         launch_roll(s, B, Wb, 2, bounce);
         launch_shadow_batch(s, B, Wb, trace_grid);
         launch_fold_batch(s, Wb, flat_grid);
@@ -258,6 +259,7 @@ pfxh_status pfxh_batch_new(const pfxh_batch_still* stills, uint32_t count, const
         }
         b->total_bounces = detail->bounces.total;
         b->left_first = detail->walk_order == PFXH_WALK_LEFT_FIRST;
+        b->sobol = detail->sampler == PFXH_SAMPLER_SOBOL;
         // the wave: at least the largest still once, at most 2^25 paths
         size_t cap = std::max<size_t>(b->max_pixels, std::min<size_t>((size_t)1 << 25, std::max<uint64_t>(work, 1)));
         if (cap >= ((size_t)1 << 28)) throw DeviceError{PFXH_REFUSED, "a still of more than 2^28 pixels"};

@@ -168,7 +168,7 @@ void launch_hits_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const
                        uint32_t grid);
 // Shades the wave's queue (length on the device), or queue[0, *count) when count is given.
 void launch_shade_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t bounce,
-                        uint32_t grid, const uint32_t* count = nullptr);
+                        uint32_t grid, const uint32_t* count, bool sobol);
 void launch_shadow_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid);
 void launch_fold_batch(hipStream_t s, const WaveDev& W, uint32_t grid);
 void launch_finish_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid);

@@ -1380,13 +1380,15 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
 }
 
 // One path vertex (trace.wgsl:2169-2501 loop body).
+// SOBOL: the scene uses the Sobol sampler (with PCG the sampler code compiles out)
+template <bool SOBOL>
 DEV void shade_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t bounce, bool& cont, uint32_t& entries,
                     uint32_t& tested) {
     float4 o4 = W.o[i], d4 = W.d[i], t4 = W.thr[i], r4 = W.res[i], hv = W.hit[i];
     v3 o = f3(o4), d = f3(d4), throughput = f3(t4), result = f3(r4);
     float previous_pdf = o4.w;
     uint32_t lobes = fbits(d4.w);
-    Draw dr{fbits(t4.w), fbits(r4.w), 16u * (bounce + 1u), S.sobol_table, S.sobol_len, {}};
+    Draw dr{fbits(t4.w), fbits(r4.w), 16u * (bounce + 1u), SOBOL ? S.sobol_table : nullptr, S.sobol_len, {}};
     float t = hv.x;
     bool done = false;
     if (S.area) result = result + throughput * light_hits(S, o, d, t, pfx::spawn_point(o, -d, o, 0.0f), previous_pdf);
@@ -1557,7 +1559,7 @@ DEV void shade_path(const SceneDev& S, const WaveDev& W, uint32_t i, uint32_t bo
 }
 
 // Queue position k of the bounce (every thread of the block calls it).
-template <bool BATCH>
+template <bool BATCH, bool SOBOL>
 DEV void shade_step(const SceneDev& S, const SceneDev* __restrict__ scenes, const WaveDev& W, const uint32_t* queue, uint32_t n,
                     uint32_t bounce, uint32_t k) {
     bool valid = k < n;
@@ -1575,13 +1577,13 @@ DEV void shade_step(const SceneDev& S, const SceneDev* __restrict__ scenes, cons
         }
         const uint32_t s0 = __builtin_amdgcn_readfirstlane(sid);
         if (__all(!valid || sid == s0)) {
-            if (valid) shade_path(scenes[s0], W, i, bounce, cont, entries, tested);
+            if (valid) shade_path<SOBOL>(scenes[s0], W, i, bounce, cont, entries, tested);
         } else if (valid) {
-            shade_path(scenes[sid], W, i, bounce, cont, entries, tested);
+            shade_path<SOBOL>(scenes[sid], W, i, bounce, cont, entries, tested);
         }
     } else if (valid) {
         i = queue ? queue[k] : k;
-        shade_path(S, W, i, bounce, cont, entries, tested);
+        shade_path<SOBOL>(S, W, i, bounce, cont, entries, tested);
     }
     bool sh = valid && entries > 0u;
     uint3 at = block_append3(W.counters, cont ? 1u : 0u, sh ? 1u : 0u, (uint32_t)__popc(tested));
@@ -1668,18 +1670,18 @@ __global__ __launch_bounds__(256) void k_hits_batch(BatchDev B, WaveDev W, const
     compact_chunk(lo, hi, queue, keep, W.counters + C_HITS, hitq);
 }
 
-template <bool BATCH>
+template <bool BATCH, bool SOBOL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(SceneDev S, const SceneDev* __restrict__ scenes, WaveDev W, const uint32_t* queue,
                                                uint32_t n, uint32_t bounce, const uint32_t* count) {
     if (!BATCH) {
         if (count) n = *count;
         if (blockIdx.x * 256u >= n) return;  // the whole block is past the queue
-        shade_step<false>(S, scenes, W, queue, n, bounce, blockIdx.x * 256u + threadIdx.x);
+        shade_step<false, SOBOL>(S, scenes, W, queue, n, bounce, blockIdx.x * 256u + threadIdx.x);
         return;
     }
     n = count ? *count : W.counters[C_N];
     for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u)
-        shade_step<true>(S, scenes, W, queue, n, bounce, base + threadIdx.x);
+        shade_step<true, SOBOL>(S, scenes, W, queue, n, bounce, base + threadIdx.x);
 }
 
 // Any-hit test of every queued shadow ray (one NEE entry each). An occluded
@@ -2106,8 +2108,12 @@ void launch_shade(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint
                   uint32_t bounce, const uint32_t* count) {
     if (!n) return;
     const SceneDev* none = nullptr;
-    hipLaunchKernelGGL(k_shade<false>, dim3(blocks_for(n, 256)), dim3(256), 0, s, S, none, W, queue, n, bounce,
-                       count);
+    if (S.sobol_table)
+        hipLaunchKernelGGL((k_shade<false, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s, S, none, W, queue, n,
+                           bounce, count);
+    else
+        hipLaunchKernelGGL((k_shade<false, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s, S, none, W, queue, n,
+                           bounce, count);
 }
 void launch_shadow(hipStream_t s, const SceneDev& S, const WaveDev& W, uint32_t n, bool count, uint32_t grid) {
     if (!n) return;
@@ -2205,9 +2211,14 @@ void launch_hits_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const
     hipLaunchKernelGGL(k_hits_batch, dim3(grid), dim3(256), 0, s, B, W, queue, hitq);
 }
 void launch_shade_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t bounce,
-                        uint32_t grid, const uint32_t* count) {
+                        uint32_t grid, const uint32_t* count, bool sobol) {
     SceneDev none{};
-    hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, s, none, B.scenes, W, queue, 0u, bounce, count);
+    if (sobol)
+        hipLaunchKernelGGL((k_shade<true, true>), dim3(grid), dim3(256), 0, s, none, B.scenes, W, queue, 0u, bounce,
+                           count);
+    else
+        hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(256), 0, s, none, B.scenes, W, queue, 0u, bounce,
+                           count);
 }
 void launch_shadow_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid) {
     SceneDev none{};
