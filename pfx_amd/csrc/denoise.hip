@@ -11,7 +11,8 @@
 //                2x2 max pool are fused into the epilogue; nearest
 //                upsampling and concatenation are fused into the staging.
 //   k_conv_bf16  the same convolution with BF16 operands for the learned
-//                denoiser (v_mfma_f32_16x16x16_bf16, FP32 sums): inputs
+//                denoiser (v_mfma_f32_16x16x16_bf16; k_conv_bf16x32 with
+//                gfx950's v_mfma_f32_16x16x32_bf16 for narrow layers; FP32 sums): inputs
 //                rounded to BF16 once as staged, two rows per wave for
 //                layers up to 64 outputs.
 //   k_bins / k_scale   autoexposure (process.wgsl:104-157)
@@ -255,6 +256,143 @@ __global__ __launch_bounds__(256) void k_conv_bf16(ConvDev c) {
         }
 }
 
+// The same with gfx950's K = 32 matrix-core op
+// (v_mfma_f32_16x16x32_bf16, FP32 sums): the K loop takes the input
+// channels 32 at a time (two 16-channel chunks of the packed weights; an
+// odd last chunk pairs with zeros), each lane holding 8 consecutive channels
+// of one pixel (A) and of one output's weights (B). A block computes 4 *
+// ROWS rows x 16 pixels, each wave ROWS rows, so with ROWS = 2 every weight
+// fragment read from memory feeds two matrix-core ops and the 2x2 pool sits
+// inside a wave. Used for layers of up to 4 N tiles (two rows per wave):
+// there it runs 9-16% faster than K = 16 at 4K; for the wide layers of one
+// row per wave, whose deep levels launch few blocks, K = 16 stays faster.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ v8bf as_v8bf(uint4 u) { return __builtin_bit_cast(v8bf, u); }
+
+template <int NT, int ROWS>
+__global__ __launch_bounds__(256) void k_conv_bf16x32(ConvDev c) {
+    constexpr int TR = 4 * ROWS, PR = TR + 2;
+    __shared__ uint4 patch[PR * PW * 4];  // [row][col][32 channels: 4 x 8 BF16]
+    __shared__ float pooled[2][NT][64][2];  // ROWS = 1: odd rows' x-pooled values
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 15, kk = lane >> 4;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TR;
+    const int h = (int)c.h, w = (int)c.w_;
+    const int sw = c.upsample ? w / 2 : w;
+    {
+        const size_t z = blockIdx.z, px = (size_t)h * w;
+        const size_t spx = c.upsample ? (size_t)(h / 2) * sw : px;
+        c.src0 += z * spx * c.c0;
+        if (c.src1) c.src1 += z * px * c.c1;
+        c.dst += z * (c.pool ? px / 4 : px) * c.n_pad;
+    }
+    v4f acc[ROWS][NT];
+    for (int r = 0; r < ROWS; ++r)
+        for (int t = 0; t < NT; ++t) acc[r][t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    const uint32_t cin = c.c0 + c.c1, chunks = cin / 16u, pairs = (chunks + 1u) / 2u;
+    for (uint32_t g = 0; g < pairs; ++g) {
+        __syncthreads();
+        // each item is 8 channels of one pixel: two float4 loads, one uint4 store
+        for (int e = tid; e < PR * PW * 4; e += 256) {
+            int q = e & 3, p = e >> 2;
+            int py = p / PW, px = p - py * PW;
+            int y = y0 + py - 1, x = x0 + px - 1;
+            uint32_t cc = g * 32u + (uint32_t)q * 8u;
+            uint4 bits = make_uint4(0u, 0u, 0u, 0u);
+            if (y >= 0 && y < h && x >= 0 && x < w && cc < cin) {
+                const float* at;
+                if (cc < c.c0) {
+                    int yy = c.upsample ? y >> 1 : y, xx = c.upsample ? x >> 1 : x;
+                    at = c.src0 + ((size_t)yy * sw + xx) * c.c0 + cc;
+                } else {
+                    at = c.src1 + ((size_t)y * w + x) * c.c1 + (cc - c.c0);
+                }
+                float4 v0 = *reinterpret_cast<const float4*>(at), v1 = *reinterpret_cast<const float4*>(at + 4);
+                uint2 lo = bf16x4_bits(v0), hi = bf16x4_bits(v1);
+                bits = make_uint4(lo.x, lo.y, hi.x, hi.y);
+            }
+            patch[e] = bits;
+        }
+        __syncthreads();
+        // lane group kk takes channels 8 kk .. 8 kk + 7 of the pair: chunk
+        // 2 g + kk / 2, its half kk % 2; a chunk past the last is zero
+        const uint32_t chunk = 2u * g + (uint32_t)(kk >> 1);
+        const bool live = chunk < chunks;
+        const uint16_t* wb = static_cast<const uint16_t*>(c.w) + (size_t)(live ? chunk : 0u) * 9u * c.n_pad * 16u +
+                             (size_t)(kk & 1) * 8u;
+#pragma unroll 3
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap - ky * 3;
+            v8bf a[ROWS];
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r) a[r] = as_v8bf(patch[((ROWS * wave + r + ky) * PW + i + kx) * 4 + kk]);
+            const uint16_t* wt = wb + (size_t)tap * c.n_pad * 16u;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                uint4 braw = live ? *reinterpret_cast<const uint4*>(wt + (size_t)(16 * t + i) * 16u)
+                                  : make_uint4(0u, 0u, 0u, 0u);
+                const v8bf b = as_v8bf(braw);
+#pragma unroll
+                for (int r = 0; r < ROWS; ++r)
+                    acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b, acc[r][t], 0, 0, 0);
+            }
+        }
+    }
+    // D[pixel 4*kk + v][channel i] of each N tile, rows y0 + ROWS * wave + r
+    if (!c.pool) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int n = 16 * t + i;
+            const float bias = c.bias ? c.bias[n] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r) {
+                const int y = y0 + ROWS * wave + r;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int x = x0 + 4 * kk + v;
+                    const float o = acc[r][t][v] + bias;
+                    if (y < h && x < w) c.dst[((size_t)y * w + x) * c.n_pad + n] = c.relu ? fmaxf(o, 0.0f) : o;
+                }
+            }
+        }
+        return;
+    }
+    // 2x2 max pool after the ReLU: x pairs (v 0-1, 2-3) sit in one lane; the
+    // row pair sits in one wave (ROWS = 2) or in waves (0,1), (2,3)
+    float m[NT][2];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const float bias = c.bias ? c.bias[16 * t + i] : 0.0f;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            float v = fmaxf(acc[0][t][2 * p] + bias, acc[0][t][2 * p + 1] + bias);
+            if (ROWS == 2) v = fmaxf(v, fmaxf(acc[ROWS - 1][t][2 * p] + bias, acc[ROWS - 1][t][2 * p + 1] + bias));
+            m[t][p] = fmaxf(v, 0.0f);
+        }
+    }
+    int oy;
+    if (ROWS == 1) {
+        if (wave & 1)
+            for (int t = 0; t < NT; ++t)
+                for (int p = 0; p < 2; ++p) pooled[wave >> 1][t][lane][p] = m[t][p];
+        __syncthreads();
+        if (wave & 1) return;
+        for (int t = 0; t < NT; ++t)
+            for (int p = 0; p < 2; ++p) m[t][p] = fmaxf(m[t][p], pooled[wave >> 1][t][lane][p]);
+        oy = (y0 + wave) / 2;
+    } else {
+        oy = (y0 + 2 * wave) / 2;
+    }
+    const int ho = h / 2, wo = w / 2;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int ox = x0 / 2 + 2 * kk + p;
+            if (oy < ho && ox < wo) c.dst[((size_t)oy * wo + ox) * c.n_pad + 16 * t + i] = m[t][p];
+        }
+}
+
 static void launch_conv_fp32(hipStream_t s, const ConvDev& c, dim3 grid) {
     switch (c.n_pad / 16u) {
 #define PFXD_CONV(n) \
@@ -270,7 +408,12 @@ static void launch_conv_fp32(hipStream_t s, const ConvDev& c, dim3 grid) {
 static void launch_conv_bf16(hipStream_t s, const ConvDev& c, dim3 grid) {
     switch (c.n_pad / 16u) {
 #define PFXD_CONV(n) \
-    case n: hipLaunchKernelGGL((k_conv_bf16<n, (n <= 4 ? 2 : 1)>), grid, dim3(256), 0, s, c); break;
+    case n:                                                                                      \
+        if (n <= 4)                                                                              \
+            hipLaunchKernelGGL((k_conv_bf16x32<n, 2>), grid, dim3(256), 0, s, c);                \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_conv_bf16<n, 1>), grid, dim3(256), 0, s, c);                   \
+        break;
         PFXD_CONV(1) PFXD_CONV(2) PFXD_CONV(3) PFXD_CONV(4) PFXD_CONV(5) PFXD_CONV(6)
         PFXD_CONV(7) PFXD_CONV(8) PFXD_CONV(9) PFXD_CONV(10) PFXD_CONV(11) PFXD_CONV(12)
         PFXD_CONV(13) PFXD_CONV(14)
