@@ -292,6 +292,27 @@ def watertight_leg(scene, mt_probe, mt_pass) -> dict:
     return out
 
 
+def walk_order_leg(scene, near_probe, near_pass) -> dict:
+    """The reference's left-first closest-hit walk beside the default
+    nearest-first walk on the same 4K still: primary rays whose triangle
+    differs, and the cost of a fixed-spp pass in each order."""
+    t = time.perf_counter()
+    with Trace.new_detailed(scene, Detail(bounces=BOUNCES, left_first=True), WIDTH, HEIGHT) as lf:
+        log(f"left-first trace built in {time.perf_counter() - t:.1f} s")
+        probe = lf.probe()
+        lf_pass = fixed_pass(lf, scene.camera, FIXED_SPP, 5)
+    a, b = near_probe.primitive, probe.primitive
+    out = {
+        "primary_rays": int(a.size),
+        "primitive_differs": int(np.sum(a != b)),
+        "spp": FIXED_SPP,
+        "nearest_first": near_pass,
+        "left_first": lf_pass,
+    }
+    log(f"walk order: {out}")
+    return out
+
+
 def batch_leg() -> dict:
     """Uneven stills in one batch (pfx_amd.batch): 32 stills of seeded
     sphere-and-terrain scenes (10k-500k triangles, 1-64 lights), sizes drawn
@@ -347,7 +368,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-quality", action="store_true", help="skip the image-quality leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-stills leg")
-    ap.add_argument("--no-watertight", action="store_true", help="skip the watertight-test leg")
+    ap.add_argument("--no-watertight", action="store_true",
+                    help="skip the watertight-test and walk-order legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); across GPUs the still would be "
@@ -467,6 +489,7 @@ def main() -> int:
     trace.close()
     if not args.no_watertight:
         result["watertight_test"] = watertight_leg(scene, mt_probe, mt_pass)
+        result["walk_order"] = walk_order_leg(scene, mt_probe, mt_pass)
     if not args.no_quality:
         result["quality"] = quality_leg(scene)
     if not args.no_batch:
