@@ -461,6 +461,11 @@ pfxh_status pfxh_layout_select(float* keys, uint32_t* ids, size_t len, size_t in
  * (sobol.rs:33-39); with table non-NULL, the 256 patterns, 4 floats a
  * point, pattern-major (256 * length * 4 floats). */
 pfxh_status pfxh_layout_sobol(uint32_t samples, float* table, uint32_t* length);
+/* sobol::animated_seed (crates/trace/src/sobol.rs:105-107), what
+ * still::Seed::Animated(frame).of(seed) (crates/still/src/trace.rs:77-82)
+ * passes to Trace::sample for one frame of an animation: Cycles'
+ * hash_uint2 (lookup3) of (frame, seed). */
+uint32_t pfxh_sobol_animated_seed(uint32_t seed, uint32_t frame);
 
 /* lights::build_emitters (crates/trace/src/lights.rs:391-440) for the
  * emissive triangles of a scene: entries (16 floats each, `GpuEmitter`,

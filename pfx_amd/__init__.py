@@ -7,7 +7,7 @@ workload generators.
 """
 from .scene import (EQUIRECT, ORTHOGRAPHIC, PERSPECTIVE, Adaptive, Bounces, Camera, Detail, Scene, Sun, material,
                     point_light, spot_light)
-from .trace import Converged, Output, Probe, Trace, TraceError
+from .trace import Converged, Output, Probe, Trace, TraceError, animated_seed
 
 __all__ = [
     "EQUIRECT",
@@ -24,6 +24,7 @@ __all__ = [
     "Sun",
     "Trace",
     "TraceError",
+    "animated_seed",
     "material",
     "point_light",
     "spot_light",

@@ -249,6 +249,7 @@ SIGNATURES = {
     "pfxh_learner_set_loss": (C.c_int, [_P, _U]),
     "pfxh_stage_build": (C.c_int, [_P, C.POINTER(_P), _S, _Z]),
     "pfxh_layout_sobol": (C.c_int, [_U, _P, C.POINTER(_U)]),
+    "pfxh_sobol_animated_seed": (_U, [_U, _U]),
     "pfxh_staged_scene": (_P, [_P]),
     "pfxh_staged_detail": (C.c_int, [_P, _P]),
     "pfxh_staged_trace": (C.c_int, [_P, _P, _U, _U, C.POINTER(_P), _S, _Z]),

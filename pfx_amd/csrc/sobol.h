@@ -93,6 +93,28 @@ PFX_HD uint32_t hash_iqnt2d(uint32_t x, uint32_t y) {
 }
 PFX_HD float fract(float x) { return x - floorf(x); }
 
+PFX_HD uint32_t rotl32(uint32_t v, uint32_t k) { return (v << k) | (v >> (32u - k)); }
+// hash_uint2 (sobol.rs:84-103): Bob Jenkins' lookup3 hash of two words
+// (hashword with length 2 and initial value 13, as Cycles' hash_uint2):
+// a = b = c = 0xdeadbeef + (2 << 2) + 13, then b += ky, a += kx, then
+// lookup3's `final` mix.
+PFX_HD uint32_t hash_uint2(uint32_t kx, uint32_t ky) {
+    uint32_t a = 0xdeadbeefu + (2u << 2) + 13u;
+    uint32_t b = a + ky, c = a;
+    a += kx;
+    c ^= b; c -= rotl32(b, 14);
+    a ^= c; a -= rotl32(c, 11);
+    b ^= a; b -= rotl32(a, 25);
+    c ^= b; c -= rotl32(b, 16);
+    a ^= c; a -= rotl32(c, 4);
+    b ^= a; b -= rotl32(a, 14);
+    c ^= b; c -= rotl32(b, 24);
+    return c;
+}
+// animated_seed (sobol.rs:105-107): the trace seed of frame `frame` of an
+// animation rendered with `seed` (still::Seed::Animated, still/src/trace.rs:77-82)
+PFX_HD uint32_t animated_seed(uint32_t seed, uint32_t frame) { return hash_uint2(frame, seed); }
+
 // sobol (sobol.wgsl:44-55): the point of dimension `dim` (rng.z + the
 // dimension) for pixel seed `seed` (rng.x) and sample `index` (rng.y)
 struct Sobol4 {

@@ -1014,6 +1014,8 @@ pfxh_status pfxh_layout_sobol(uint32_t samples, float* table, uint32_t* length) 
     return PFXH_OK;
 }
 
+uint32_t pfxh_sobol_animated_seed(uint32_t seed, uint32_t frame) { return pfx::animated_seed(seed, frame); }
+
 pfxh_status pfxh_layout_emitters(const pfxh_scene* scene, float* entries, float* cdf, uint32_t* emitter_count,
                                  char* err, size_t err_len) {
     if (!scene || !emitter_count) return PFXH_INVALID;

@@ -324,3 +324,10 @@ def hbm_probe(nbytes: int = 1 << 30, reps: int = 5) -> float:
     if st != 0:
         raise TraceError(err.value.decode(errors="replace"))
     return float(gbs.value)
+
+
+def animated_seed(seed: int, frame: int) -> int:
+    """sobol::animated_seed (crates/trace/src/sobol.rs:105-107): the seed an
+    animation rendered with `seed` passes to `sample` for frame `frame`
+    (still::Seed::Animated, crates/still/src/trace.rs:77-82)."""
+    return int(N.lib().pfxh_sobol_animated_seed(int(seed) & 0xFFFFFFFF, int(frame) & 0xFFFFFFFF))

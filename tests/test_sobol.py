@@ -155,6 +155,18 @@ def test_sequence_size_known_answers():
         assert n.value == want
 
 
+def test_the_animated_seed_is_lookup3_of_frame_and_seed():
+    """sobol.rs:189-197: hash_uint2(0, 0), hash_uint2(1, 0) and
+    animated_seed(7, 12) known answers (animated_seed(seed, frame) is
+    hash_uint2(frame, seed)), and 240 frames of one seed are distinct."""
+    from pfx_amd import animated_seed
+
+    assert animated_seed(0, 0) == 0xDC3D74F9
+    assert animated_seed(0, 1) == 0x4B85F272
+    assert animated_seed(7, 12) == 0xDDE190D2
+    assert len({animated_seed(7, frame) for frame in range(240)}) == 240
+
+
 @pytest.mark.parametrize("size,seed", [(256, 0), (256, 37), (1024, 5), (512, 255)])
 def test_patterns_equal_the_restatement(size, seed):
     np.testing.assert_array_equal(oracle_pattern(size, seed), generate_4d(size, seed))
