@@ -70,9 +70,25 @@ FIXED_SPP = 8
 COUNTER_FILE = ROOT / "profiles" / "k_extend_counters.json"
 KERNEL_SOURCES = ["kernels.hip", "kernels.h", "device_types.h", "vec3.h", "fmath.h", "sobol.h"]
 CLOCK_GHZ, CUS = 2.4, 256  # MI355X peak engine clock and compute units
-# without packed FP32 (this build turns it off) a CU issues at most one
-# 64-lane VALU instruction per cycle
-VALU_ISSUE_PEAK = 1.0
+# The rocprofv3 passes behind COUNTER_FILE, each its own run of this file
+# with `B` = "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-quality
+# --no-batch --no-watertight", written under one directory D and summarised
+# by `python3 bench.py --counters-from D` (the bench line itself goes to
+# D/pmc_bench.json, from `B > D/pmc_bench.json`):
+COUNTER_PASSES = {
+    "pmc_fetch": "FETCH_SIZE",
+    "pmc_write": "WRITE_SIZE",
+    "pmc_valu": "SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_INSTS_SALU SQ_WAVES SQ_WAVE_CYCLES "
+                "GRBM_GUI_ACTIVE",
+    "pmc_l2": "TCC_HIT_sum TCC_MISS_sum",
+    "pmc_stall": "SQ_WAVE_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_ACTIVE_INST_ANY SQ_ACTIVE_INST_VALU "
+                 "SQ_ACTIVE_INST_LDS SQ_BUSY_CYCLES",
+}
+# calibration: FETCH_SIZE (pmc_calf) and WRITE_SIZE (pmc_calw) of a 1 GiB
+# device-to-device copy
+CALIBRATION = ("import torch; a=torch.ones(2**28, device='cuda'); b=torch.empty_like(a); "
+               "torch.cuda.synchronize(); b.copy_(a); torch.cuda.synchronize()")
+COUNTED_KERNEL = "k_extend<false, false, true>"
 
 
 def kernel_hash() -> str:
@@ -93,6 +109,66 @@ def counters() -> dict | None:
     return c if c.get("kernel_sources_sha16") == kernel_hash() else None
 
 
+def summarise_counters(out_dir: Path) -> dict:
+    """COUNTER_FILE's figures from the rocprofv3 passes under out_dir: HBM
+    bytes per ray (FETCH_SIZE and WRITE_SIZE scaled by what they read for a
+    1 GiB copy), VALU and SALU instructions per ray, VALU lane utilisation,
+    the L2 hit rate, and how the kernel's waves spend their cycles."""
+    import csv
+
+    def per_kernel(sub: str, part: str) -> dict:
+        vals: dict = {}
+        for f in sorted((out_dir / sub).glob("**/*counter_collection.csv")):
+            with open(f) as fh:
+                for r in csv.DictReader(fh):
+                    if part in r["Kernel_Name"]:
+                        vals.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+        return {k: (sum(v) / len(v), len(v)) for k, v in vals.items()}
+
+    kernel = COUNTED_KERNEL
+    line = (out_dir / "pmc_bench.json").read_text().strip().splitlines()[-1]
+    rays = json.loads(line)["roofline"]["rays_per_launch"]
+    fetch = per_kernel("pmc_fetch", kernel)["FETCH_SIZE"]
+    write = per_kernel("pmc_write", kernel)["WRITE_SIZE"]
+    cal_f = per_kernel("pmc_calf", "copyBuffer")["FETCH_SIZE"][0]
+    cal_w = per_kernel("pmc_calw", "copyBuffer")["WRITE_SIZE"][0]
+    copied_kib = (1 << 30) / 1024
+    ff, wf = copied_kib / cal_f, copied_kib / cal_w
+    valu = per_kernel("pmc_valu", kernel)
+    l2 = per_kernel("pmc_l2", kernel)
+    stall = per_kernel("pmc_stall", kernel)
+    fb, wb = fetch[0] * 1024 * ff / rays, write[0] * 1024 * wf / rays
+    cycles = stall["SQ_WAVE_CYCLES"][0]
+    return {
+        "kernel": f"pfxh::{kernel} (closest hit, nearest first)",
+        "workload": "terrain still 3840x2160, adaptive<=64 spp, 6 bounces (bench.py defaults, one step)",
+        "kernel_sources_sha16": kernel_hash(),
+        "dispatches": fetch[1],
+        "rays_per_dispatch": rays,
+        "hbm_bytes_per_ray": round(fb + wb, 1),
+        "fetch_bytes_per_ray": round(fb, 1),
+        "write_bytes_per_ray": round(wb, 1),
+        "fetch_kib_per_dispatch_raw": round(fetch[0], 1),
+        "write_kib_per_dispatch_raw": round(write[0], 1),
+        "fetch_factor": round(ff, 4),
+        "write_factor": round(wf, 4),
+        "calibration": f"1 GiB device-to-device copy: FETCH_SIZE {cal_f:.0f} KiB, WRITE_SIZE {cal_w:.0f} KiB "
+                       f"for {copied_kib:.0f} KiB each way",
+        "valu_wave_instr_per_ray": round(valu["SQ_INSTS_VALU"][0] / rays, 1),
+        "salu_instr_per_ray": round(valu["SQ_INSTS_SALU"][0] / rays, 1),
+        "valu_lane_utilisation": round(valu["SQ_THREAD_CYCLES_VALU"][0] / (valu["SQ_ACTIVE_INST_VALU"][0] * 64), 4),
+        "l2_hit_rate": round(l2["TCC_HIT_sum"][0] / (l2["TCC_HIT_sum"][0] + l2["TCC_MISS_sum"][0]), 4),
+        # shares of the waves' cycles: waiting on an operand or a memory
+        # result, ready but not issued, issuing
+        "wave_cycles": {
+            "waiting_on_dependencies": round(stall["SQ_WAIT_ANY"][0] / cycles, 4),
+            "waiting_to_issue": round(stall["SQ_WAIT_INST_ANY"][0] / cycles, 4),
+            "issuing": round(stall["SQ_ACTIVE_INST_ANY"][0] / cycles, 4),
+            "issuing_valu": round(stall["SQ_ACTIVE_INST_VALU"][0] / cycles, 4),
+        },
+    }
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -107,7 +183,16 @@ def counting_pass(trace: Trace, camera, seed: int) -> dict:
     s = trace.stats()
     trace.reset_stats()
     rays = max(s["closest_rays"], 1)
+    rounds = max(s["rounds_inner"] + s["rounds_leaf"], 1)
     return {
+        # closest-hit wave rounds: how many lanes of 64 step in each phase,
+        # how many hold a ray, and the leaf rounds' triangle-loop trips
+        "wave_rounds_per_ray": 64.0 * rounds / rays,
+        "inner_round_share": s["rounds_inner"] / rounds,
+        "inner_lanes_stepping": s["lanes_inner"] / max(64 * s["rounds_inner"], 1),
+        "leaf_lanes_stepping": s["lanes_leaf"] / max(64 * s["rounds_leaf"], 1),
+        "lanes_live": s["lanes_live"] / (64 * rounds),
+        "leaf_trips_per_round": s["leaf_trips"] / max(s["rounds_leaf"], 1),
         "nodes_per_ray": s["nodes_visited"] / rays,
         "boxes_per_ray": s["boxes_tested"] / rays,
         "prims_per_ray": s["prims_tested"] / rays,
@@ -370,7 +455,14 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-stills leg")
     ap.add_argument("--no-watertight", action="store_true",
                     help="skip the watertight-test and walk-order legs")
+    ap.add_argument("--counters-from", type=Path, default=None,
+                    help="write the counter file from the rocprofv3 passes under this directory and exit")
     args = ap.parse_args()
+    if args.counters_from is not None:
+        c = summarise_counters(args.counters_from)
+        COUNTER_FILE.write_text(json.dumps(c, indent=1) + "\n")
+        print(json.dumps(c, indent=1))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); across GPUs the still would be "
               "split into frame tiles, one tracer per GPU", file=sys.stderr)
@@ -425,13 +517,22 @@ def main() -> int:
         launch_s = avg_launch_ms * 1e-3
         hbm_util = traffic / launch_s / (HBM_PEAK_GBS * 1e9)
         issue = measured["valu_wave_instr_per_ray"] * rays_per_launch / (launch_s * CLOCK_GHZ * 1e9 * CUS)
+        waves = measured["wave_cycles"]
         valu = {"wave_instr_per_ray": measured["valu_wave_instr_per_ray"],
                 "issue_per_cu_cycle": round(issue, 3),
-                "issue_frac": round(issue / VALU_ISSUE_PEAK, 3),
                 "lane_utilisation": measured["valu_lane_utilisation"],
-                "l2_hit_rate": measured.get("l2_hit_rate")}
-        # the measured resource nearest its ceiling bounds the kernel
-        limiter = "valu-issue" if issue / VALU_ISSUE_PEAK > hbm_util else "hbm"
+                "l2_hit_rate": measured.get("l2_hit_rate"),
+                "wave_cycles": waves,
+                "hbm_util": round(hbm_util, 4)}
+        # what the kernel's waves wait on: HBM when its traffic nears the
+        # peak, else operands and memory results (latency) when waves wait on
+        # them more than they issue or wait to issue, else issue
+        if hbm_util >= 0.6:
+            limiter = "hbm"
+        elif waves["waiting_on_dependencies"] >= max(waves["issuing"], waves["waiting_to_issue"]):
+            limiter = "memory-latency"
+        else:
+            limiter = "issue"
     else:
         log(f"{COUNTER_FILE.name} missing or measured on other kernel sources: traffic and VALU figures omitted")
     result = {

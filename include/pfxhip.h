@@ -256,6 +256,14 @@ typedef struct pfxh_stats {
     uint32_t launches;      /* closest-hit launches behind trace_ms */
     uint64_t boxes_tested;  /* closest-hit child boxes tested (the reference's binary node
                              * visits), when counting is on */
+    /* Closest-hit wave rounds, when counting is on: a wave of 64 lanes steps
+     * either inner nodes or leaves/instances per round. */
+    uint64_t rounds_inner;  /* inner-node rounds */
+    uint64_t rounds_leaf;   /* leaf and instance rounds */
+    uint64_t lanes_inner;   /* lanes that stepped in inner rounds */
+    uint64_t lanes_leaf;    /* lanes that stepped in leaf rounds */
+    uint64_t lanes_live;    /* lanes holding a ray, over all rounds */
+    uint64_t leaf_trips;    /* triangle-loop trips of leaf rounds (the wave's longest leaf) */
 } pfxh_stats;
 
 typedef struct pfxh_trace pfxh_trace;

@@ -177,6 +177,12 @@ class StatsC(C.Structure):
         ("shade_ms", C.c_double),
         ("launches", C.c_uint32),
         ("boxes_tested", C.c_uint64),
+        ("rounds_inner", C.c_uint64),
+        ("rounds_leaf", C.c_uint64),
+        ("lanes_inner", C.c_uint64),
+        ("lanes_leaf", C.c_uint64),
+        ("lanes_live", C.c_uint64),
+        ("leaf_trips", C.c_uint64),
     ]
 
 

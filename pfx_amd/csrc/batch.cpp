@@ -149,7 +149,7 @@ void record_wave(pfxh_batch* b, const pfxh_batch_tuning& t) {
         launch_hits_batch(s, B, Wb, queue, b->qh, flat_grid);
         launch_shade_batch(s, B, Wb, b->qh, bounce, shade_grid, Wb.counters + C_HITS, b->sobol);
         launch_roll(s, B, Wb, 2, bounce);
-        launch_shadow_batch(s, B, Wb, trace_grid);
+        launch_shadow_batch(s, B, Wb, trace_grid, b->left_first);
         launch_fold_batch(s, Wb, flat_grid);
         queue = outq;
         outq = (outq == b->qa) ? b->qb : b->qa;

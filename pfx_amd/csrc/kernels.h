@@ -96,6 +96,7 @@ enum Counter : uint32_t {
     C_SNODES = 5 * SLOT + 4,
     C_SPRIMS = 5 * SLOT + 6,
     C_BOXES = 5 * SLOT + 8,            // closest-hit child boxes tested
+    C_WAVE = 5 * SLOT + 10,            // closest-hit wave rounds (6 x 64-bit, WaveCount order)
     C_FETCH = 8 * SLOT,                // PARTS work counters of k_extend
     C_SFETCH = C_FETCH + PARTS * SLOT, // PARTS work counters of k_shadow
     C_N = C_SFETCH + PARTS * SLOT,     // batch: rays, shadow rays, shadow paths of the bounce
@@ -132,7 +133,7 @@ struct BatchDev {
 };
 
 constexpr int TRACE_BLOCK = 128;
-constexpr int STACK_SHORT = 12;  // LDS head per lane: 12 KiB per block, 12 blocks per CU fit 160 KiB
+constexpr int STACK_SHORT = 11;  // LDS head per lane: 11 KiB per block, 14 blocks per CU fit 160 KiB
 constexpr int STACK_DEPTH = STACK_LIMIT;
 
 void launch_camera(hipStream_t s, const SceneDev& S, const WaveDev& W, const uint32_t* active, uint32_t npix,
@@ -169,7 +170,7 @@ void launch_hits_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const
 // Shades the wave's queue (length on the device), or queue[0, *count) when count is given.
 void launch_shade_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t bounce,
                         uint32_t grid, const uint32_t* count, bool sobol);
-void launch_shadow_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid);
+void launch_shadow_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid, bool left_first);
 void launch_fold_batch(hipStream_t s, const WaveDev& W, uint32_t grid);
 void launch_finish_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid);
 void launch_next_wave(hipStream_t s, const BatchDev& B);

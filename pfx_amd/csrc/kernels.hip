@@ -191,6 +191,29 @@ DEV bool slab_fast(float lx, float ly, float lz, float hx, float hy, float hz, c
     nr = near_;
     return !(near_ > far_);
 }
+// The conservative test of the default walk for a fast ray: quotients from
+// the reciprocal, q' = RN(RN(lo - o) * RN(1 / d)), which lies within 3
+// units of 2^-24 (relative) of the correctly rounded RN(RN(lo - o) / d) the
+// exact test uses. Entry and exit distances are the same min/max of these,
+// so each lies within that bound of its exact value too (min and max commute
+// with the monotone bounds x -> x -+ eps|x|), and with eps = 2^-21 (8 units,
+// enough to cover the rounding of the bounds themselves) the test accepts
+// every box the exact test accepts: near' (1 - eps) <= far' + eps |far'|.
+// `nr` is near' (1 - eps), a lower bound of the exact entry distance, so the
+// pop-time test nr <= t defers no child the exact walk would take. A box
+// accepted beyond the exact test is an inner node whose children the exact
+// entry distance already culls, or a leaf the ray passes within rounding of
+// its faces.
+DEV bool slab_wide(float lx, float ly, float lz, float hx, float hy, float hz, const Ray& R, float limit, float& nr) {
+    float ax = (lx - R.o.x) * R.y.x, bx = (hx - R.o.x) * R.y.x;
+    float ay = (ly - R.o.y) * R.y.y, by = (hy - R.o.y) * R.y.y;
+    float az = (lz - R.o.z) * R.y.z, bz = (hz - R.o.z) * R.y.z;
+    float near_ = fmaxf(fmaxf(fmaxf(0.0f, fminf(ax, bx)), fminf(ay, by)), fminf(az, bz));
+    float far_ = fminf(fminf(fminf(limit, fmaxf(ax, bx)), fmaxf(ay, by)), fmaxf(az, bz));
+    constexpr float EPS = 0x1p-21f;
+    nr = near_ * (1.0f - EPS);
+    return !(nr > __builtin_fmaf(EPS, fabsf(far_), far_));
+}
 
 // trace.wgsl:449-452 to_object: rows . (p, w), summed x, y, z, w
 DEV v3 to_object(const float* r, v3 p, float w) {
@@ -255,26 +278,33 @@ DEV void order2(uint32_t& ka, uint32_t& ra, uint32_t& kb, uint32_t& rb) {
 // entry distance is compared with the best hit when it is popped, which is
 // the reference's pop-time box test (the intermediate binary node's box
 // contains its children's, so its own test never decides anything).
-// Nearest first, the walk still tests every triangle whose box is entered
-// within the best hit so far, and ties at one distance go to the triangle
-// the left-first walk meets first (Walk::closer); the result can differ
-// from the reference's only where the reference's own answer depends on its
-// order: a hit within an ulp of a box face, where a box's entry distance and
-// the triangle's distance round apart (measured on rays aimed at the
-// vertices and edges of a closed icosphere: 251 of 46,080 Moller-Trumbore
-// and 525 watertight rays keep another triangle, each within 4 ulps of the
-// oracle's distance). Nearest first cuts the closest-hit walk on the 4K
+// Nearest first (with slab_wide), the walk still tests every triangle whose
+// box is entered within the best hit so far, and ties at one distance go to
+// the triangle the left-first walk meets first (Walk::closer); the result
+// can differ from the reference's only where the reference's own answer
+// depends on its order: a hit within an ulp of a box face, where a box's
+// entry distance and the triangle's distance round apart (measured on rays
+// aimed at the vertices and edges of a closed icosphere: 175 of 46,080
+// Moller-Trumbore and 110 watertight rays keep another triangle, each
+// within 4 ulps of the oracle's distance). Nearest first cuts the closest-hit walk on the 4K
 // terrain from 25.0 to 20.3 node visits per ray; shadow rays start on a
 // surface whose own boxes are nearest, and walk fewer nodes left first
 // (22.6 against 23.2), which is also where any-hit may stop.
-template <bool NEAR, class G>
+// WIDE: a fast ray tests boxes with slab_wide (the default walks), else with
+// the exact quotients (left-first walks, the reference's arithmetic).
+template <bool NEAR, bool WIDE, class G>
 DEV bool visit4(const G& S, uint32_t index, const Ray& R, float t, Stack& st, uint32_t& cur) {
     const float4* np = reinterpret_cast<const float4*>(S.nodes + index);
     float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
     uint4 rf = reinterpret_cast<const uint4*>(np)[6];
     float n0, n1, n2, n3;
     bool h0, h1, h2, h3;
-    if (R.fast) {
+    if (WIDE && R.fast) {
+        h0 = slab_wide(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, R, t, n0);
+        h1 = slab_wide(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, R, t, n1) && rf.y != REF_NONE;
+        h2 = slab_wide(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, R, t, n2) && rf.z != REF_NONE;
+        h3 = slab_wide(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, R, t, n3) && rf.w != REF_NONE;
+    } else if (R.fast) {
         h0 = slab_fast(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, R, t, n0);
         h1 = slab_fast(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, R, t, n1) && rf.y != REF_NONE;
         h2 = slab_fast(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, R, t, n2) && rf.z != REF_NONE;
@@ -525,9 +555,11 @@ struct Walk {
     DEV bool step(const G& S, Stack& st, bool& blocked, uint32_t& nodes_ctr, uint32_t& prims_ctr) {
         if (inner()) {
             if (COUNT) nodes_ctr++;
-            if (ANY || (S.geo_flags & GEO_LEFT_FIRST) != 0u) {
-                if (visit4<false>(S, cur, R, h.t, st, cur)) return true;
-            } else if (visit4<true>(S, cur, R, h.t, st, cur)) {
+            if ((S.geo_flags & GEO_LEFT_FIRST) != 0u) {
+                if (visit4<false, false>(S, cur, R, h.t, st, cur)) return true;
+            } else if (ANY) {
+                if (visit4<false, true>(S, cur, R, h.t, st, cur)) return true;
+            } else if (visit4<true, true>(S, cur, R, h.t, st, cur)) {
                 return true;
             }
         } else if (leafy()) {
@@ -1320,12 +1352,35 @@ DEV void store_closest(const WaveDev& W, uint32_t i, const Closest& h, uint32_t 
 // counters.
 // NEAR: nearest-first walk (the default); otherwise the reference's
 // left-first order (a scene built with PFXH_WALK_LEFT_FIRST).
+// What the instrumented closest-hit pass counts per wave round (wave-uniform
+// sums, added by lane 0): inner rounds, leaf rounds, lanes stepping in
+// each, lanes live over all rounds, and the leaf rounds' triangle-loop
+// trips (the most triangles any stepping lane tests).
+struct WaveCount {
+    unsigned long long v[6] = {0, 0, 0, 0, 0, 0};
+    DEV void round(bool inner, bool step, bool live) {
+        v[inner ? 0 : 1] += 1;
+        v[inner ? 2 : 3] += (unsigned long long)__popcll(__ballot(step));
+        v[4] += (unsigned long long)__popcll(__ballot(live));
+    }
+    DEV void trips(uint32_t n) {
+        for (int o = 32; o > 0; o >>= 1) n = max(n, (uint32_t)__shfl_xor((int)n, o));
+        v[5] += n;
+    }
+    DEV void flush(uint32_t* counters) {
+        if (__lane_id() != 0u) return;
+        for (int k = 0; k < 6; ++k)
+            atomicAdd(reinterpret_cast<unsigned long long*>(counters + C_WAVE) + k, v[k]);
+    }
+};
+
 template <bool COUNT, bool BATCH, bool NEAR>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(SceneDev S, const SceneDev* scenes, WaveDev W, const uint32_t* queue, uint32_t n) {
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
     uint32_t nodes = 0, prims = 0, boxes = 0;
+    WaveCount wc;
     Walk<false> w;
     Geo g{};
     if (BATCH) n = W.counters[C_N];
@@ -1349,13 +1404,18 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         // has more lanes waiting (inner steps favoured 3:2)
         bool in = live && w.inner();
         bool out = live && !in;
-        if (inner_round(in, out)) {
+        bool inner_phase = inner_round(in, out);
+        if (COUNT) {
+            wc.round(inner_phase, inner_phase ? in : out, live);
+            if (!inner_phase) wc.trips(out && w.leafy() ? ((w.cur >> 27) & 15u) + 1u : 0u);
+        }
+        if (inner_phase) {
             if (in) {
                 if (COUNT) {
                     nodes++;
                     boxes += child_boxes(walk_scene<BATCH>(S, g), w.cur);
                 }
-                if (!visit4<NEAR>(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) {
+                if (!visit4<NEAR, NEAR>(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) {
                     store_closest(W, ray, w.h, w.record(walk_scene<BATCH>(S, g)));
                     live = false;
                 }
@@ -1376,6 +1436,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_NODES), (unsigned long long)nodes);
         atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_PRIMS), (unsigned long long)prims);
         atomicAdd(reinterpret_cast<unsigned long long*>(W.counters + C_BOXES), (unsigned long long)boxes);
+        wc.flush(W.counters);
     }
 }
 
@@ -1686,8 +1747,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // Any-hit test of every queued shadow ray (one NEE entry each). An occluded
 // entry is marked in its contribution's w; k_fold adds the others.
-template <bool COUNT, bool BATCH>
-__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(SceneDev S, const SceneDev* scenes, WaveDev W, uint32_t n) {
+// WIDE: the default walk's conservative box test (a trace built with the
+// left-first walk keeps the exact quotients for its shadow rays too).
+// Seven waves per SIMD for a single trace (72 registers, none spilled, and an
+// 11-entry LDS stack so 14 blocks fit a CU): 3% faster than six on the 4K
+// terrain. A batch walk holds its still's scene in registers and keeps six.
+template <bool COUNT, bool BATCH, bool WIDE>
+__global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(BATCH ? 6 : 7))) void k_shadow(SceneDev S, const SceneDev* scenes, WaveDev W, uint32_t n) {
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     Stack st{lds + threadIdx.x, W.spill + gtid, W.spill_threads, 0};
@@ -1716,7 +1782,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
         if (inner_round(in, out)) {
             if (in) {
                 if (COUNT) nodes++;
-                if (!visit4<false>(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) live = false;
+                if (!visit4<false, WIDE>(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) live = false;
             }
         } else if (out) {
             bool go = true;
@@ -2119,10 +2185,15 @@ void launch_shadow(hipStream_t s, const SceneDev& S, const WaveDev& W, uint32_t 
     if (!n) return;
     uint32_t blocks = std::min(blocks_for(n, TRACE_BLOCK), grid);
     const SceneDev* none = nullptr;
-    if (count)
-        hipLaunchKernelGGL((k_shadow<true, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, n);
+    bool left = (S.geo_flags & GEO_LEFT_FIRST) != 0u;
+    if (count && left)
+        hipLaunchKernelGGL((k_shadow<true, false, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, n);
+    else if (count)
+        hipLaunchKernelGGL((k_shadow<true, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, n);
+    else if (left)
+        hipLaunchKernelGGL((k_shadow<false, false, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, n);
     else
-        hipLaunchKernelGGL((k_shadow<false, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, n);
+        hipLaunchKernelGGL((k_shadow<false, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, n);
 }
 void launch_fold(hipStream_t s, const WaveDev& W, uint32_t n) {
     if (!n) return;
@@ -2220,9 +2291,12 @@ void launch_shade_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, cons
         hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(256), 0, s, none, B.scenes, W, queue, 0u, bounce,
                            count);
 }
-void launch_shadow_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid) {
+void launch_shadow_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid, bool left_first) {
     SceneDev none{};
-    hipLaunchKernelGGL((k_shadow<false, true>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W, 0u);
+    if (left_first)
+        hipLaunchKernelGGL((k_shadow<false, true, false>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W, 0u);
+    else
+        hipLaunchKernelGGL((k_shadow<false, true, true>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W, 0u);
 }
 void launch_fold_batch(hipStream_t s, const WaveDev& W, uint32_t grid) {
     hipLaunchKernelGGL(k_fold<true>, dim3(grid), dim3(256), 0, s, W, 0u);

@@ -928,6 +928,12 @@ pfxh_status pfxh_trace_stats(const pfxh_trace* t, pfxh_stats* out) {
     out->boxes_tested = u64(C_BOXES);
     out->shadow_nodes = u64(C_SNODES);
     out->shadow_prims = u64(C_SPRIMS);
+    out->rounds_inner = u64(C_WAVE);
+    out->rounds_leaf = u64(C_WAVE + 2);
+    out->lanes_inner = u64(C_WAVE + 4);
+    out->lanes_leaf = u64(C_WAVE + 6);
+    out->lanes_live = u64(C_WAVE + 8);
+    out->leaf_trips = u64(C_WAVE + 10);
     return PFXH_OK;
 }
 

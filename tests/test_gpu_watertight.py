@@ -121,11 +121,17 @@ def test_edge_and_vertex_rays_against_the_oracle(instanced):
     170 (Moller-Trumbore) and 70 (watertight); at exact ties the same
     culling decides which of the tied triangles is kept. The near-first
     device walk may then keep another triangle of the group (world scene:
-    251 and 525 rays). What holds, and is asserted: the same rays miss
-    (watertight: none; Moller-Trumbore: the same leaks); at least 98% of ids
-    are equal, with equal distances; every other device distance is within 4
-    ulps of the oracle's and, for world triangles with Moller-Trumbore, is
-    that triangle's own distance bit for bit."""
+    175 and 110 rays; instanced: 199 and 74). Its box test is also conservative (it accepts every
+    box the exact quotients accept, and a few more within rounding), so where
+    Moller-Trumbore lets a ray leak between two triangles the device may test
+    a neighbour the oracle's walk culled, and hit it (instanced: 41 rays).
+    What holds, and is
+    asserted: every ray the oracle hits, the device hits (watertight: no
+    misses at all), and a ray only the device hits hits the surface at the
+    aimed point (t within 1e-5 of 1); of the rays both hit, at least 98% of
+    ids are equal, with equal distances; every other device distance is
+    within 4 ulps of the oracle's and, for world triangles with
+    Moller-Trumbore, is that triangle's own distance bit for bit."""
     scene, world = ico_scene(instanced=instanced)
     targets = edge_targets(world)
     rng = np.random.default_rng(5)
@@ -139,8 +145,12 @@ def test_edge_and_vertex_rays_against_the_oracle(instanced):
         with Trace.new_detailed(scene, detail, 4, 4) as t:
             ids, dist = t.cast(o, d)
         rid, rt = oracle_cast(scene, detail, o, d)
-        assert np.array_equal(ids == MISS, rid == MISS)
-        differ = np.flatnonzero(ids != rid)
+        assert not np.any((ids == MISS) & (rid != MISS))
+        rescued = (ids != MISS) & (rid == MISS)
+        print(f"  hit only on the device: {int(rescued.sum())} rays")
+        assert np.all(np.abs(dist[rescued] - 1.0) <= 1e-5)
+        both = (ids != MISS) & (rid != MISS)
+        differ = np.flatnonzero(both & (ids != rid))
         print(f"  ids differ on {len(differ)} rays")
         assert len(differ) <= len(o) // 50
         if not instanced and not wt:
