@@ -142,7 +142,8 @@ class Batch:
 
 def _main() -> None:
     """`python -m pfx_amd.batch [SEED]`: build the 32-still uneven mix
-    (scenes.batch_mix), render it once as a batch and print the time."""
+    (scenes.batch_mix), tune the batch, render it twice and print the
+    times."""
     import sys
     import time
 
@@ -151,6 +152,9 @@ def _main() -> None:
     seed = int(sys.argv[1]) if len(sys.argv) > 1 else 3
     mix = scenes.batch_mix(seed=seed)
     with Batch([Still(*m) for m in mix], Detail(bounces=Bounces.deep(6))) as b:
+        t = time.perf_counter()
+        tuning = b.tune()
+        print(f"tuned in {time.perf_counter() - t:.1f} s: {tuning}", flush=True)
         for k in range(2):
             b.reset_stats()
             t = time.perf_counter()

@@ -449,36 +449,57 @@ struct Walk {
         uint32_t first = base + (cur & REF_FIRST_MASK);
         uint32_t count = ((cur >> 27) & 15u) + 1u;
         if (S.geo_flags & GEO_WATERTIGHT) return leaf_watertight<COUNT>(S, first, count, prims_ctr);
+        if (ANY) {
+            // two triangles' loads in flight at once (a shadow walk has the
+            // registers to spare), tested in order
+            for (uint32_t j = 0; j < count; j += 2) {
+                uint32_t i = first + j;
+                bool two = j + 1u < count;
+                const float4* tp = reinterpret_cast<const float4*>(S.tris + i);
+                const float4* tq = reinterpret_cast<const float4*>(S.tris + (two ? i + 1u : i));
+                float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                float4 s0 = tq[0], s1 = tq[1], s2 = tq[2];
+                if (COUNT) prims_ctr++;
+                if (tri_mt(t0, t1, t2, i)) return true;
+                if (two) {
+                    if (COUNT) prims_ctr++;
+                    if (tri_mt(s0, s1, s2, i + 1u)) return true;
+                }
+            }
+            return false;
+        }
         for (uint32_t j = 0; j < count; ++j) {
             uint32_t i = first + j;
             if (COUNT) prims_ctr++;
             const float4* tp = reinterpret_cast<const float4*>(S.tris + i);
-            float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-            v3 A = mk(t0.x, t0.y, t0.z), e1 = mk(t0.w, t1.x, t1.y), e2 = mk(t1.z, t1.w, t2.x);
-            v3 p = pfx::cross(R.d, e2);
-            float det = dot(e1, p);
-            float seen = det * inv_det;
-            if (fabsf(seen) < 1e-9f) continue;
-            if (!ANY && (word & ONE_SIDED) != 0u && seen < 0.0f) continue;
-            float inv = 1.0f / det;
-            v3 s = R.o - A;
-            float u = dot(s, p) * inv;
-            if (u < 0.0f || u > 1.0f) continue;
-            v3 q = pfx::cross(s, e1);
-            float v = dot(R.d, q) * inv;
-            if (v < 0.0f || u + v > 1.0f) continue;
-            float t = dot(e2, q) * inv;
-            if (ANY) {
-                if (t <= 0.0001f || t >= limit) continue;
-                return true;
-            }
-            if (t > 0.0001f && closer(t, i)) {
-                h.t = t;
-                h.u = u;
-                h.v = v;
-                h.slot = i;
-                h.grp = grp;
-            }
+            tri_mt(tp[0], tp[1], tp[2], i);
+        }
+        return false;
+    }
+    // One triangle (edge form a, e1, e2) with the reference's test; with ANY
+    // true when it blocks, else the best hit is updated when it is closer.
+    DEV bool tri_mt(float4 t0, float4 t1, float4 t2, uint32_t i) {
+        v3 A = mk(t0.x, t0.y, t0.z), e1 = mk(t0.w, t1.x, t1.y), e2 = mk(t1.z, t1.w, t2.x);
+        v3 p = pfx::cross(R.d, e2);
+        float det = dot(e1, p);
+        float seen = det * inv_det;
+        if (fabsf(seen) < 1e-9f) return false;
+        if (!ANY && (word & ONE_SIDED) != 0u && seen < 0.0f) return false;
+        float inv = 1.0f / det;
+        v3 s = R.o - A;
+        float u = dot(s, p) * inv;
+        if (u < 0.0f || u > 1.0f) return false;
+        v3 q = pfx::cross(s, e1);
+        float v = dot(R.d, q) * inv;
+        if (v < 0.0f || u + v > 1.0f) return false;
+        float t = dot(e2, q) * inv;
+        if (ANY) return t > 0.0001f && t < limit;
+        if (t > 0.0001f && closer(t, i)) {
+            h.t = t;
+            h.u = u;
+            h.v = v;
+            h.slot = i;
+            h.grp = grp;
         }
         return false;
     }
