@@ -21,7 +21,8 @@ struct DwArgs {
 // Launch shape of the weight gradient for one layer and frame.
 struct DwPlan {
     uint32_t ct, groups, chunks;
-    size_t lds;
+    size_t lds;    // FP32-operand kernel
+    size_t lds16;  // BF16-operand kernel
 };
 DwPlan dw_plan(uint32_t np, uint32_t cin, uint32_t batch, uint32_t h, uint32_t w);
 

@@ -10,10 +10,10 @@
 //   k_relu_bwd              gradient through ReLU (the output is kept)
 //   k_scatter               the stacked input gradient back to the upsampled
 //                           coarser tensor (2x2 sums) and the skip tensor
-//   k_dw                    weight and bias gradients: for each tap a GEMM
-//                           over pixels on the matrix cores (BF16 or FP32
-//                           operands), partial sums per pixel chunk, then
-//                           k_dw_sum / k_db_sum
+//   k_dw32 / k_dw           weight and bias gradients: for each tap a GEMM
+//                           over pixels on the matrix cores (BF16 operands
+//                           on the K = 32 op, or FP32 operands), partial
+//                           sums per pixel chunk, then k_dw_sum / k_db_sum
 //   k_learn_in / k_learn_out / k_loss   input encoding, decoding, the loss (L1 on
 //                           log radiance, or relative L2 of linear radiance)
 //                           and its gradient
@@ -172,15 +172,12 @@ void scatter(hipStream_t s, const float* dX, uint32_t b, uint32_t h, uint32_t w,
 // row at a time: dPre [64 px][np] and the input rows y-1..y+1 over 66 px
 // for the block's `ct` 16-channel input tiles are staged in LDS with float4
 // loads, then each wave adds the segment to its (16 outputs, 16 inputs)
-// tiles for the nine taps -- v_mfma_f32_16x16x16_bf16 with K = 16 pixels, or
-// four v_mfma_f32_16x16x4_f32 (A[o][k] = dPre[k][o], B[k][c] = X[k][c]).
-// Blocks of the first input group also sum dPre over the segment for the
-// bias gradient. Partial sums per chunk; k_dw_sum and k_db_sum add them.
+// tiles for the nine taps with FP32 operands -- four v_mfma_f32_16x16x4_f32
+// per 4 pixels (A[o][k] = dPre[k][o], B[k][c] = X[k][c]). Blocks of the
+// first input group also sum dPre over the segment for the bias gradient.
+// Partial sums per chunk; k_dw_sum and k_db_sum add them.
 constexpr int SEG = 64;
-typedef short v4s __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ short bf16_of(float f) { return (short)bf16_round(f); }
 
-template <bool BF16>
 __global__ __launch_bounds__(256) void k_dw(DwArgs a) {
     extern __shared__ float lds[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -241,28 +238,15 @@ __global__ __launch_bounds__(256) void k_dw(DwArgs a) {
                 const uint32_t ot = p % ot_n, ctl = p / ot_n;
                 const float* dcol = D + ot * 16 + i;
                 const float* xcol = X + ctl * 16 + i;
-                if (BF16) {
-                    const int k0 = 16 * s + 4 * kk;
-                    const v4s av = v4s{bf16_of(dcol[(k0 + 0) * dstride]), bf16_of(dcol[(k0 + 1) * dstride]),
-                                       bf16_of(dcol[(k0 + 2) * dstride]), bf16_of(dcol[(k0 + 3) * dstride])};
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int k = 16 * s + 4 * g + kk;
+                    const float av = dcol[k * dstride];
 #pragma unroll
                     for (int tap = 0; tap < 9; ++tap) {
                         const int ky = tap / 3, kx = tap - ky * 3;
-                        const float* xr = xcol + (ky * (SEG + 2) + k0 + kx) * xw;
-                        const v4s bv = v4s{bf16_of(xr[0]), bf16_of(xr[xw]), bf16_of(xr[2 * xw]), bf16_of(xr[3 * xw])};
-                        acc[pi][tap] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[pi][tap], 0, 0, 0);
-                    }
-                } else {
-#pragma unroll
-                    for (int g = 0; g < 4; ++g) {
-                        const int k = 16 * s + 4 * g + kk;
-                        const float av = dcol[k * dstride];
-#pragma unroll
-                        for (int tap = 0; tap < 9; ++tap) {
-                            const int ky = tap / 3, kx = tap - ky * 3;
-                            const float bv = xcol[(ky * (SEG + 2) + k + kx) * xw];
-                            acc[pi][tap] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[pi][tap], 0, 0, 0);
-                        }
+                        const float bv = xcol[(ky * (SEG + 2) + k + kx) * xw];
+                        acc[pi][tap] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[pi][tap], 0, 0, 0);
                     }
                 }
             }
@@ -281,17 +265,164 @@ __global__ __launch_bounds__(256) void k_dw(DwArgs a) {
     }
     if (bias_block) a.part_b[(size_t)blockIdx.y * np + tid] = bsum;
 }
-// dW[o][pc][tap] = sum over chunks (threads walk the partials' own order,
-// so the chunk reads are coalesced)
-__global__ void k_dw_sum(const float* part, uint32_t chunks, uint32_t np, uint32_t cin, float* dW) {
-    size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
-    size_t n = (size_t)np * cin * 9;
-    if (e >= n) return;
-    uint32_t pc = e % cin, rest = (uint32_t)(e / cin);
-    uint32_t o = rest % np, tap = rest / np;
+// BF16 operands on gfx950's K = 32 matrix-core op (v_mfma_f32_16x16x32_bf16):
+// the same GEMMs with K = 32 pixels per instruction. A segment's dPre and
+// input rows are rounded to BF16 once as they are staged, pixel-contiguous
+// per channel, so each lane takes its 8 pixels of an operand with one
+// 16-byte LDS read: A = Dt[o][p0 + 8g ..], and for the three taps of an
+// input row one 20-byte read gives B at pixel offsets 0, 1 and 2 (the odd
+// offset by a 2-byte funnel shift). The bias sums stay FP32: each staging
+// item sums its 8 pixels, and the bias threads add the items in pixel order.
+constexpr uint32_t XQ = SEG + 8;  // staged input pixels per row: SEG + 2 halo, padded to 16 bytes
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ v8bf as_v8bf(uint4 u) { return __builtin_bit_cast(v8bf, u); }
+__device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
+    return (uint32_t)bf16_round(lo) | ((uint32_t)bf16_round(hi) << 16);
+}
+
+__global__ __launch_bounds__(256) void k_dw32(DwArgs a) {
+    extern __shared__ uint4 lds16[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 15, g = lane >> 4;
+    const uint32_t np = a.np, cin = a.c0 + a.c1, ot_n = np / 16, ct_all = cin / 16;
+    const uint32_t ct0 = blockIdx.x * a.ct, ctn = min(a.ct, ct_all - ct0), pairs = ot_n * ctn;
+    uint16_t* Dt = reinterpret_cast<uint16_t*>(lds16);  // [np][SEG]
+    uint16_t* Xt = Dt + (size_t)np * SEG;                // [3 rows][a.ct * 16 channels][XQ]
+    const uint32_t xrows = a.ct * 16;
+    float* bpart = reinterpret_cast<float*>(Xt + (size_t)3 * xrows * XQ);  // [SEG / 8][np]
+    const uint32_t h = a.h, w = a.w, segw = (w + SEG - 1) / SEG;
+    const size_t segs = (size_t)a.batch * h * segw;
+    const size_t s0 = segs * blockIdx.y / gridDim.y, s1 = segs * (blockIdx.y + 1) / gridDim.y;
+    const uint32_t sh = a.up ? h / 2 : h, sw = a.up ? w / 2 : w;
+    const bool bias_block = blockIdx.x == 0 && (uint32_t)tid < np;
+    float bsum = 0.0f;
+    v4f acc[2][9];
+    for (int p = 0; p < 2; ++p)
+        for (int t = 0; t < 9; ++t) acc[p][t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    for (size_t sg = s0; sg < s1; ++sg) {
+        const uint32_t xs = (uint32_t)(sg % segw) * SEG;
+        const uint32_t y = (uint32_t)((sg / segw) % h);
+        const size_t b = sg / ((size_t)segw * h);
+        const float* drow = a.dpre + (b * h + y) * w * np;
+        __syncthreads();
+        // dPre: items of 4 outputs x 8 pixels
+        for (uint32_t e = tid; e < (np / 4) * (SEG / 8); e += 256) {
+            const uint32_t oq = e % (np / 4), po = e / (np / 4);
+            float4 v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t x = xs + 8 * po + j;
+                v[j] = x < w ? *reinterpret_cast<const float4*>(drow + (size_t)x * np + 4 * oq)
+                             : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+            float4 bs = v[0];
+#pragma unroll
+            for (int j = 1; j < 8; ++j) bs = make_float4(bs.x + v[j].x, bs.y + v[j].y, bs.z + v[j].z, bs.w + v[j].w);
+            *reinterpret_cast<float4*>(bpart + (size_t)po * np + 4 * oq) = bs;
+            uint16_t* d = Dt + (size_t)(4 * oq) * SEG + 8 * po;
+            *reinterpret_cast<uint4*>(d) = make_uint4(bf16_pair(v[0].x, v[1].x), bf16_pair(v[2].x, v[3].x),
+                                                      bf16_pair(v[4].x, v[5].x), bf16_pair(v[6].x, v[7].x));
+            *reinterpret_cast<uint4*>(d + SEG) = make_uint4(bf16_pair(v[0].y, v[1].y), bf16_pair(v[2].y, v[3].y),
+                                                            bf16_pair(v[4].y, v[5].y), bf16_pair(v[6].y, v[7].y));
+            *reinterpret_cast<uint4*>(d + 2 * SEG) = make_uint4(bf16_pair(v[0].z, v[1].z), bf16_pair(v[2].z, v[3].z),
+                                                                bf16_pair(v[4].z, v[5].z), bf16_pair(v[6].z, v[7].z));
+            *reinterpret_cast<uint4*>(d + 3 * SEG) = make_uint4(bf16_pair(v[0].w, v[1].w), bf16_pair(v[2].w, v[3].w),
+                                                                bf16_pair(v[4].w, v[5].w), bf16_pair(v[6].w, v[7].w));
+        }
+        // input rows y - 1 .. y + 1, pixels xs - 1 + q: items of 4 channels x 8 pixels
+        for (uint32_t e = tid; e < 3 * (ctn * 4) * (XQ / 8); e += 256) {
+            const uint32_t cq = e % (ctn * 4), rest = e / (ctn * 4), qo = rest % (XQ / 8), r = rest / (XQ / 8);
+            const int yy = (int)y + (int)r - 1;
+            const uint32_t cc = ct0 * 16 + 4 * cq;
+            float4 v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t q = 8 * qo + j;
+                const int xx = (int)xs + (int)q - 1;
+                v[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (q < SEG + 2 && yy >= 0 && yy < (int)h && xx >= 0 && xx < (int)w) {
+                    if (cc < a.c0) {
+                        const int sy = a.up ? yy >> 1 : yy, sx = a.up ? xx >> 1 : xx;
+                        v[j] = *reinterpret_cast<const float4*>(
+                            a.src0 + ((b * sh + (uint32_t)sy) * sw + (uint32_t)sx) * a.c0 + cc);
+                    } else {
+                        v[j] = *reinterpret_cast<const float4*>(
+                            a.src1 + ((b * h + (uint32_t)yy) * w + (uint32_t)xx) * a.c1 + (cc - a.c0));
+                    }
+                }
+            }
+            uint16_t* d = Xt + ((size_t)r * xrows + 4 * cq) * XQ + 8 * qo;
+            *reinterpret_cast<uint4*>(d) = make_uint4(bf16_pair(v[0].x, v[1].x), bf16_pair(v[2].x, v[3].x),
+                                                      bf16_pair(v[4].x, v[5].x), bf16_pair(v[6].x, v[7].x));
+            *reinterpret_cast<uint4*>(d + XQ) = make_uint4(bf16_pair(v[0].y, v[1].y), bf16_pair(v[2].y, v[3].y),
+                                                           bf16_pair(v[4].y, v[5].y), bf16_pair(v[6].y, v[7].y));
+            *reinterpret_cast<uint4*>(d + 2 * XQ) = make_uint4(bf16_pair(v[0].z, v[1].z), bf16_pair(v[2].z, v[3].z),
+                                                               bf16_pair(v[4].z, v[5].z), bf16_pair(v[6].z, v[7].z));
+            *reinterpret_cast<uint4*>(d + 3 * XQ) = make_uint4(bf16_pair(v[0].w, v[1].w), bf16_pair(v[2].w, v[3].w),
+                                                               bf16_pair(v[4].w, v[5].w), bf16_pair(v[6].w, v[7].w));
+        }
+        __syncthreads();
+        if (bias_block)
+            for (uint32_t po = 0; po < SEG / 8; ++po) bsum += bpart[(size_t)po * np + tid];
+        for (uint32_t p0 = 0; p0 < SEG && xs + p0 < w; p0 += 32) {
+#pragma unroll
+            for (int pi = 0; pi < 2; ++pi) {
+                const uint32_t p = (uint32_t)wave + 4u * pi;
+                if (p >= pairs) continue;
+                const uint32_t ot = p % ot_n, ctl = p / ot_n;
+                const v8bf av = as_v8bf(*reinterpret_cast<const uint4*>(Dt + (size_t)(ot * 16 + i) * SEG + p0 + 8 * g));
+#pragma unroll
+                for (int ky = 0; ky < 3; ++ky) {
+                    const uint16_t* xr = Xt + ((size_t)ky * xrows + ctl * 16 + i) * XQ + p0 + 8 * g;
+                    const uint4 w0 = *reinterpret_cast<const uint4*>(xr);
+                    const uint32_t w4 = *reinterpret_cast<const uint32_t*>(xr + 8);
+                    const uint4 w1 = make_uint4(__builtin_amdgcn_alignbyte(w0.y, w0.x, 2u),
+                                                __builtin_amdgcn_alignbyte(w0.z, w0.y, 2u),
+                                                __builtin_amdgcn_alignbyte(w0.w, w0.z, 2u),
+                                                __builtin_amdgcn_alignbyte(w4, w0.w, 2u));
+                    const uint4 w2 = make_uint4(w0.y, w0.z, w0.w, w4);
+                    acc[pi][3 * ky + 0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, as_v8bf(w0), acc[pi][3 * ky + 0], 0, 0, 0);
+                    acc[pi][3 * ky + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, as_v8bf(w1), acc[pi][3 * ky + 1], 0, 0, 0);
+                    acc[pi][3 * ky + 2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, as_v8bf(w2), acc[pi][3 * ky + 2], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // lane: input column c = i, output rows 4 g + v
+#pragma unroll
+    for (int pi = 0; pi < 2; ++pi) {
+        const uint32_t p = (uint32_t)wave + 4u * pi;
+        if (p >= pairs) continue;
+        const uint32_t ot = p % ot_n, ctl = p / ot_n;
+        for (int tap = 0; tap < 9; ++tap)
+            for (int v = 0; v < 4; ++v)
+                a.part[(((size_t)blockIdx.y * 9 + tap) * np + ot * 16 + 4 * g + v) * cin + (ct0 + ctl) * 16 + i] =
+                    acc[pi][tap][v];
+    }
+    if (bias_block) a.part_b[(size_t)blockIdx.y * np + tid] = bsum;
+}
+
+// dW[o][pc][tap] = sum over chunks: a block takes 16 consecutive partial
+// entries, its 16 slices of threads the chunks g = slice, slice + 16, ... in
+// order, and the slices' sums are added in slice order (a fixed order, so
+// the sum is the same every run)
+constexpr uint32_t SUM_SLICES = 16;
+__global__ __launch_bounds__(256) void k_dw_sum(const float* part, uint32_t chunks, uint32_t np, uint32_t cin, float* dW) {
+    __shared__ float red[SUM_SLICES][16];
+    const uint32_t el = threadIdx.x & 15u, slice = threadIdx.x >> 4;
+    const size_t n = (size_t)np * cin * 9;
+    const size_t e = (size_t)blockIdx.x * 16 + el;
     float s = 0.0f;
-    for (uint32_t g = 0; g < chunks; ++g) s += part[(size_t)g * n + e];
-    dW[((size_t)o * cin + pc) * 9 + tap] = s;
+    if (e < n)
+        for (uint32_t g = slice; g < chunks; g += SUM_SLICES) s += part[(size_t)g * n + e];
+    red[slice][el] = s;
+    __syncthreads();
+    if (slice != 0 || e >= n) return;
+    float t = 0.0f;
+    for (uint32_t k = 0; k < SUM_SLICES; ++k) t += red[k][el];
+    const uint32_t pc = (uint32_t)(e % cin), rest = (uint32_t)(e / cin);
+    const uint32_t o = rest % np, tap = rest / np;
+    dW[((size_t)o * cin + pc) * 9 + tap] = t;
 }
 // one block per output channel
 __global__ void k_db_sum(const float* part, uint32_t chunks, uint32_t np, float* db) {
@@ -314,8 +445,9 @@ DwPlan dw_plan(uint32_t np, uint32_t cin, uint32_t batch, uint32_t h, uint32_t w
     size_t segs = (size_t)batch * h * ((w + SEG - 1) / SEG);
     p.chunks = (uint32_t)std::max<size_t>(1, std::min<size_t>(segs, (1024 + p.groups - 1) / p.groups));
     p.lds = (size_t)(SEG * (np + 4) + 3 * (SEG + 2) * (p.ct * 16 + 4)) * 4;
+    p.lds16 = ((size_t)np * SEG + 3 * (size_t)p.ct * 16 * XQ) * 2 + (size_t)(SEG / 8) * np * 4;
     // each of the 4 waves holds at most two (output, input) tile pairs
-    if (ot_n * p.ct > 8 || p.lds > 64 * 1024) p.chunks = 0;
+    if (ot_n * p.ct > 8 || p.lds > 64 * 1024 || p.lds16 > 64 * 1024) p.chunks = 0;
     return p;
 }
 void weight_grads(hipStream_t s, const DwArgs& a, const DwPlan& p, bool bf16, float* dW, float* db) {
@@ -325,11 +457,11 @@ void weight_grads(hipStream_t s, const DwArgs& a, const DwPlan& p, bool bf16, fl
     b.ct = p.ct;
     dim3 grid(p.groups, p.chunks);
     if (bf16)
-        hipLaunchKernelGGL(k_dw<true>, grid, dim3(256), p.lds, s, b);
+        hipLaunchKernelGGL(k_dw32, grid, dim3(256), p.lds16, s, b);
     else
-        hipLaunchKernelGGL(k_dw<false>, grid, dim3(256), p.lds, s, b);
+        hipLaunchKernelGGL(k_dw, grid, dim3(256), p.lds, s, b);
     size_t n = (size_t)a.np * cin * 9;
-    hipLaunchKernelGGL(k_dw_sum, dim3(blocks(n, 256)), dim3(256), 0, s, a.part, p.chunks, a.np, cin, dW);
+    hipLaunchKernelGGL(k_dw_sum, dim3(blocks(n, 16)), dim3(256), 0, s, a.part, p.chunks, a.np, cin, dW);
     hipLaunchKernelGGL(k_db_sum, dim3(a.np), dim3(256), 0, s, a.part_b, p.chunks, a.np, db);
 }
 
