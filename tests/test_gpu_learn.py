@@ -1,8 +1,10 @@
 """The learned denoiser's HIP kernels against the FP64 oracle: loss and
-gradient of every layer (FP32 operands: 1e-3 of each layer's gradient norm;
-BF16 operands, the training mode: 0.15 per layer and 0.05 over the whole
-gradient, the deep layers of a 32-pixel crop seeing only 16 to 64 pixels,
-where single ReLU flips from rounding show), the batch gather under all eight
+gradient of every layer (FP32 operands: 1e-5 of each layer's gradient norm,
+measured 6e-7 and 1.3e-6; BF16 operands, the training mode: 0.12 per layer
+and 0.03 over the whole gradient, measured 0.097 and 0.019, the deep layers
+of a 32-pixel crop seeing only 16 to 64 pixels, where single ReLU flips from
+rounding show; the 256-pixel crops of test_gpu_learn_c4 hold 0.05 and
+0.02), the batch gather under all eight
 symmetries of the square crop, inference on a frame that is not a multiple of
 16, the first Adam step in closed form (p - lr * g / (|g| + eps)), and
 training that lowers the loss."""
@@ -59,8 +61,8 @@ def filled(learner, data, side):
     return learner
 
 
-@pytest.mark.parametrize("fp32,tol,total,loss", [(True, 1e-3, 1e-3, "l1_log"), (False, 0.15, 0.05, "l1_log"),
-                                                  (True, 1e-3, 1e-3, "rel_l2")])
+@pytest.mark.parametrize("fp32,tol,total,loss", [(True, 1e-5, 1e-5, "l1_log"), (False, 0.12, 0.03, "l1_log"),
+                                                  (True, 1e-5, 1e-5, "rel_l2")])
 def test_gradient_matches_fp64_oracle(fp32, tol, total, loss):
     print(f"learn gradient fp32={fp32} loss={loss}", flush=True)
     rng = np.random.default_rng(21)
@@ -74,12 +76,16 @@ def test_gradient_matches_fp64_oracle(fp32, tol, total, loss):
     ref_data = [np.stack([a[0], symmetry(a[1], 5)]) for a in data]
     ref_loss, ref_g = O.learn_grad(p.astype(np.float64), *ref_data, loss=loss)
     loss = loss_value
-    assert loss == pytest.approx(ref_loss, rel=tol * 1e-2 if fp32 else 1e-2)
+    assert loss == pytest.approx(ref_loss, rel=1e-5 if fp32 else 1e-2)
+    worst = 0.0
     for k, (lo, hi) in enumerate(layer_slices()):
         gap = np.linalg.norm(g[lo:hi] - ref_g[lo:hi])
         size = np.linalg.norm(ref_g[lo:hi])
+        worst = max(worst, gap / size)
         assert gap <= tol * size, (k, gap, size)
-    assert np.linalg.norm(g - ref_g) <= total * np.linalg.norm(ref_g)
+    whole = np.linalg.norm(g - ref_g) / np.linalg.norm(ref_g)
+    print(f"  worst layer {worst:.4g}, whole gradient {whole:.4g}", flush=True)
+    assert whole <= total
 
 
 def test_gather_symmetries_match_the_oracle():
