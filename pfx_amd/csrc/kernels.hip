@@ -2031,11 +2031,16 @@ __global__ __launch_bounds__(256) void k_clear(AccumDev A, uint32_t pixels) {
 }
 
 // Pixels not frozen by adaptive sampling (trace.wgsl:2517).
+// Each block compacts one contiguous chunk of pixels with one atomic (one
+// atomic per wave on the one counter serialised across the XCDs: 963 us per
+// call on the 4K frame against 84 this way); the order of the active list
+// does not matter, every pixel's samples being seeded by the pixel and the
+// sample index.
 __global__ __launch_bounds__(256) void k_active(AccumDev A, uint32_t pixels, uint32_t* active, uint32_t* counter) {
-    uint32_t pix = blockIdx.x * 256u + threadIdx.x;
-    bool live = pix < pixels && !(A.sum[pix * 3u].w < 0.0f);
-    uint32_t at = wave_append(counter, live);
-    if (live) active[at] = pix;
+    uint32_t chunk = (pixels + gridDim.x - 1u) / gridDim.x;
+    uint32_t lo = min(pixels, blockIdx.x * chunk), hi = min(pixels, lo + chunk);
+    auto live = [&](uint32_t pix) { return !(A.sum[pix * 3u].w < 0.0f); };
+    compact_chunk(lo, hi, nullptr, live, counter, active);
 }
 
 // Pixel-centre primary hits (probe.wgsl:3-39) plus the caller's primitive.
@@ -2109,11 +2114,15 @@ __global__ __launch_bounds__(256) void k_measure(AccumDev A, float4* snaps, uint
 }
 
 // converge.wgsl:32-51
+// Blocks walk the frame with a grid stride and add their running count with
+// one atomic each (one per wave took 989 us per call on the 4K frame, 148
+// this way).
 __global__ __launch_bounds__(256) void k_mark(AccumDev A, const float4* snaps, uint32_t width, uint32_t height,
                                               uint32_t min_samples, float threshold, uint32_t* running) {
-    uint32_t pix = blockIdx.x * 256u + threadIdx.x;
-    bool run = false;
-    if (pix < width * height) {
+    __shared__ uint32_t tot[4];
+    const uint32_t pixels = width * height;
+    uint32_t runs = 0;
+    for (uint32_t pix = blockIdx.x * 256u + threadIdx.x; pix < pixels; pix += gridDim.x * 256u) {
         float count = A.sum[pix * 3u].w;
         if (count > 0.0f) {
             int x = (int)(pix % width), y = (int)(pix / width);
@@ -2123,14 +2132,19 @@ __global__ __launch_bounds__(256) void k_mark(AccumDev A, const float4* snaps, u
                     int xx = min(max(x + dx, 0), (int)width - 1), yy = min(max(y + dy, 0), (int)height - 1);
                     worst = fmaxf(worst, snaps[((uint32_t)yy * width + (uint32_t)xx) * 2u + 1u].y);
                 }
-            if (count >= (float)min_samples && worst < threshold) {
+            if (count >= (float)min_samples && worst < threshold)
                 A.sum[pix * 3u].w = -count;
-            } else {
-                run = true;
-            }
+            else
+                ++runs;
         }
     }
-    wave_append(running, run);
+    uint32_t ws = wave_scan(runs);
+    if (__lane_id() == 63u) tot[threadIdx.x >> 6] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = tot[0] + tot[1] + tot[2] + tot[3];
+        if (sum) atomicAdd(running, sum);
+    }
 }
 
 __global__ void k_zero(uint32_t* p, uint32_t n) {
@@ -2236,7 +2250,9 @@ void launch_clear(hipStream_t s, const AccumDev& A, uint32_t pixels) {
 }
 void launch_active(hipStream_t s, const AccumDev& A, uint32_t pixels, uint32_t* active, uint32_t* counter) {
     if (!pixels) return;
-    hipLaunchKernelGGL(k_active, dim3(blocks_for(pixels, 256)), dim3(256), 0, s, A, pixels, active, counter);
+    // chunks of at most 64 tiles of 256 (the bit mask), at least 2048 blocks
+    uint32_t blocks = std::max(2048u, blocks_for(pixels, 64u * 256u));
+    hipLaunchKernelGGL(k_active, dim3(blocks), dim3(256), 0, s, A, pixels, active, counter);
 }
 // Closest hits of a caller's rays (origins and directions as float4, w
 // unused): the caller's triangle id per ray, UINT32_MAX on a miss, and the
@@ -2275,8 +2291,8 @@ void launch_mark(hipStream_t s, const AccumDev& A, const float4* snaps, uint32_t
                  uint32_t min_samples, float threshold, uint32_t* running) {
     uint32_t pixels = width * height;
     if (!pixels) return;
-    hipLaunchKernelGGL(k_mark, dim3(blocks_for(pixels, 256)), dim3(256), 0, s, A, snaps, width, height, min_samples,
-                       threshold, running);
+    hipLaunchKernelGGL(k_mark, dim3(std::min(2048u, blocks_for(pixels, 256))), dim3(256), 0, s, A, snaps, width,
+                       height, min_samples, threshold, running);
 }
 void launch_zero(hipStream_t s, uint32_t* p, uint32_t n) {
     if (!n) return;
