@@ -297,6 +297,12 @@ pfxh_status pfxh_trace_sample(pfxh_trace* trace, uint32_t count, uint32_t seed, 
 pfxh_status pfxh_trace_sample_adaptive(pfxh_trace* trace, const pfxh_adaptive* adaptive,
                                        uint32_t seed, pfxh_converged* out, char* err,
                                        size_t err_len);
+/* The standard error of each pixel's mean luminance ((r + g + b) / 3) over
+ * its samples, sqrt(max(E[l^2] - E[l]^2, 0) / n), width*height floats,
+ * caller-allocated; 0 where a pixel has no samples. No counterpart in the
+ * reference: the learned denoiser's noise channel. After pfxh_trace_restore
+ * it covers the samples taken since (PFXCKPT1 holds no moments). */
+pfxh_status pfxh_trace_readback_deviation(pfxh_trace* trace, float* out, char* err, size_t err_len);
 /* Trace::readback (gpu.rs:3236-3246): three RGBA32F images, row-major,
  * width*height*4 floats each, caller-allocated. Any pointer may be NULL. */
 pfxh_status pfxh_trace_readback(pfxh_trace* trace, float* color, float* albedo, float* normal,

@@ -50,6 +50,7 @@ def lib() -> C.CDLL:
             "or_sample": (None, [_P, _U, _U, _U, _U]),
             "or_samples": (_U, [_P]),
             "or_readback": (None, [_P, _P, _P, _P]),
+            "or_deviation": (None, [_P, _P]),
             "or_counters": (None, [_P, _P]),
             "or_probe": (None, [_P, _P]),
             "or_sample_adaptive": (C.c_int, [_P, _P, _U, _P]),
@@ -122,6 +123,11 @@ class Oracle:
         c, a, n = (np.empty(shape, np.float32) for _ in range(3))
         lib().or_readback(self._h, c.ctypes.data, a.ctypes.data, n.ctypes.data)
         return c, a, n
+
+    def deviation(self):
+        out = np.empty((self.height, self.width), np.float32)
+        lib().or_deviation(self._h, out.ctypes.data)
+        return out
 
     def set_camera(self, camera) -> None:
         cc = camera.to_c()

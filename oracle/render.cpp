@@ -76,6 +76,7 @@ struct Scene {
     // accumulation: 12 sums per pixel (color.xyzw, albedo, normal)
     std::vector<float> sum, comp;
     std::vector<double> sum64;
+    std::vector<float> m2;  // per pixel: sum over samples of ((r + g + b) / 3)^2
     std::vector<float> snaps;  // adaptive Snap{mean, info}: 8 floats per pixel
     std::atomic<uint64_t> closest{0}, shadow{0}, paths{0};
 };
@@ -1086,6 +1087,7 @@ static void sample_pixel(Scene& sc, uint32_t x, uint32_t y, uint32_t count, uint
             C[k] = 0.0f;
             D[k] = 0.0;
         }
+        sc.m2[index] = 0.0f;
     }
     if (S[3] < 0.0f) return;
     uint32_t px = x + sc.window[2], py = y + sc.window[3];
@@ -1105,6 +1107,8 @@ static void sample_pixel(Scene& sc, uint32_t x, uint32_t y, uint32_t count, uint
         v3 o, d;
         camera_ray(sc, px, py, jx, jy, &o, &d, &state, false);
         Result r = radiance(sc, o, d, &state, closest, shadows);
+        float l = (r.color.x + r.color.y + r.color.z) / 3.0f;
+        sc.m2[index] = sc.m2[index] + l * l;
         float vals[12] = {r.color.x, r.color.y, r.color.z, 0.0f, r.albedo[0], r.albedo[1], r.albedo[2], r.albedo[3],
                           r.normal[0], r.normal[1], r.normal[2], r.normal[3]};
         for (int k = 0; k < 12; ++k) {
@@ -1384,6 +1388,7 @@ static std::string prepare(Scene& sc, const pfxh_scene& in, const pfxh_detail& d
     sc.sum.assign((size_t)w * h * 12, 0.0f);
     sc.comp.assign((size_t)w * h * 12, 0.0f);
     sc.sum64.assign((size_t)w * h * 12, 0.0);
+    sc.m2.assign((size_t)w * h, 0.0f);
     sc.snaps.assign((size_t)w * h * 8, 0.0f);
     return "";
 }
@@ -1452,6 +1457,21 @@ void or_sample(void* h, uint32_t count, uint32_t seed, uint32_t row0, uint32_t r
 }
 uint32_t or_samples(void* h) { return ((Scene*)h)->samples; }
 void or_readback(void* h, float* color, float* albedo, float* normal) { develop(*(Scene*)h, color, albedo, normal); }
+// the standard error of each pixel's mean luminance, sqrt(max(m2 / n - l^2,
+// 0) / n) with l the mean of the developed colour's channels; the product's
+// noise channel for the learned denoiser (no reference counterpart)
+void or_deviation(void* h, float* out) {
+    const Scene& sc = *(const Scene*)h;
+    for (size_t i = 0; i < (size_t)sc.width * sc.height; ++i) {
+        float n = fabsf(sc.sum[i * 12 + 3]);
+        float se = 0.0f;
+        if (n > 0.0f) {
+            float l = (total_of(sc, i, 0) / n + total_of(sc, i, 1) / n + total_of(sc, i, 2) / n) / 3.0f;
+            se = sqrtf(fmaxf(sc.m2[i] / n - l * l, 0.0f) / n);
+        }
+        out[i] = se;
+    }
+}
 void or_counters(void* h, uint64_t* out) {
     Scene* sc = (Scene*)h;
     out[0] = sc->closest;

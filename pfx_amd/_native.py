@@ -245,6 +245,7 @@ SIGNATURES = {
     "pfxh_trace_sample": (C.c_int, [_P, _U, _U, _S, _Z]),
     "pfxh_trace_sample_adaptive": (C.c_int, [_P, _P, _U, _P, _S, _Z]),
     "pfxh_trace_readback": (C.c_int, [_P, _P, _P, _P, _S, _Z]),
+    "pfxh_trace_readback_deviation": (C.c_int, [_P, _P, _S, _Z]),
     "pfxh_trace_set_camera": (C.c_int, [_P, _P, _S, _Z]),
     "pfxh_trace_set_window": (C.c_int, [_P, _U, _U, _U, _U, _S, _Z]),
     "pfxh_trace_set_sun": (C.c_int, [_P, _P, _S, _Z]),

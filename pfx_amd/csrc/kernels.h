@@ -77,6 +77,7 @@ struct AccumDev {
     float4* sum;   // color (w = count), albedo, normal: 3 float4 per pixel
     float4* comp;  // Neumaier compensation, 3 float4 per pixel
     double* sum64; // 12 doubles per pixel in FP64 mode
+    float* m2;     // per pixel: the sum over samples of ((r + g + b) / 3)^2 (plain FP32)
     uint32_t mode;
 };
 
@@ -151,6 +152,9 @@ void launch_finish(hipStream_t s, const WaveDev& W, const AccumDev& A, const uin
 void launch_develop(hipStream_t s, const AccumDev& A, uint32_t pixels, float4* color, float4* albedo,
                     float4* normal);
 void launch_clear(hipStream_t s, const AccumDev& A, uint32_t pixels);
+// The standard error of each pixel's mean luminance ((r + g + b) / 3) over its
+// samples, sqrt(max(m2 / n - mean^2, 0) / n); 0 for a pixel with no samples.
+void launch_deviation(hipStream_t s, const AccumDev& A, uint32_t pixels, float* out);
 void launch_active(hipStream_t s, const AccumDev& A, uint32_t pixels, uint32_t* active, uint32_t* counter);
 void launch_probe(hipStream_t s, const SceneDev& S, const WaveDev& W, void* out);
 void launch_cast(hipStream_t s, const SceneDev& S, const WaveDev& W, const float4* o, const float4* d, uint32_t n,

@@ -1856,9 +1856,17 @@ DEV void neumaier(float& s, float& c, float x) {
 // (trace.wgsl:2562-2571).
 // Pixel `pix` of a still whose samples in the wave are paths
 // first + s * npix + p, s < nsamples.
+DEV float lum_sq(float4 r) {
+    const float l = (r.x + r.y + r.z) / 3.0f;
+    return l * l;
+}
 DEV void finish_pixel(const WaveDev& W, const AccumDev& A, uint32_t pix, uint32_t first, uint32_t npix, uint32_t p,
                       uint32_t nsamples, float count_after) {
     p += first;
+    // the luminance moment, in sample order, as the oracle sums it
+    float m2 = A.m2[pix];
+    for (uint32_t s = 0; s < nsamples; ++s) m2 = m2 + lum_sq(W.res[s * npix + p]);
+    A.m2[pix] = m2;
     if (A.mode == PFXH_ACCUM_PLAIN) {
         float4 c = A.sum[pix * 3u], a = A.sum[pix * 3u + 1u], nn = A.sum[pix * 3u + 2u];
         for (uint32_t s = 0; s < nsamples; ++s) {
@@ -1990,6 +1998,20 @@ DEV float total_of(const AccumDev& A, uint32_t pix, int k) {
     return s[k] + c[k];
 }
 
+// The standard error of each pixel's mean luminance: the learned denoiser's
+// noise channel (a firefly is a pixel whose mean a few samples dominate).
+__global__ __launch_bounds__(256) void k_deviation(AccumDev A, uint32_t pixels, float* out) {
+    uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix >= pixels) return;
+    const float n = fabsf(A.sum[pix * 3u].w);
+    float se = 0.0f;
+    if (n > 0.0f) {
+        const float l = (total_of(A, pix, 0) / n + total_of(A, pix, 1) / n + total_of(A, pix, 2) / n) / 3.0f;
+        se = sqrtf(fmaxf(A.m2[pix] / n - l * l, 0.0f) / n);
+    }
+    out[pix] = se;
+}
+
 // The three RGBA32F images (trace.wgsl:2572-2578).
 __global__ __launch_bounds__(256) void k_develop(AccumDev A, uint32_t pixels, float4* color, float4* albedo,
                                                  float4* normal) {
@@ -2028,6 +2050,7 @@ __global__ __launch_bounds__(256) void k_clear(AccumDev A, uint32_t pixels) {
     }
     if (A.sum64)
         for (int k = 0; k < 12; ++k) A.sum64[(size_t)pix * 12u + k] = 0.0;
+    A.m2[pix] = 0.0f;
 }
 
 // Pixels not frozen by adaptive sampling (trace.wgsl:2517).
@@ -2247,6 +2270,10 @@ void launch_develop(hipStream_t s, const AccumDev& A, uint32_t pixels, float4* c
 void launch_clear(hipStream_t s, const AccumDev& A, uint32_t pixels) {
     if (!pixels) return;
     hipLaunchKernelGGL(k_clear, dim3(blocks_for(pixels, 256)), dim3(256), 0, s, A, pixels);
+}
+void launch_deviation(hipStream_t s, const AccumDev& A, uint32_t pixels, float* out) {
+    if (!pixels) return;
+    hipLaunchKernelGGL(k_deviation, dim3(blocks_for(pixels, 256)), dim3(256), 0, s, A, pixels, out);
 }
 void launch_active(hipStream_t s, const AccumDev& A, uint32_t pixels, uint32_t* active, uint32_t* counter) {
     if (!pixels) return;

@@ -163,6 +163,7 @@ void trace_setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& deta
     A.sum = (float4*)t->alloc(pixels * 48, "accumulation");
     A.comp = A.mode == PFXH_ACCUM_COMPENSATED ? (float4*)t->alloc(pixels * 48, "accumulation compensation") : nullptr;
     A.sum64 = A.mode == 2u ? (double*)t->alloc(pixels * 96, "fp64 accumulation") : nullptr;
+    A.m2 = (float*)t->alloc(pixels * 4, "luminance moments");
     t->images = (float4*)t->alloc(pixels * 48, "output images");
     launch_clear(t->stream, A, (uint32_t)pixels);
     if (!with_wave) {
@@ -682,6 +683,9 @@ pfxh_status pfxh_trace_restore(pfxh_trace* t, const char* key, const uint8_t* by
         if (!pixels) return;
         HIP_TRY(hipMemcpy(t->A.sum, c.accum, pixels * 48, hipMemcpyHostToDevice));
         if (t->A.mode == PFXH_ACCUM_COMPENSATED) HIP_TRY(hipMemset(t->A.comp, 0, pixels * 48));
+        // PFXCKPT1 holds no luminance moments: the deviation covers the
+        // samples taken after the restore (it reads 0 until then)
+        HIP_TRY(hipMemset(t->A.m2, 0, pixels * 4));
         if (t->A.mode == 2u) {
             std::vector<double> wide(pixels * 12);
             for (size_t i = 0; i < wide.size(); ++i) {
@@ -702,6 +706,19 @@ pfxh_status pfxh_trace_restore(pfxh_trace* t, const char* key, const uint8_t* by
         t->any_frozen = w < 0.0f;
     }
     return PFXH_OK;
+}
+
+pfxh_status pfxh_trace_readback_deviation(pfxh_trace* t, float* out, char* err, size_t err_len) {
+    if (!t || !out) return PFXH_INVALID;
+    size_t pixels = (size_t)t->width * t->height;
+    return guarded(err, err_len, [&] {
+        if (!pixels) return;
+        if (!t->deviation) t->deviation = (float*)t->alloc(pixels * 4, "deviation image");
+        launch_deviation(t->stream, t->A, (uint32_t)pixels, t->deviation);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out, t->deviation, pixels * 4, hipMemcpyDeviceToHost, t->stream));
+        HIP_TRY(hipStreamSynchronize(t->stream));
+    });
 }
 
 pfxh_status pfxh_trace_denoise(pfxh_trace* t, pfxh_denoiser* d, float* out, char* err, size_t err_len) {

@@ -70,6 +70,7 @@ struct pfxh_trace {
     bool probe_fresh = false;           // probe holds the current view's hits
     float* tile_buf = nullptr;          // per-tile check of the learned denoiser
     float4* atrous = nullptr;           // a-trous passes: two images
+    float* deviation = nullptr;         // per-pixel standard error of the mean luminance (lazily)
     uint32_t tile_cap = 0;
     uint32_t width = 0, height = 0, samples = 0;
     uint32_t capacity = 0;

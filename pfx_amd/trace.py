@@ -140,6 +140,14 @@ class Trace:
         self._check(N.lib().pfxh_trace_readback(self._handle(), color.ctypes.data, None, None, err, len(err)), err)
         return color
 
+    def readback_deviation(self) -> np.ndarray:
+        """The standard error of each pixel's mean luminance ((r + g + b) / 3),
+        (H, W) float32: the learned denoiser's noise channel."""
+        out = np.empty((self.height, self.width), np.float32)
+        err = N.err_buffer()
+        self._check(N.lib().pfxh_trace_readback_deviation(self._handle(), out.ctypes.data, err, len(err)), err)
+        return out
+
     def set_camera(self, camera: Camera) -> None:
         err = N.err_buffer()
         c = camera.to_c()

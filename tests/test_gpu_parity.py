@@ -257,3 +257,28 @@ def test_checkpoint_resume_continues_the_render(accumulate):
             t.restore("still", data)
         with pytest.raises(RuntimeError, match="another render"):
             t.restore("other", data)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_deviation_matches_oracle(case):
+    """The learned denoiser's noise channel (pfxh_trace_readback_deviation):
+    the standard error of each pixel's mean luminance, from the luminance
+    moments both sides sum in sample order, over two calls of 4 and 12 spp."""
+    make, bounces, w, h = CASES[case]
+    scene = make(w, h)
+    detail = Detail(bounces=bounces)
+    print(f"deviation parity {case} {w}x{h}", flush=True)
+    o = Oracle(scene, detail, w, h)
+    try:
+        o.sample(4, 3)
+        o.sample(12, 4)
+        ref = o.deviation()
+    finally:
+        o.close()
+    with Trace.new_detailed(scene, detail, w, h) as t:
+        t.sample(4, 3)
+        t.sample(12, 4)
+        got = t.readback_deviation()
+    print(f"  exact {float(np.mean(got == ref)):.4f}, max |gap| {float(np.max(np.abs(got - ref))):.3g}", flush=True)
+    assert np.all(np.isfinite(got)) and np.all(got >= 0.0)
+    np.testing.assert_array_equal(got, ref)  # the same sums in the same order
