@@ -660,10 +660,11 @@ void pfxh_batch_free(pfxh_batch* batch);
  * No counterpart in the reference, which runs only the fixed-weight filter
  * above (crates/denoise/src/lib.rs:15-17). A 5-level encoder-decoder of 16
  * 3x3 convolutions with skip connections (the U-Net shape of net.rs:202-232,
- * 32->128 channels, 1,238,787 parameters), BF16 operands on matrix cores
+ * 32->128 channels, 1,239,651 parameters), BF16 operands on matrix cores
  * with FP32 sums, trained on the GPU (forward and backward kernels, L1 loss
  * on log radiance, Adam) from pairs the tracer renders. Inputs are the
- * still's colour, albedo, normal and depth (10 channels); the output is
+ * still's colour, albedo, normal, depth and the standard error of each
+ * pixel's mean luminance (11 channels); the output is
  * colour. Parameters cross the boundary in a canonical order: per layer,
  * weights [out][in][3][3] (the inputs being the first source's channels,
  * then the skip source's) and then the bias [out]. */
@@ -713,10 +714,12 @@ void pfxh_learner_reset_optimizer(pfxh_learner* learner);
 pfxh_status pfxh_learner_reserve(pfxh_learner* learner, uint32_t side, uint32_t capacity, char* err,
                                  size_t err_len);
 /* Add one crop: RGBA32F colour, albedo, normal and target (side*side*4
- * floats each) and depth (side*side floats, or NULL). */
+ * floats each), depth and deviation (side*side floats each, or NULL; the
+ * deviation is the standard error of each pixel's mean luminance, as
+ * pfxh_trace_readback_deviation gives it). */
 pfxh_status pfxh_learner_add(pfxh_learner* learner, const float* color, const float* albedo,
-                             const float* normal, const float* depth, const float* target, char* err,
-                             size_t err_len);
+                             const float* normal, const float* depth, const float* deviation,
+                             const float* target, char* err, size_t err_len);
 uint32_t pfxh_learner_crops(const pfxh_learner* learner);
 /* Loss and gradient on `batch` crops, no update: picks[b] = crop << 3 |
  * symmetry (bit 0 mirrors x, bit 1 mirrors y, bit 2 transposes). */
@@ -726,13 +729,15 @@ pfxh_status pfxh_learner_gradient(pfxh_learner* learner, const uint32_t* picks, 
  * training set with `seed`; losses[k] gets step k's mean loss. */
 pfxh_status pfxh_learner_train(pfxh_learner* learner, uint32_t steps, uint32_t batch, float lr,
                                uint64_t seed, float* losses, char* err, size_t err_len);
-/* Denoise host images (RGBA32F colour, albedo, normal; depth NULL or
- * width*height floats); `out` gets RGBA32F, alpha from the colour. */
+/* Denoise host images (RGBA32F colour, albedo, normal; depth and
+ * deviation NULL or width*height floats); `out` gets RGBA32F, alpha from
+ * the colour. */
 pfxh_status pfxh_learned_denoise(pfxh_learner* learner, uint32_t width, uint32_t height,
                                  const float* color, const float* albedo, const float* normal,
-                                 const float* depth, float* out, char* err, size_t err_len);
-/* The same on a trace's current colour, albedo and normal and its probe's
- * depth, without a host round trip for the inputs. */
+                                 const float* depth, const float* deviation, float* out, char* err,
+                                 size_t err_len);
+/* The same on a trace's current colour, albedo and normal, its probe's
+ * depth and its deviation, without a host round trip for the inputs. */
 pfxh_status pfxh_trace_learned_denoise(pfxh_trace* trace, pfxh_learner* learner, float* out, char* err,
                                        size_t err_len);
 /* The same with the repo denoiser as fallback: per `tile` x `tile` tile the

@@ -38,7 +38,7 @@ const L NET[16] = {
     {12, -1, 64, 1, false, false, true}, {13, 0, 64, 0, true, false, true},  {14, -1, 32, 0, false, false, true},
     {15, -1, 3, 0, false, false, false},
 };
-const int IN_CH = 10;
+const int IN_CH = 11;
 
 int chans(int t) { return t == 0 ? IN_CH : NET[t - 1].out; }
 int cin_of(int k) { return chans(NET[k].in0) + (NET[k].in1 >= 0 ? chans(NET[k].in1) : 0); }
@@ -119,7 +119,7 @@ struct Pass {
 };
 
 void encode(int b, int h, int w, int h16, int w16, const float* color, const float* albedo, const float* normal,
-            const float* depth, T& x) {
+            const float* depth, const float* deviation, T& x) {
     x.make(b, h16, w16, IN_CH);
     for (int n = 0; n < b; ++n)
         for (int y = 0; y < h; ++y)
@@ -133,6 +133,7 @@ void encode(int b, int h, int w, int h16, int w16, const float* color, const flo
                 for (int k = 0; k < 3; ++k) x.at(n, y, xx, 3 + k) = fin(a[k]);
                 for (int k = 0; k < 3; ++k) x.at(n, y, xx, 6 + k) = fin(m[k]) * 0.5 + 0.5;
                 x.at(n, y, xx, 9) = log1p(fmax(d, 0.0)) * 0.125;
+                x.at(n, y, xx, 10) = deviation ? log1p(fmax(fin(deviation[p]), 0.0)) : 0.0;
             }
 }
 
@@ -184,12 +185,12 @@ void or_learn_set_loss(int mode) { g_loss_mode = mode; }
 
 // crops of h x w pixels (multiples of 16) stored one after another.
 double or_learn_grad(const double* params, uint32_t batch, uint32_t h, uint32_t w, const float* color,
-                     const float* albedo, const float* normal, const float* depth, const float* target,
-                     double* grad) {
+                     const float* albedo, const float* normal, const float* depth, const float* deviation,
+                     const float* target, double* grad) {
     Net net(params);
     Pass ps;
     ps.t.resize(17);
-    encode((int)batch, (int)h, (int)w, (int)h, (int)w, color, albedo, normal, depth, ps.t[0]);
+    encode((int)batch, (int)h, (int)w, (int)h, (int)w, color, albedo, normal, depth, deviation, ps.t[0]);
     forward(net, ps);
     const T& y = ps.t[16];
     size_t px = (size_t)batch * h * w;
@@ -308,12 +309,12 @@ double or_learn_grad(const double* params, uint32_t batch, uint32_t h, uint32_t 
 
 // Inference on one image of any size: RGBA out (alpha from the colour).
 void or_learn_forward(const double* params, uint32_t h, uint32_t w, const float* color, const float* albedo,
-                      const float* normal, const float* depth, float* out) {
+                      const float* normal, const float* depth, const float* deviation, float* out) {
     Net net(params);
     Pass ps;
     ps.t.resize(17);
     int h16 = (int)((h + 15) / 16 * 16), w16 = (int)((w + 15) / 16 * 16);
-    encode(1, (int)h, (int)w, h16, w16, color, albedo, normal, depth, ps.t[0]);
+    encode(1, (int)h, (int)w, h16, w16, color, albedo, normal, depth, deviation, ps.t[0]);
     forward(net, ps);
     const T& y = ps.t[16];
     for (uint32_t yy = 0; yy < h; ++yy)

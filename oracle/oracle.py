@@ -39,9 +39,9 @@ def lib() -> C.CDLL:
             "or_atrous": (C.c_int, [_U, _U, _P, _P, _P, _U, _P]),
             "or_learn_param_count": (C.c_uint64, []),
             "or_learn_layer": (None, [_U, _P]),
-            "or_learn_grad": (C.c_double, [_P, _U, _U, _U, _P, _P, _P, _P, _P, _P]),
+            "or_learn_grad": (C.c_double, [_P, _U, _U, _U, _P, _P, _P, _P, _P, _P, _P]),
             "or_learn_set_loss": (None, [C.c_int]),
-            "or_learn_forward": (None, [_P, _U, _U, _P, _P, _P, _P, _P]),
+            "or_learn_forward": (None, [_P, _U, _U, _P, _P, _P, _P, _P, _P]),
             "or_scene_free": (None, [_P]),
             "or_set_camera": (None, [_P, _P]),
             "or_set_sun_radius": (None, [_P, _F]),
@@ -238,27 +238,29 @@ def _f32(a):
     return None if a is None else np.ascontiguousarray(a, np.float32)
 
 
-def learn_grad(params, color, albedo, normal, depth, target, want_grad: bool = True, loss: str = "l1_log"):
-    """Loss and gradient for crops shaped (batch, h, w, 4) (depth (batch, h, w));
-    loss 'l1_log' or 'rel_l2'."""
+def learn_grad(params, color, albedo, normal, depth, target, deviation=None, want_grad: bool = True,
+               loss: str = "l1_log"):
+    """Loss and gradient for crops shaped (batch, h, w, 4) (depth and
+    deviation (batch, h, w) or None); loss 'l1_log' or 'rel_l2'."""
     lib().or_learn_set_loss({"l1_log": 0, "rel_l2": 1}[loss])
     p = np.ascontiguousarray(params, np.float64)
-    color, albedo, normal, depth, target = (_f32(a) for a in (color, albedo, normal, depth, target))
+    color, albedo, normal, depth, deviation, target = (_f32(a) for a in (color, albedo, normal, depth, deviation,
+                                                                           target))
     b, h, w = color.shape[:3]
     grad = np.zeros(len(p), np.float64) if want_grad else None
     loss = lib().or_learn_grad(p.ctypes.data, b, h, w, color.ctypes.data, albedo.ctypes.data, normal.ctypes.data,
-                               _ptr(depth), target.ctypes.data, _ptr(grad))
+                               _ptr(depth), _ptr(deviation), target.ctypes.data, _ptr(grad))
     return float(loss), grad
 
 
-def learn_forward(params, color, albedo, normal, depth):
+def learn_forward(params, color, albedo, normal, depth, deviation=None):
     """Denoise one (h, w, 4) image; returns (h, w, 4) float32."""
     p = np.ascontiguousarray(params, np.float64)
-    color, albedo, normal, depth = (_f32(a) for a in (color, albedo, normal, depth))
+    color, albedo, normal, depth, deviation = (_f32(a) for a in (color, albedo, normal, depth, deviation))
     h, w = color.shape[:2]
     out = np.zeros((h, w, 4), np.float32)
     lib().or_learn_forward(p.ctypes.data, h, w, color.ctypes.data, albedo.ctypes.data, normal.ctypes.data,
-                           _ptr(depth), out.ctypes.data)
+                           _ptr(depth), _ptr(deviation), out.ctypes.data)
     return out
 
 

@@ -295,11 +295,11 @@ SIGNATURES = {
     "pfxh_learner_grads": (C.c_int, [_P, _P, C.c_uint64, _S, _Z]),
     "pfxh_learner_reset_optimizer": (None, [_P]),
     "pfxh_learner_reserve": (C.c_int, [_P, _U, _U, _S, _Z]),
-    "pfxh_learner_add": (C.c_int, [_P, _P, _P, _P, _P, _P, _S, _Z]),
+    "pfxh_learner_add": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _S, _Z]),
     "pfxh_learner_crops": (_U, [_P]),
     "pfxh_learner_gradient": (C.c_int, [_P, _P, _U, C.POINTER(C.c_float), _S, _Z]),
     "pfxh_learner_train": (C.c_int, [_P, _U, _U, C.c_float, C.c_uint64, _P, _S, _Z]),
-    "pfxh_learned_denoise": (C.c_int, [_P, _U, _U, _P, _P, _P, _P, _P, _S, _Z]),
+    "pfxh_learned_denoise": (C.c_int, [_P, _U, _U, _P, _P, _P, _P, _P, _P, _S, _Z]),
     "pfxh_trace_learned_denoise": (C.c_int, [_P, _P, _P, _S, _Z]),
     "pfxh_trace_learned_denoise_checked": (C.c_int, [_P, _P, _P, C.c_float, _U, _P, C.POINTER(_U), _S, _Z]),
     "pfxh_learner_free": (None, [_P]),

@@ -35,7 +35,8 @@ void scatter(hipStream_t s, const float* dx, uint32_t b, uint32_t h, uint32_t w,
              float* d0, float* d1);
 void weight_grads(hipStream_t s, const DwArgs& a, const DwPlan& p, bool bf16, float* dw, float* db);
 void learn_in(hipStream_t s, const float4* color, const float4* albedo, const float4* normal, const float* depth,
-              uint32_t depth_stride, uint32_t w, uint32_t h, uint32_t w16, uint32_t h16, float* x);
+              uint32_t depth_stride, const float* deviation, uint32_t w, uint32_t h, uint32_t w16, uint32_t h16,
+              float* x);
 uint32_t loss(hipStream_t s, const float* y, const float4* target, size_t n, uint32_t mode, float* dy, float* sums);
 void loss_total(hipStream_t s, const float* sums, uint32_t nb, size_t n, float* out);
 void learn_out(hipStream_t s, const float* y, uint32_t w16, const float4* color, uint32_t w, uint32_t h, float4* out);

@@ -466,11 +466,13 @@ void weight_grads(hipStream_t s, const DwArgs& a, const DwPlan& p, bool bf16, fl
 }
 
 // ---- encoding, loss, decoding, batches ---------------------------------------------
-// channels: log(1 + colour) x3, albedo x3, normal * 0.5 + 0.5 x3, log(1 + depth) / 8;
+// channels: log(1 + colour) x3, albedo x3, normal * 0.5 + 0.5 x3, log(1 + depth) / 8,
+// log(1 + the standard error of the pixel's mean luminance);
 // the network's frame is the image padded to multiples of 16 with zeros
 __device__ __forceinline__ float finite_or0(float v) { return v == v && fabsf(v) < 1e30f ? v : 0.0f; }
 __global__ void k_learn_in(const float4* color, const float4* albedo, const float4* normal, const float* depth,
-                           uint32_t depth_stride, uint32_t w, uint32_t h, uint32_t w16, uint32_t h16, float* x) {
+                           uint32_t depth_stride, const float* deviation, uint32_t w, uint32_t h, uint32_t w16,
+                           uint32_t h16, float* x) {
     size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (q >= (size_t)w16 * h16) return;
     uint32_t px = q % w16, py = (uint32_t)(q / w16);
@@ -489,6 +491,7 @@ __global__ void k_learn_in(const float4* color, const float4* albedo, const floa
         v[7] = finite_or0(m.y) * 0.5f + 0.5f;
         v[8] = finite_or0(m.z) * 0.5f + 0.5f;
         v[9] = log1pf(fmaxf(d, 0.0f)) * 0.125f;
+        v[10] = deviation ? log1pf(fmaxf(finite_or0(deviation[p]), 0.0f)) : 0.0f;
     }
     float4* o = reinterpret_cast<float4*>(x + q * 16);
     for (int k = 0; k < 4; ++k) o[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
@@ -564,9 +567,9 @@ __global__ void k_gather(const float* inputs, const float4* targets, size_t crop
     t[e] = targets[src];
 }
 void learn_in(hipStream_t s, const float4* c, const float4* a, const float4* nm, const float* d, uint32_t ds,
-              uint32_t w, uint32_t h, uint32_t w16, uint32_t h16, float* x) {
-    hipLaunchKernelGGL(k_learn_in, dim3(blocks((size_t)w16 * h16, 256)), dim3(256), 0, s, c, a, nm, d, ds, w, h, w16,
-                       h16, x);
+              const float* dev, uint32_t w, uint32_t h, uint32_t w16, uint32_t h16, float* x) {
+    hipLaunchKernelGGL(k_learn_in, dim3(blocks((size_t)w16 * h16, 256)), dim3(256), 0, s, c, a, nm, d, ds, dev, w,
+                       h, w16, h16, x);
 }
 uint32_t loss(hipStream_t s, const float* y, const float4* t, size_t n, uint32_t mode, float* dy, float* sums) {
     uint32_t nb = blocks(n, 256);

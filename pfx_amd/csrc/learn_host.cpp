@@ -5,7 +5,7 @@
 // The network is a 5-level encoder-decoder of 16 3x3 convolutions with skip
 // connections, shaped like the U-Net of OIDN's RT filter that the repo
 // denoiser runs (crates/denoise/src/net.rs:202-232) but 32->128 channels
-// wide: 1,238,787 parameters. The reference has no trained denoiser; this is
+// wide: 1,239,651 parameters. The reference has no trained denoiser; this is
 // the part its still path lacks.
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -70,7 +70,7 @@ struct Def {
     uint32_t out, level;
     bool up, pool, relu;
 };
-constexpr uint32_t INPUT_CHANNELS = 10;
+constexpr uint32_t INPUT_CHANNELS = 11;
 constexpr Def NET[PFXH_LEARN_LAYERS] = {
     {"enc_conv0", 0, -1, 32, 0, false, false, true},    {"enc_conv1", 1, -1, 32, 0, false, true, true},
     {"enc_conv2", 2, -1, 64, 1, false, true, true},     {"enc_conv3", 3, -1, 96, 2, false, true, true},
@@ -150,6 +150,7 @@ struct pfxh_learner {
         float4* target = nullptr;
         float4* images = nullptr;  // inference: colour, albedo, normal, output
         float* depth = nullptr;
+        float* deviation = nullptr;  // inference: the noise channel's input
         uint32_t* picks = nullptr;
         float* losses = nullptr;
         uint32_t pick_cap = 0, loss_cap = 0;
@@ -265,6 +266,7 @@ struct pfxh_learner {
         } else {
             f.images = (float4*)dalloc(f.mem, px * 16 * 4, "images");
             f.depth = (float*)dalloc(f.mem, px * 4, "depth");
+            f.deviation = (float*)dalloc(f.mem, px * 4, "deviation");
         }
         f.batch = batch;
         f.h = h;
@@ -381,11 +383,12 @@ struct pfxh_learner {
     }
 
     void infer(uint32_t width, uint32_t height, const float4* color, const float4* albedo, const float4* normal,
-               const float* depth, uint32_t depth_stride, float4* out) {
+               const float* depth, uint32_t depth_stride, const float* deviation, float4* out) {
         uint32_t w16 = pad16(width), h16 = pad16(height);
         frame(infer_frame, 1, h16, w16, false);
         Frame& f = infer_frame;
-        pfxl::learn_in(stream, color, albedo, normal, depth, depth_stride, width, height, w16, h16, f.t[0]);
+        pfxl::learn_in(stream, color, albedo, normal, depth, depth_stride, deviation, width, height, w16, h16,
+                       f.t[0]);
         forward(f);
         pfxl::learn_out(stream, f.t[TENSORS - 1], w16, color, width, height, out);
         hip_check(hipGetLastError(), "learned denoise launch");
@@ -401,13 +404,14 @@ struct pfxh_learner {
 };
 
 // Internal: the learned denoiser on device images of `width` x `height`
-// (depth with a stride of `depth_stride` floats), written on `after`; the
+// (depth with a stride of `depth_stride` floats; deviation, the standard
+// error of each pixel's mean luminance, or null), written on `after`; the
 // result is copied to the host buffer `out`, or with `out` null left on the
 // device at *device_out (valid until the next call).
 pfxh_status pfxh_learned_denoise_device(pfxh_learner* l, uint32_t width, uint32_t height, const void* color,
                                         const void* albedo, const void* normal, const float* depth,
-                                        uint32_t depth_stride, float* out, hipStream_t after, char* err,
-                                        size_t err_len, void** device_out) {
+                                        uint32_t depth_stride, const float* deviation, float* out,
+                                        hipStream_t after, char* err, size_t err_len, void** device_out) {
     return guard(err, err_len, [&] {
         if (width == 0 || height == 0) throw Failure{PFXH_INVALID, "image dimensions must be nonzero"};
         if (after) hip_check(hipStreamSynchronize(after), "learned denoise input");
@@ -415,7 +419,7 @@ pfxh_status pfxh_learned_denoise_device(pfxh_learner* l, uint32_t width, uint32_
         l->frame(l->infer_frame, 1, h16, w16, false);
         float4* o = l->infer_frame.images + 3 * (size_t)w16 * h16;
         l->infer(width, height, (const float4*)color, (const float4*)albedo, (const float4*)normal, depth,
-                 depth_stride, o);
+                 depth_stride, deviation, o);
         if (device_out) *device_out = o;
         if (out)
             hip_check(hipMemcpyAsync(out, o, (size_t)width * height * 16, hipMemcpyDeviceToHost, l->stream),
@@ -525,14 +529,15 @@ pfxh_status pfxh_learner_reserve(pfxh_learner* l, uint32_t side, uint32_t capaci
         size_t px = (size_t)side * side;
         l->inputs = (float*)l->dalloc(l->mem, px * capacity * 16 * 4, "training inputs");
         l->targets = (float4*)l->dalloc(l->mem, px * capacity * 16, "training targets");
-        l->staging = (float4*)l->dalloc(l->mem, px * 16 * 3 + px * 4, "crop staging");
+        l->staging = (float4*)l->dalloc(l->mem, px * 16 * 3 + px * 8, "crop staging");
         l->side = side;
         l->capacity = capacity;
     });
 }
 
 pfxh_status pfxh_learner_add(pfxh_learner* l, const float* color, const float* albedo, const float* normal,
-                             const float* depth, const float* target, char* err, size_t err_len) {
+                             const float* depth, const float* deviation, const float* target, char* err,
+                             size_t err_len) {
     if (!l || !color || !albedo || !normal || !target) return PFXH_INVALID;
     return guard(err, err_len, [&] {
         if (!l->inputs) throw Failure{PFXH_INVALID, "no training set reserved"};
@@ -540,13 +545,15 @@ pfxh_status pfxh_learner_add(pfxh_learner* l, const float* color, const float* a
         size_t px = (size_t)l->side * l->side, bytes = px * 16;
         float4* st = l->staging;
         float* dep = depth ? (float*)(st + 3 * px) : nullptr;
+        float* dev = deviation ? (float*)(st + 3 * px) + px : nullptr;
         hip_check(hipMemcpyAsync(st, color, bytes, hipMemcpyHostToDevice, l->stream), "crop upload");
         hip_check(hipMemcpyAsync(st + px, albedo, bytes, hipMemcpyHostToDevice, l->stream), "crop upload");
         hip_check(hipMemcpyAsync(st + 2 * px, normal, bytes, hipMemcpyHostToDevice, l->stream), "crop upload");
         if (dep) hip_check(hipMemcpyAsync(dep, depth, px * 4, hipMemcpyHostToDevice, l->stream), "crop upload");
+        if (dev) hip_check(hipMemcpyAsync(dev, deviation, px * 4, hipMemcpyHostToDevice, l->stream), "crop upload");
         hip_check(hipMemcpyAsync(l->targets + px * l->crops, target, bytes, hipMemcpyHostToDevice, l->stream),
                   "crop upload");
-        pfxl::learn_in(l->stream, st, st + px, st + 2 * px, dep, 1, l->side, l->side, l->side, l->side,
+        pfxl::learn_in(l->stream, st, st + px, st + 2 * px, dep, 1, dev, l->side, l->side, l->side, l->side,
                        l->inputs + px * 16 * l->crops);
         hip_check(hipGetLastError(), "crop encode");
         hip_check(hipStreamSynchronize(l->stream), "crop upload");
@@ -605,8 +612,8 @@ pfxh_status pfxh_learner_train(pfxh_learner* l, uint32_t steps, uint32_t batch, 
 }
 
 pfxh_status pfxh_learned_denoise(pfxh_learner* l, uint32_t width, uint32_t height, const float* color,
-                                 const float* albedo, const float* normal, const float* depth, float* out, char* err,
-                                 size_t err_len) {
+                                 const float* albedo, const float* normal, const float* depth, const float* deviation,
+                                 float* out, char* err, size_t err_len) {
     if (!l || !color || !albedo || !normal || !out) return PFXH_INVALID;
     return guard(err, err_len, [&] {
         if (width == 0 || height == 0) throw Failure{PFXH_INVALID, "image dimensions must be nonzero"};
@@ -620,7 +627,10 @@ pfxh_status pfxh_learned_denoise(pfxh_learner* l, uint32_t width, uint32_t heigh
         hip_check(hipMemcpyAsync(im + 2 * cap, normal, bytes, hipMemcpyHostToDevice, l->stream),
                   "learned denoise upload");
         if (depth) hip_check(hipMemcpyAsync(f.depth, depth, px * 4, hipMemcpyHostToDevice, l->stream), "upload");
-        l->infer(width, height, im, im + cap, im + 2 * cap, depth ? f.depth : nullptr, 1, im + 3 * cap);
+        if (deviation)
+            hip_check(hipMemcpyAsync(f.deviation, deviation, px * 4, hipMemcpyHostToDevice, l->stream), "upload");
+        l->infer(width, height, im, im + cap, im + 2 * cap, depth ? f.depth : nullptr, 1,
+                 deviation ? f.deviation : nullptr, im + 3 * cap);
         hip_check(hipMemcpyAsync(out, im + 3 * cap, bytes, hipMemcpyDeviceToHost, l->stream),
                   "learned denoise readback");
         hip_check(hipStreamSynchronize(l->stream), "learned denoise");

@@ -23,8 +23,8 @@ pfxh_status pfxh_denoise_device(pfxh_denoiser* d, uint32_t width, uint32_t heigh
                                 size_t err_len, const void** device_out = nullptr);
 pfxh_status pfxh_learned_denoise_device(pfxh_learner* l, uint32_t width, uint32_t height, const void* color,
                                         const void* albedo, const void* normal, const float* depth,
-                                        uint32_t depth_stride, float* out, hipStream_t after, char* err,
-                                        size_t err_len, void** device_out = nullptr);
+                                        uint32_t depth_stride, const float* deviation, float* out,
+                                        hipStream_t after, char* err, size_t err_len, void** device_out = nullptr);
 #include "learn.h"
 hipError_t pfxh_atrous_device(hipStream_t s, float4* a, float4* b, const float4* albedo, const float4* normal,
                               uint32_t w, uint32_t h, uint32_t samples);
@@ -742,6 +742,8 @@ pfxh_status pfxh_trace_learned_denoise(pfxh_trace* t, pfxh_learner* l, float* ou
         if (!pixels) return;
         float4* c = t->images;
         launch_develop(t->stream, t->A, (uint32_t)pixels, c, c + pixels, c + 2 * pixels);
+        if (!t->deviation) t->deviation = (float*)t->alloc(pixels * 4, "deviation image");
+        launch_deviation(t->stream, t->A, (uint32_t)pixels, t->deviation);
         if (!t->probe) t->probe = (pfxh_probe_hit*)t->alloc(pixels * sizeof(pfxh_probe_hit), "probe");
         if (!t->probe_fresh) {  // depth depends on the view only: kept until a setter changes it
             launch_probe(t->stream, t->S, t->W, t->probe);
@@ -753,7 +755,8 @@ pfxh_status pfxh_trace_learned_denoise(pfxh_trace* t, pfxh_learner* l, float* ou
     float4* c = t->images;
     static_assert(offsetof(pfxh_probe_hit, depth) == 0 && sizeof(pfxh_probe_hit) % 4 == 0, "probe layout");
     return pfxh_learned_denoise_device(l, t->width, t->height, c, c + pixels, c + 2 * pixels, (const float*)t->probe,
-                                       (uint32_t)(sizeof(pfxh_probe_hit) / 4), out, t->stream, err, err_len);
+                                       (uint32_t)(sizeof(pfxh_probe_hit) / 4), t->deviation, out, t->stream, err,
+                                       err_len);
 }
 
 pfxh_status pfxh_trace_learned_denoise_checked(pfxh_trace* t, pfxh_learner* l, pfxh_denoiser* d, float threshold,
@@ -766,6 +769,8 @@ pfxh_status pfxh_trace_learned_denoise_checked(pfxh_trace* t, pfxh_learner* l, p
     pfxh_status st = guarded(err, err_len, [&] {
         float4* c = t->images;
         launch_develop(t->stream, t->A, (uint32_t)pixels, c, c + pixels, c + 2 * pixels);
+        if (!t->deviation) t->deviation = (float*)t->alloc(pixels * 4, "deviation image");
+        launch_deviation(t->stream, t->A, (uint32_t)pixels, t->deviation);
         if (!t->probe) t->probe = (pfxh_probe_hit*)t->alloc(pixels * sizeof(pfxh_probe_hit), "probe");
         if (!t->probe_fresh) {  // depth depends on the view only: kept until a setter changes it
             launch_probe(t->stream, t->S, t->W, t->probe);
@@ -777,8 +782,8 @@ pfxh_status pfxh_trace_learned_denoise_checked(pfxh_trace* t, pfxh_learner* l, p
     float4* c = t->images;
     void* learned = nullptr;
     st = pfxh_learned_denoise_device(l, t->width, t->height, c, c + pixels, c + 2 * pixels, (const float*)t->probe,
-                                     (uint32_t)(sizeof(pfxh_probe_hit) / 4), nullptr, t->stream, err, err_len,
-                                     &learned);
+                                     (uint32_t)(sizeof(pfxh_probe_hit) / 4), t->deviation, nullptr, t->stream, err,
+                                     err_len, &learned);
     if (st != PFXH_OK) return st;
     uint32_t tiles_x = (t->width + tile - 1) / tile, tiles = tiles_x * ((t->height + tile - 1) / tile);
     std::vector<float> r(tiles);

@@ -1,6 +1,6 @@
 """`Learner`: the learned denoiser over libpfxhip -- a 5-level encoder-decoder
 conv net (16 3x3 convolutions with skip connections, 32->128 channels,
-1,238,787 parameters) trained on the GPU by the library's own forward and
+1,239,651 parameters) trained on the GPU by the library's own forward and
 backward kernels from pairs the tracer renders, BF16 operands on matrix
 cores with FP32 sums.
 
@@ -135,13 +135,18 @@ class Learner:
         self._check(N.lib().pfxh_learner_reserve(self._h, side, capacity, err, len(err)), err)
         self.side = side
 
-    def add(self, color, albedo, normal, depth, target) -> None:
+    def add(self, color, albedo, normal, depth, target, deviation=None) -> None:
+        """One training crop; depth and deviation (the standard error of each
+        pixel's mean luminance, Trace.readback_deviation) are (side, side)
+        or None."""
         s = self.side
         c, a, n, t = (_img(x, s, s) for x in (color, albedo, normal, target))
         d = None if depth is None else _img(depth, s, s, None)
+        v = None if deviation is None else _img(deviation, s, s, None)
         err = N.err_buffer()
         st = N.lib().pfxh_learner_add(self._h, c.ctypes.data, a.ctypes.data, n.ctypes.data,
-                                      None if d is None else d.ctypes.data, t.ctypes.data, err, len(err))
+                                      None if d is None else d.ctypes.data, None if v is None else v.ctypes.data,
+                                      t.ctypes.data, err, len(err))
         self._check(st, err)
 
     @property
@@ -162,14 +167,16 @@ class Learner:
         self._check(st, err)
         return losses[:steps]
 
-    def denoise(self, color, albedo, normal, depth=None) -> np.ndarray:
+    def denoise(self, color, albedo, normal, depth=None, deviation=None) -> np.ndarray:
         h, w = np.shape(color)[:2]
         c, a, n = (_img(x, h, w) for x in (color, albedo, normal))
         d = None if depth is None else _img(depth, h, w, None)
+        v = None if deviation is None else _img(deviation, h, w, None)
         out = np.empty_like(c)
         err = N.err_buffer()
         st = N.lib().pfxh_learned_denoise(self._h, w, h, c.ctypes.data, a.ctypes.data, n.ctypes.data,
-                                          None if d is None else d.ctypes.data, out.ctypes.data, err, len(err))
+                                          None if d is None else d.ctypes.data, None if v is None else v.ctypes.data,
+                                          out.ctypes.data, err, len(err))
         self._check(st, err)
         return out
 

@@ -52,11 +52,12 @@ def views(rng: np.random.Generator, scene, count: int, aspect: float) -> list[Ca
 
 
 def frame_pairs(trace: Trace, camera: Camera, input_spp: int, target_spp: int, seed: int):
-    """(color, albedo, normal, depth) at input_spp and the target colour at
-    target_spp, from independent sample sequences."""
+    """(color, albedo, normal, depth, deviation) at input_spp and the target
+    colour at target_spp, from independent sample sequences."""
     trace.set_camera(camera)
     trace.sample(input_spp, seed)
     out = trace.readback()
+    deviation = trace.readback_deviation()
     depth = trace.probe().depth.astype(np.float32)
     trace.set_camera(camera)
     left, chunk = target_spp, 64
@@ -64,7 +65,7 @@ def frame_pairs(trace: Trace, camera: Camera, input_spp: int, target_spp: int, s
         trace.sample(min(chunk, left), seed + 7919)
         left -= chunk
     target = trace.readback_color()
-    return out.color, out.albedo, out.normal, depth, target
+    return out.color, out.albedo, out.normal, depth, deviation, target
 
 
 def generate(learner: Lm.Learner, crops: int, side: int, input_spp, target_spp: int, seed: int,
@@ -87,14 +88,14 @@ def generate(learner: Lm.Learner, crops: int, side: int, input_spp, target_spp: 
             for k, cam in enumerate(views(rng, scene, views_per_scene, fw / fh)):
                 t1 = time.perf_counter()
                 spp = int(spps[int(rng.integers(0, len(spps)))])
-                color, albedo, normal, depth, target = frame_pairs(trace, cam, spp, target_spp,
-                                                                   int(rng.integers(1, 2**31)))
+                color, albedo, normal, depth, deviation, target = frame_pairs(trace, cam, spp, target_spp,
+                                                                              int(rng.integers(1, 2**31)))
                 for cy in range(fh // side):
                     for cx in range(fw // side):
                         if added >= crops:
                             break
                         sl = (slice(cy * side, (cy + 1) * side), slice(cx * side, (cx + 1) * side))
-                        learner.add(color[sl], albedo[sl], normal[sl], depth[sl], target[sl])
+                        learner.add(color[sl], albedo[sl], normal[sl], depth[sl], target[sl], deviation[sl])
                         added += 1
                 log(f"  view {k} ({spp} spp): {per_frame} crops in {time.perf_counter() - t1:.1f} s, {added}/{crops}")
                 if added >= crops or (deadline and time.perf_counter() > deadline):
@@ -113,7 +114,7 @@ def synthetic(learner: Lm.Learner, crops: int, side: int, seed: int) -> int:
         shape = (side, side, 4)
         learner.add(rng.exponential(0.5, shape).astype(np.float32), rng.uniform(0, 1, shape).astype(np.float32),
                     rng.uniform(-1, 1, shape).astype(np.float32), rng.uniform(0, 50, (side, side)).astype(np.float32),
-                    rng.exponential(0.5, shape).astype(np.float32))
+                    rng.exponential(0.5, shape).astype(np.float32), rng.exponential(0.1, (side, side)).astype(np.float32))
     return crops
 
 

@@ -1,10 +1,12 @@
 """The learned denoiser's HIP kernels against the FP64 oracle: loss and
-gradient of every layer (FP32 operands: 1e-5 of each layer's gradient norm,
-measured 6e-7 and 1.3e-6; BF16 operands, the training mode: 0.12 per layer
-and 0.03 over the whole gradient, measured 0.097 and 0.019, the deep layers
+gradient of every layer (FP32 operands: 1e-4 of each layer's gradient norm,
+measured 4.7e-5 and 6.9e-5, where an FP32 pre-activation within rounding of
+zero flips its ReLU against the FP64 oracle at a few pixels (2.6e-7 in layer
+0 with the noise channel zeroed); BF16 operands, the training mode: 0.12 per
+layer and 0.03 over the whole gradient, measured 0.075 and 0.014, the deep layers
 of a 32-pixel crop seeing only 16 to 64 pixels, where single ReLU flips from
 rounding show; the 256-pixel crops of test_gpu_learn_c4 hold 0.05 and
-0.02), the batch gather under all eight
+0.02, measured 0.029 and 0.017), the batch gather under all eight
 symmetries of the square crop, inference on a frame that is not a multiple of
 16, the first Adam step in closed form (p - lr * g / (|g| + eps)), and
 training that lowers the loss."""
@@ -23,7 +25,8 @@ def crops(rng, b, h, w):
     normal = rng.uniform(-1, 1, (b, h, w, 4)).astype(np.float32)
     depth = rng.uniform(0, 20, (b, h, w)).astype(np.float32)
     target = rng.exponential(0.5, (b, h, w, 4)).astype(np.float32)
-    return color, albedo, normal, depth, target
+    deviation = rng.exponential(0.1, (b, h, w)).astype(np.float32)
+    return color, albedo, normal, depth, target, deviation
 
 
 def symmetry(a, sym):
@@ -61,8 +64,8 @@ def filled(learner, data, side):
     return learner
 
 
-@pytest.mark.parametrize("fp32,tol,total,loss", [(True, 1e-5, 1e-5, "l1_log"), (False, 0.12, 0.03, "l1_log"),
-                                                  (True, 1e-5, 1e-5, "rel_l2")])
+@pytest.mark.parametrize("fp32,tol,total,loss", [(True, 1e-4, 1e-4, "l1_log"), (False, 0.12, 0.03, "l1_log"),
+                                                  (True, 1e-4, 1e-4, "rel_l2")])
 def test_gradient_matches_fp64_oracle(fp32, tol, total, loss):
     print(f"learn gradient fp32={fp32} loss={loss}", flush=True)
     rng = np.random.default_rng(21)
@@ -106,11 +109,11 @@ def test_inference_matches_fp64_oracle(fp32, tol):
     print(f"learn inference fp32={fp32}", flush=True)
     rng = np.random.default_rng(23)
     h, w = 24, 40
-    color, albedo, normal, depth, _ = (a[0] for a in crops(rng, 1, h, w))
+    color, albedo, normal, depth, _, deviation = (a[0] for a in crops(rng, 1, h, w))
     p = params_for_test(23)
     with Lm.Learner(fp32=fp32, params=p) as L:
-        out = L.denoise(color, albedo, normal, depth)
-    ref = O.learn_forward(p.astype(np.float64), color, albedo, normal, depth)
+        out = L.denoise(color, albedo, normal, depth, deviation)
+    ref = O.learn_forward(p.astype(np.float64), color, albedo, normal, depth, deviation)
     gap = np.abs(np.log1p(out[..., :3]) - np.log1p(ref[..., :3]))
     assert float(gap.max()) <= tol * max(1.0, float(np.log1p(ref[..., :3]).max()))
     np.testing.assert_array_equal(out[..., 3], color[..., 3])

@@ -31,18 +31,18 @@ def test_tracer_made_crops_batch16_gradient_against_the_fp64_oracle():
         recorded = []
         add = learner.add
 
-        def keep(color, albedo, normal, depth, target):
-            recorded.append([np.array(a, np.float32) for a in (color, albedo, normal, depth, target)])
-            add(color, albedo, normal, depth, target)
+        def keep(color, albedo, normal, depth, target, deviation=None):
+            recorded.append([np.array(a, np.float32) for a in (color, albedo, normal, depth, target, deviation)])
+            add(color, albedo, normal, depth, target, deviation)
 
         learner.add = keep
         print("generating 16 crops of 256^2 from terrain renders", flush=True)
         n = train.generate(learner, crops=16, side=256, input_spp=4, target_spp=64, seed=3, frame=(512, 256),
                            views_per_scene=8)
         assert n == 16 and learner.crops == 16 and len(recorded) == 16
-        for c, a, nr, d, t in recorded:
-            assert c.shape == (256, 256, 4) and d.shape == (256, 256)
-            assert np.all(np.isfinite(c)) and np.all(np.isfinite(t))
+        for c, a, nr, d, t, v in recorded:
+            assert c.shape == (256, 256, 4) and d.shape == (256, 256) and v.shape == (256, 256)
+            assert np.all(np.isfinite(c)) and np.all(np.isfinite(t)) and np.all(v >= 0.0)
         picks = [k << 3 | (k % 8) for k in range(16)]
         loss16 = learner.gradient(picks)
         g16 = learner.grads()

@@ -18,7 +18,8 @@ def crops(rng, b, h, w):
     normal = rng.uniform(-1, 1, (b, h, w, 4)).astype(np.float32)
     depth = rng.uniform(0, 20, (b, h, w)).astype(np.float32)
     target = rng.exponential(0.5, (b, h, w, 4)).astype(np.float32)
-    return color, albedo, normal, depth, target
+    deviation = rng.exponential(0.1, (b, h, w)).astype(np.float32)
+    return color, albedo, normal, depth, target, deviation
 
 
 def layer_slices():
@@ -37,7 +38,7 @@ def test_library_and_oracle_define_the_same_network(native_lib):
     for a, b in zip(lib_layers, orc):
         assert (a["in0"], a["in1"], a["in_channels"], a["out_channels"], a["level"], int(a["upsample"]),
                 int(a["pool"]), int(a["relu"])) == b
-    assert Lm.param_count() == O.learn_param_count() == 1238787
+    assert Lm.param_count() == O.learn_param_count() == 1239651
     # 5 levels, 32 -> 128 channels
     assert {l["level"] for l in lib_layers} == {0, 1, 2, 3, 4}
     assert max(l["out_channels"] for l in lib_layers) == 128 and lib_layers[0]["out_channels"] == 32
@@ -61,7 +62,9 @@ def test_oracle_gradient_matches_finite_differences():
                 p1[i] += eps
                 p2[i] -= eps
                 fd = (O.learn_grad(p1, *data, want_grad=False)[0] - O.learn_grad(p2, *data, want_grad=False)[0]) / (2 * eps)
-                assert abs(fd - g[i]) <= 1e-6 * max(abs(fd), 1e-3), (i, fd, g[i])
+                # central differences at eps 1e-6 carry about 1e-9 of rounding
+                # (the loss sums 768 terms): relative 1e-5, at least 1e-8
+                assert abs(fd - g[i]) <= 1e-5 * max(abs(fd), 1e-3), (i, fd, g[i])
                 checked += 1
     assert checked >= 60
 
@@ -90,8 +93,8 @@ def identity_params(shift=(0, 0)):
 def test_oracle_forward_reduces_to_known_networks():
     rng = np.random.default_rng(4)
     h, w = 24, 40  # not multiples of 16: the frame is padded with zeros
-    color, albedo, normal, depth, _ = (a[0] for a in crops(rng, 1, h, w))
-    out = O.learn_forward(identity_params(), color, albedo, normal, depth)
+    color, albedo, normal, depth, _, deviation = (a[0] for a in crops(rng, 1, h, w))
+    out = O.learn_forward(identity_params(), color, albedo, normal, depth, deviation)
     np.testing.assert_allclose(out[..., :3], color[..., :3], rtol=1e-5, atol=1e-6)
     np.testing.assert_array_equal(out[..., 3], color[..., 3])
     # tap (0, 0) reads the pixel above and to the left
