@@ -11,10 +11,11 @@
 //                2x2 max pool are fused into the epilogue; nearest
 //                upsampling and concatenation are fused into the staging.
 //   k_conv_bf16  the same convolution with BF16 operands for the learned
-//                denoiser (v_mfma_f32_16x16x16_bf16; k_conv_bf16x32 with
-//                gfx950's v_mfma_f32_16x16x32_bf16 for narrow layers; FP32 sums): inputs
-//                rounded to BF16 once as staged, two rows per wave for
-//                layers up to 64 outputs.
+//                denoiser (v_mfma_f32_16x16x16_bf16, FP32 sums): inputs
+//                rounded to BF16 once as staged; for layers up to 64
+//                outputs k_conv_bf16x32 (gfx950's v_mfma_f32_16x16x32_bf16,
+//                two rows per wave, the next channel pair prefetched while
+//                weights and patch are read from LDS).
 //   k_bins / k_scale   autoexposure (process.wgsl:104-157)
 //   k_input / k_output PU transfer in and out of the network
 //                      (process.wgsl:33-102)
@@ -36,6 +37,7 @@ __device__ __forceinline__ short bf16_bits(float f) {
 __device__ __forceinline__ v4s bf16x4(float4 v) { return v4s{bf16_bits(v.x), bf16_bits(v.y), bf16_bits(v.z), bf16_bits(v.w)}; }
 
 constexpr int TW = 16, TH = 4, PW = TW + 2, PH = TH + 2;
+
 
 __device__ __forceinline__ uint2 bf16x4_bits(float4 v) {
     return make_uint2((uint32_t)(uint16_t)bf16_bits(v.x) | (uint32_t)(uint16_t)bf16_bits(v.y) << 16,
@@ -139,6 +141,76 @@ __global__ __launch_bounds__(256) void k_conv(ConvDev c) {
         }
 }
 
+// The epilogue of the BF16 convolutions: bias, ReLU, and the fused 2x2 max
+// pool when the layer pools. acc holds D[pixel 4*kk + v][channel i] of each
+// N tile for rows y0 + ROWS * wave + r.
+template <int NT, int ROWS>
+__device__ __forceinline__ void conv_store(const ConvDev& c, v4f (&acc)[ROWS][NT], float (&pooled)[2][NT][64][2],
+                                           int x0, int y0) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 15, kk = lane >> 4;
+    const int h = (int)c.h, w = (int)c.w_;
+    if (!c.pool) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int n = 16 * t + i;
+            const float bias = c.bias ? c.bias[n] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r) {
+                const int y = y0 + ROWS * wave + r;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int x = x0 + 4 * kk + v;
+                    const float o = acc[r][t][v] + bias;
+                    if (y < h && x < w) c.dst[((size_t)y * w + x) * c.n_pad + n] = c.relu ? fmaxf(o, 0.0f) : o;
+                }
+            }
+        }
+        return;
+    }
+    // 2x2 max pool after the ReLU: x pairs (v 0-1, 2-3) sit in one lane; the
+    // row pairs (2j, 2j + 1) sit in one wave (even ROWS) or in waves (0,1),
+    // (2,3) (ROWS = 1)
+    constexpr int PAIRS = ROWS == 1 ? 1 : ROWS / 2;
+    float m[PAIRS][NT][2];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const float bias = c.bias ? c.bias[16 * t + i] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < PAIRS; ++j)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                float v = fmaxf(acc[2 * j][t][2 * p] + bias, acc[2 * j][t][2 * p + 1] + bias);
+                if (ROWS > 1)
+                    v = fmaxf(v, fmaxf(acc[2 * j + 1][t][2 * p] + bias, acc[2 * j + 1][t][2 * p + 1] + bias));
+                m[j][t][p] = fmaxf(v, 0.0f);
+            }
+    }
+    int oy;
+    if (ROWS == 1) {
+        if (wave & 1)
+            for (int t = 0; t < NT; ++t)
+                for (int p = 0; p < 2; ++p) pooled[wave >> 1][t][lane][p] = m[0][t][p];
+        __syncthreads();
+        if (wave & 1) return;
+        for (int t = 0; t < NT; ++t)
+            for (int p = 0; p < 2; ++p) m[0][t][p] = fmaxf(m[0][t][p], pooled[wave >> 1][t][lane][p]);
+        oy = (y0 + wave) / 2;
+    } else {
+        oy = (y0 + ROWS * wave) / 2;
+    }
+    const int ho = h / 2, wo = w / 2;
+#pragma unroll
+    for (int j = 0; j < PAIRS; ++j)
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int ox = x0 / 2 + 2 * kk + p;
+                if (oy + j < ho && ox < wo) c.dst[((size_t)(oy + j) * wo + ox) * c.n_pad + 16 * t + i] = m[j][t][p];
+            }
+}
+
 // BF16 operands for the learned denoiser: a block computes 4 * ROWS rows x
 // 16 pixels, each wave ROWS rows, so with ROWS = 2 every weight fragment
 // read from memory feeds two matrix-core ops and the 2x2 pool sits inside a
@@ -201,78 +273,39 @@ __global__ __launch_bounds__(256) void k_conv_bf16(ConvDev c) {
             }
         }
     }
-    // D[pixel 4*kk + v][channel i] of each N tile, rows y0 + ROWS * wave + r
-    if (!c.pool) {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int n = 16 * t + i;
-            const float bias = c.bias ? c.bias[n] : 0.0f;
-#pragma unroll
-            for (int r = 0; r < ROWS; ++r) {
-                const int y = y0 + ROWS * wave + r;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int x = x0 + 4 * kk + v;
-                    const float o = acc[r][t][v] + bias;
-                    if (y < h && x < w) c.dst[((size_t)y * w + x) * c.n_pad + n] = c.relu ? fmaxf(o, 0.0f) : o;
-                }
-            }
-        }
-        return;
-    }
-    // 2x2 max pool after the ReLU: x pairs (v 0-1, 2-3) sit in one lane; the
-    // row pair sits in one wave (ROWS = 2) or in waves (0,1), (2,3)
-    float m[NT][2];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const float bias = c.bias ? c.bias[16 * t + i] : 0.0f;
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            float v = fmaxf(acc[0][t][2 * p] + bias, acc[0][t][2 * p + 1] + bias);
-            if (ROWS == 2) v = fmaxf(v, fmaxf(acc[ROWS - 1][t][2 * p] + bias, acc[ROWS - 1][t][2 * p + 1] + bias));
-            m[t][p] = fmaxf(v, 0.0f);
-        }
-    }
-    int oy;
-    if (ROWS == 1) {
-        if (wave & 1)
-            for (int t = 0; t < NT; ++t)
-                for (int p = 0; p < 2; ++p) pooled[wave >> 1][t][lane][p] = m[t][p];
-        __syncthreads();
-        if (wave & 1) return;
-        for (int t = 0; t < NT; ++t)
-            for (int p = 0; p < 2; ++p) m[t][p] = fmaxf(m[t][p], pooled[wave >> 1][t][lane][p]);
-        oy = (y0 + wave) / 2;
-    } else {
-        oy = (y0 + 2 * wave) / 2;
-    }
-    const int ho = h / 2, wo = w / 2;
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            const int ox = x0 / 2 + 2 * kk + p;
-            if (oy < ho && ox < wo) c.dst[((size_t)oy * wo + ox) * c.n_pad + 16 * t + i] = m[t][p];
-        }
+    conv_store<NT, ROWS>(c, acc, pooled, x0, y0);
 }
 
-// The same with gfx950's K = 32 matrix-core op
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ v8bf as_v8bf(uint4 u) { return __builtin_bit_cast(v8bf, u); }
+
+
+// The BF16 convolution on gfx950's K = 32 matrix-core op
 // (v_mfma_f32_16x16x32_bf16, FP32 sums): the K loop takes the input
 // channels 32 at a time (two 16-channel chunks of the packed weights; an
 // odd last chunk pairs with zeros), each lane holding 8 consecutive channels
 // of one pixel (A) and of one output's weights (B). A block computes 4 *
 // ROWS rows x 16 pixels, each wave ROWS rows, so with ROWS = 2 every weight
-// fragment read from memory feeds two matrix-core ops and the 2x2 pool sits
-// inside a wave. Used for layers of up to 4 N tiles (two rows per wave):
-// there it runs 9-16% faster than K = 16 at 4K; for the wide layers of one
-// row per wave, whose deep levels launch few blocks, K = 16 stays faster.
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-__device__ __forceinline__ v8bf as_v8bf(uint4 u) { return __builtin_bit_cast(v8bf, u); }
-
+// fragment feeds two matrix-core ops and the 2x2 pool sits inside a wave.
+// Software-pipelined: the input patch of the next channel pair is loaded
+// into registers while the current pair is computed, and the current
+// pair's weights are staged in LDS beside its patch, so the matrix cores
+// read both operands from LDS and no vector-memory wait (vmcnt, which
+// retires loads in order) ties a weight fragment to the prefetch in
+// flight. Per pair: a barrier, the registers rounded to BF16 into the patch
+// and the weights copied (from L2: every block reads the same ones) into
+// LDS, a barrier, the next pair's loads issued, then the nine taps. Used for
+// layers of up to 4 N tiles: there it runs 17-24% faster than the same
+// kernel reading weight fragments from memory per op, which was 9-16%
+// faster than K = 16; the wide layers (one row per wave, few blocks at the
+// deep levels) keep K = 16.
 template <int NT, int ROWS>
 __global__ __launch_bounds__(256) void k_conv_bf16x32(ConvDev c) {
     constexpr int TR = 4 * ROWS, PR = TR + 2;
-    __shared__ uint4 patch[PR * PW * 4];  // [row][col][32 channels: 4 x 8 BF16]
+    constexpr int ITEMS = PR * PW * 4, PER = (ITEMS + 255) / 256;  // patch items: 8 channels of one pixel
+    constexpr int WUNITS = 9 * NT * 16 * 4;                       // weight uint4s of a pair: [tap][n][4 x 8 ch]
+    __shared__ uint4 patch[ITEMS];
+    __shared__ uint4 wl[WUNITS];
     __shared__ float pooled[2][NT][64][2];  // ROWS = 1: odd rows' x-pooled values
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i = lane & 15, kk = lane >> 4;
@@ -290,107 +323,72 @@ __global__ __launch_bounds__(256) void k_conv_bf16x32(ConvDev c) {
     for (int r = 0; r < ROWS; ++r)
         for (int t = 0; t < NT; ++t) acc[r][t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     const uint32_t cin = c.c0 + c.c1, chunks = cin / 16u, pairs = (chunks + 1u) / 2u;
-    for (uint32_t g = 0; g < pairs; ++g) {
-        __syncthreads();
-        // each item is 8 channels of one pixel: two float4 loads, one uint4 store
-        for (int e = tid; e < PR * PW * 4; e += 256) {
-            int q = e & 3, p = e >> 2;
-            int py = p / PW, px = p - py * PW;
-            int y = y0 + py - 1, x = x0 + px - 1;
-            uint32_t cc = g * 32u + (uint32_t)q * 8u;
-            uint4 bits = make_uint4(0u, 0u, 0u, 0u);
-            if (y >= 0 && y < h && x >= 0 && x < w && cc < cin) {
+    float4 pre[PER][2];
+    auto fetch = [&](uint32_t g) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int e = tid + 256 * k;
+            float4 v0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v1 = v0;
+            const int q = e & 3, p = e >> 2;
+            const int py = p / PW, px = p - py * PW;
+            const int y = y0 + py - 1, x = x0 + px - 1;
+            const uint32_t cc = g * 32u + (uint32_t)q * 8u;
+            if (e < ITEMS && y >= 0 && y < h && x >= 0 && x < w && cc < cin) {
                 const float* at;
                 if (cc < c.c0) {
-                    int yy = c.upsample ? y >> 1 : y, xx = c.upsample ? x >> 1 : x;
+                    const int yy = c.upsample ? y >> 1 : y, xx = c.upsample ? x >> 1 : x;
                     at = c.src0 + ((size_t)yy * sw + xx) * c.c0 + cc;
                 } else {
                     at = c.src1 + ((size_t)y * w + x) * c.c1 + (cc - c.c0);
                 }
-                float4 v0 = *reinterpret_cast<const float4*>(at), v1 = *reinterpret_cast<const float4*>(at + 4);
-                uint2 lo = bf16x4_bits(v0), hi = bf16x4_bits(v1);
-                bits = make_uint4(lo.x, lo.y, hi.x, hi.y);
+                v0 = *reinterpret_cast<const float4*>(at);
+                v1 = *reinterpret_cast<const float4*>(at + 4);
             }
-            patch[e] = bits;
+            pre[k][0] = v0;
+            pre[k][1] = v1;
+        }
+    };
+    const uint16_t* wg = static_cast<const uint16_t*>(c.w);
+    fetch(0);
+    for (uint32_t g = 0; g < pairs; ++g) {
+        __syncthreads();  // the block is done with pair g - 1's patch and weights
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int e = tid + 256 * k;
+            if (e < ITEMS) {
+                const uint2 lo = bf16x4_bits(pre[k][0]), hi = bf16x4_bits(pre[k][1]);
+                patch[e] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+            }
+        }
+        // weights of chunks 2g and 2g + 1 (zero past the last chunk) as
+        // [tap][n][32 channels]
+        for (int u = tid; u < WUNITS; u += 256) {
+            const int q = u & 3, n = (u >> 2) % (NT * 16), tap = (u >> 2) / (NT * 16);
+            const uint32_t chunk = 2u * g + (uint32_t)(q >> 1);
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (chunk < chunks)
+                v = *reinterpret_cast<const uint4*>(wg + ((size_t)(chunk * 9u + (uint32_t)tap) * c.n_pad + (uint32_t)n) * 16u +
+                                                    (size_t)(q & 1) * 8u);
+            wl[u] = v;
         }
         __syncthreads();
-        // lane group kk takes channels 8 kk .. 8 kk + 7 of the pair: chunk
-        // 2 g + kk / 2, its half kk % 2; a chunk past the last is zero
-        const uint32_t chunk = 2u * g + (uint32_t)(kk >> 1);
-        const bool live = chunk < chunks;
-        const uint16_t* wb = static_cast<const uint16_t*>(c.w) + (size_t)(live ? chunk : 0u) * 9u * c.n_pad * 16u +
-                             (size_t)(kk & 1) * 8u;
+        if (g + 1u < pairs) fetch(g + 1u);
 #pragma unroll 3
         for (int tap = 0; tap < 9; ++tap) {
             const int ky = tap / 3, kx = tap - ky * 3;
             v8bf a[ROWS];
 #pragma unroll
             for (int r = 0; r < ROWS; ++r) a[r] = as_v8bf(patch[((ROWS * wave + r + ky) * PW + i + kx) * 4 + kk]);
-            const uint16_t* wt = wb + (size_t)tap * c.n_pad * 16u;
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-                uint4 braw = live ? *reinterpret_cast<const uint4*>(wt + (size_t)(16 * t + i) * 16u)
-                                  : make_uint4(0u, 0u, 0u, 0u);
-                const v8bf b = as_v8bf(braw);
+                const v8bf b = as_v8bf(wl[(tap * NT * 16 + 16 * t + i) * 4 + kk]);
 #pragma unroll
                 for (int r = 0; r < ROWS; ++r)
                     acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b, acc[r][t], 0, 0, 0);
             }
         }
     }
-    // D[pixel 4*kk + v][channel i] of each N tile, rows y0 + ROWS * wave + r
-    if (!c.pool) {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int n = 16 * t + i;
-            const float bias = c.bias ? c.bias[n] : 0.0f;
-#pragma unroll
-            for (int r = 0; r < ROWS; ++r) {
-                const int y = y0 + ROWS * wave + r;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int x = x0 + 4 * kk + v;
-                    const float o = acc[r][t][v] + bias;
-                    if (y < h && x < w) c.dst[((size_t)y * w + x) * c.n_pad + n] = c.relu ? fmaxf(o, 0.0f) : o;
-                }
-            }
-        }
-        return;
-    }
-    // 2x2 max pool after the ReLU: x pairs (v 0-1, 2-3) sit in one lane; the
-    // row pair sits in one wave (ROWS = 2) or in waves (0,1), (2,3)
-    float m[NT][2];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const float bias = c.bias ? c.bias[16 * t + i] : 0.0f;
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            float v = fmaxf(acc[0][t][2 * p] + bias, acc[0][t][2 * p + 1] + bias);
-            if (ROWS == 2) v = fmaxf(v, fmaxf(acc[ROWS - 1][t][2 * p] + bias, acc[ROWS - 1][t][2 * p + 1] + bias));
-            m[t][p] = fmaxf(v, 0.0f);
-        }
-    }
-    int oy;
-    if (ROWS == 1) {
-        if (wave & 1)
-            for (int t = 0; t < NT; ++t)
-                for (int p = 0; p < 2; ++p) pooled[wave >> 1][t][lane][p] = m[t][p];
-        __syncthreads();
-        if (wave & 1) return;
-        for (int t = 0; t < NT; ++t)
-            for (int p = 0; p < 2; ++p) m[t][p] = fmaxf(m[t][p], pooled[wave >> 1][t][lane][p]);
-        oy = (y0 + wave) / 2;
-    } else {
-        oy = (y0 + 2 * wave) / 2;
-    }
-    const int ho = h / 2, wo = w / 2;
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            const int ox = x0 / 2 + 2 * kk + p;
-            if (oy < ho && ox < wo) c.dst[((size_t)oy * wo + ox) * c.n_pad + 16 * t + i] = m[t][p];
-        }
+    conv_store<NT, ROWS>(c, acc, pooled, x0, y0);
 }
 
 static void launch_conv_fp32(hipStream_t s, const ConvDev& c, dim3 grid) {
@@ -405,14 +403,20 @@ static void launch_conv_fp32(hipStream_t s, const ConvDev& c, dim3 grid) {
     }
 }
 
+// Rows of 16 pixels per wave of the BF16 convolutions by their N tiles: two
+// for layers of up to 4 tiles (each weight fragment feeds two matrix-core
+// ops), one for wider layers, whose deep levels launch few blocks (four rows
+// for the narrow layers and two for the wide ones measured slower at 4K).
+constexpr uint32_t bf16_rows(uint32_t nt) { return nt <= 4u ? 2u : 1u; }
+
 static void launch_conv_bf16(hipStream_t s, const ConvDev& c, dim3 grid) {
     switch (c.n_pad / 16u) {
 #define PFXD_CONV(n) \
     case n:                                                                                      \
         if (n <= 4)                                                                              \
-            hipLaunchKernelGGL((k_conv_bf16x32<n, 2>), grid, dim3(256), 0, s, c);                \
+            hipLaunchKernelGGL((k_conv_bf16x32<n, bf16_rows(n)>), grid, dim3(256), 0, s, c);     \
         else                                                                                     \
-            hipLaunchKernelGGL((k_conv_bf16<n, 1>), grid, dim3(256), 0, s, c);                   \
+            hipLaunchKernelGGL((k_conv_bf16<n, bf16_rows(n)>), grid, dim3(256), 0, s, c);        \
         break;
         PFXD_CONV(1) PFXD_CONV(2) PFXD_CONV(3) PFXD_CONV(4) PFXD_CONV(5) PFXD_CONV(6)
         PFXD_CONV(7) PFXD_CONV(8) PFXD_CONV(9) PFXD_CONV(10) PFXD_CONV(11) PFXD_CONV(12)
@@ -425,7 +429,7 @@ static void launch_conv_bf16(hipStream_t s, const ConvDev& c, dim3 grid) {
 void launch_conv(hipStream_t s, const ConvDev& c) {
     const uint32_t z = c.batch ? c.batch : 1u;
     if (c.bf16) {
-        const uint32_t rows = (c.n_pad / 16u <= 4u ? 2u : 1u) * 4u;
+        const uint32_t rows = bf16_rows(c.n_pad / 16u) * 4u;
         launch_conv_bf16(s, c, dim3((c.w_ + TW - 1) / TW, (c.h + rows - 1) / rows, z));
     } else {
         launch_conv_fp32(s, c, dim3((c.w_ + TW - 1) / TW, (c.h + TH - 1) / TH, z));
