@@ -309,7 +309,10 @@ __global__ __launch_bounds__(256) void k_conv_bf16x32(ConvDev c) {
     __shared__ float pooled[2][NT][64][2];  // ROWS = 1: odd rows' x-pooled values
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i = lane & 15, kk = lane >> 4;
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TR;
+    // blockIdx.x: the 16-pixel column tile and, for layers wider than NT
+    // tiles, the group of NT output tiles this block computes
+    const int groups = (int)(c.n_pad / 16u) / NT, grp = (int)blockIdx.x % groups;
+    const int x0 = ((int)blockIdx.x / groups) * TW, y0 = blockIdx.y * TR;
     const int h = (int)c.h, w = (int)c.w_;
     const int sw = c.upsample ? w / 2 : w;
     {
@@ -317,7 +320,8 @@ __global__ __launch_bounds__(256) void k_conv_bf16x32(ConvDev c) {
         const size_t spx = c.upsample ? (size_t)(h / 2) * sw : px;
         c.src0 += z * spx * c.c0;
         if (c.src1) c.src1 += z * px * c.c1;
-        c.dst += z * (c.pool ? px / 4 : px) * c.n_pad;
+        c.dst += z * (c.pool ? px / 4 : px) * c.n_pad + grp * NT * 16;
+        if (c.bias) c.bias += grp * NT * 16;
     }
     v4f acc[ROWS][NT];
     for (int r = 0; r < ROWS; ++r)
@@ -367,8 +371,9 @@ __global__ __launch_bounds__(256) void k_conv_bf16x32(ConvDev c) {
             const uint32_t chunk = 2u * g + (uint32_t)(q >> 1);
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
             if (chunk < chunks)
-                v = *reinterpret_cast<const uint4*>(wg + ((size_t)(chunk * 9u + (uint32_t)tap) * c.n_pad + (uint32_t)n) * 16u +
-                                                    (size_t)(q & 1) * 8u);
+                v = *reinterpret_cast<const uint4*>(
+                    wg + ((size_t)(chunk * 9u + (uint32_t)tap) * c.n_pad + (uint32_t)(grp * NT * 16 + n)) * 16u +
+                    (size_t)(q & 1) * 8u);
             wl[u] = v;
         }
         __syncthreads();
@@ -403,24 +408,30 @@ static void launch_conv_fp32(hipStream_t s, const ConvDev& c, dim3 grid) {
     }
 }
 
-// Rows of 16 pixels per wave of the BF16 convolutions by their N tiles: two
-// for layers of up to 4 tiles (each weight fragment feeds two matrix-core
-// ops), one for wider layers, whose deep levels launch few blocks (four rows
-// for the narrow layers and two for the wide ones measured slower at 4K).
-constexpr uint32_t bf16_rows(uint32_t nt) { return nt <= 4u ? 2u : 1u; }
+// Output tiles per block of k_conv_bf16x32 for a layer of nt tiles of 16
+// channels, 0 where the layer takes k_conv_bf16: up to four tiles in one
+// block, six and eight split over two groups of blocks (so a block's
+// weights stay within 36 KiB of LDS); other widths keep the K = 16 kernel.
+constexpr uint32_t x32_tiles(uint32_t nt) { return nt <= 4u ? nt : (nt == 6u ? 3u : (nt == 8u ? 4u : 0u)); }
 
-static void launch_conv_bf16(hipStream_t s, const ConvDev& c, dim3 grid) {
-    switch (c.n_pad / 16u) {
+static void launch_conv_bf16(hipStream_t s, const ConvDev& c, uint32_t z) {
+    const uint32_t nt = c.n_pad / 16u, tiles_x = (c.w_ + TW - 1) / TW, bt = x32_tiles(nt);
+    if (bt) {  // two rows per wave, 8 rows per block
+        const dim3 grid(tiles_x * (nt / bt), (c.h + 7) / 8, z);
+        switch (bt) {
+            case 1: hipLaunchKernelGGL((k_conv_bf16x32<1, 2>), grid, dim3(256), 0, s, c); break;
+            case 2: hipLaunchKernelGGL((k_conv_bf16x32<2, 2>), grid, dim3(256), 0, s, c); break;
+            case 3: hipLaunchKernelGGL((k_conv_bf16x32<3, 2>), grid, dim3(256), 0, s, c); break;
+            default: hipLaunchKernelGGL((k_conv_bf16x32<4, 2>), grid, dim3(256), 0, s, c); break;
+        }
+        return;
+    }
+    const dim3 grid(tiles_x, (c.h + 3) / 4, z);  // one row per wave
+    switch (nt) {
 #define PFXD_CONV(n) \
-    case n:                                                                                      \
-        if (n <= 4)                                                                              \
-            hipLaunchKernelGGL((k_conv_bf16x32<n, bf16_rows(n)>), grid, dim3(256), 0, s, c);     \
-        else                                                                                     \
-            hipLaunchKernelGGL((k_conv_bf16<n, bf16_rows(n)>), grid, dim3(256), 0, s, c);        \
-        break;
-        PFXD_CONV(1) PFXD_CONV(2) PFXD_CONV(3) PFXD_CONV(4) PFXD_CONV(5) PFXD_CONV(6)
-        PFXD_CONV(7) PFXD_CONV(8) PFXD_CONV(9) PFXD_CONV(10) PFXD_CONV(11) PFXD_CONV(12)
-        PFXD_CONV(13) PFXD_CONV(14)
+    case n: hipLaunchKernelGGL((k_conv_bf16<n, 1>), grid, dim3(256), 0, s, c); break;
+        PFXD_CONV(5) PFXD_CONV(6) PFXD_CONV(7) PFXD_CONV(8) PFXD_CONV(9) PFXD_CONV(10) PFXD_CONV(11)
+        PFXD_CONV(12) PFXD_CONV(13) PFXD_CONV(14)
 #undef PFXD_CONV
         default: break;  // the host refuses other widths
     }
@@ -429,8 +440,7 @@ static void launch_conv_bf16(hipStream_t s, const ConvDev& c, dim3 grid) {
 void launch_conv(hipStream_t s, const ConvDev& c) {
     const uint32_t z = c.batch ? c.batch : 1u;
     if (c.bf16) {
-        const uint32_t rows = bf16_rows(c.n_pad / 16u) * 4u;
-        launch_conv_bf16(s, c, dim3((c.w_ + TW - 1) / TW, (c.h + rows - 1) / rows, z));
+        launch_conv_bf16(s, c, z);
     } else {
         launch_conv_fp32(s, c, dim3((c.w_ + TW - 1) / TW, (c.h + TH - 1) / TH, z));
     }
