@@ -300,8 +300,10 @@ pfxh_status pfxh_trace_sample_adaptive(pfxh_trace* trace, const pfxh_adaptive* a
 /* The standard error of each pixel's mean luminance ((r + g + b) / 3) over
  * its samples, sqrt(max(E[l^2] - E[l]^2, 0) / n), width*height floats,
  * caller-allocated; 0 where a pixel has no samples. No counterpart in the
- * reference: the learned denoiser's noise channel. After pfxh_trace_restore
- * it covers the samples taken since (PFXCKPT1 holds no moments). */
+ * reference: the learned denoiser's noise channel. PFXCKPT1 holds no
+ * moments: after pfxh_trace_restore the restored samples count at their
+ * pixel's mean with no spread, so it reads 0 until new samples are taken
+ * and understates the spread while the restored samples dominate. */
 pfxh_status pfxh_trace_readback_deviation(pfxh_trace* trace, float* out, char* err, size_t err_len);
 /* Trace::readback (gpu.rs:3236-3246): three RGBA32F images, row-major,
  * width*height*4 floats each, caller-allocated. Any pointer may be NULL. */

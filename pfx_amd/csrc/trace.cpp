@@ -683,9 +683,17 @@ pfxh_status pfxh_trace_restore(pfxh_trace* t, const char* key, const uint8_t* by
         if (!pixels) return;
         HIP_TRY(hipMemcpy(t->A.sum, c.accum, pixels * 48, hipMemcpyHostToDevice));
         if (t->A.mode == PFXH_ACCUM_COMPENSATED) HIP_TRY(hipMemset(t->A.comp, 0, pixels * 48));
-        // PFXCKPT1 holds no luminance moments: the deviation covers the
-        // samples taken after the restore (it reads 0 until then)
-        HIP_TRY(hipMemset(t->A.m2, 0, pixels * 4));
+        // PFXCKPT1 holds no luminance moments: the restored samples enter the
+        // moment sum at their pixel's mean luminance, with no spread, so the
+        // deviation reads 0 until new samples add theirs
+        std::vector<float> m2(pixels);
+        for (size_t p = 0; p < pixels; ++p) {
+            float s[4];
+            memcpy(s, c.accum + p * 48, 16);
+            const float n = fabsf(s[3]), l = (s[0] + s[1] + s[2]) / 3.0f;
+            m2[p] = n > 0.0f ? l * l / n : 0.0f;
+        }
+        HIP_TRY(hipMemcpy(t->A.m2, m2.data(), pixels * 4, hipMemcpyHostToDevice));
         if (t->A.mode == 2u) {
             std::vector<double> wide(pixels * 12);
             for (size_t i = 0; i < wide.size(); ++i) {

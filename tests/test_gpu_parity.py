@@ -237,6 +237,7 @@ def test_checkpoint_resume_continues_the_render(accumulate):
         first = t.readback()
         t.sample(4, 3)
         full = t.readback()
+        dev_full = t.readback_deviation()
     from pfx_amd.checkpoint import Saved
 
     s = Saved.read_from("still", data)
@@ -246,8 +247,17 @@ def test_checkpoint_resume_continues_the_render(accumulate):
         t.restore("still", data)
         assert t.samples() == 4
         assert np.array_equal(t.readback().color, first.color) or accumulate != ACCUM_PLAIN
+        # PFXCKPT1 holds no moments: the restored samples count at their mean
+        # with no spread, so the noise channel reads 0 (to rounding) ...
+        lum = first.color[..., :3].mean(axis=-1)
+        assert np.all(t.readback_deviation() <= 1e-3 * lum + 1e-7)
         t.sample(4, 3)
         resumed = t.readback()
+        # ... and after more samples it cannot exceed the uninterrupted one:
+        # 4 mean^2 <= the sum of the 4 restored samples' squares
+        dev_resumed = t.readback_deviation()
+        assert np.isfinite(dev_resumed).all() and (dev_resumed >= 0).all() and dev_resumed.max() > 0
+        assert np.all(dev_resumed <= dev_full * (1 + 1e-4) + 1e-6)
     if accumulate == ACCUM_PLAIN:
         assert np.array_equal(resumed.color, full.color)
     else:
