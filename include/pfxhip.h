@@ -709,6 +709,16 @@ pfxh_status pfxh_learner_grads(pfxh_learner* learner, float* out, uint64_t count
 #define PFXH_LEARN_LOSS_L1_LOG 0u
 #define PFXH_LEARN_LOSS_REL_L2 1u
 pfxh_status pfxh_learner_set_loss(pfxh_learner* learner, uint32_t mode);
+/* Inference passes (self-ensemble over mirror images): 1 (the default), 2
+ * (also the image mirrored in x) or 4 (also mirrored in y, and in both);
+ * the frames run through the network as one batch and the denoised image is
+ * the mean of their outputs, each mirrored back, in linear radiance. The
+ * network was trained under these symmetries, so the mean keeps its bias
+ * and averages out part of its error; the cost is the passes' convolutions.
+ * Applies to pfxh_learned_denoise and the trace entry points; other values
+ * give PFXH_INVALID. No counterpart in the reference (it has no trained
+ * denoiser). */
+pfxh_status pfxh_learner_set_passes(pfxh_learner* learner, uint32_t passes);
 /* Clear Adam's moments and step count. */
 void pfxh_learner_reset_optimizer(pfxh_learner* learner);
 /* Make room on the GPU for `capacity` training crops of side x side pixels

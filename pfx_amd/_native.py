@@ -254,6 +254,7 @@ SIGNATURES = {
     "pfxh_trace_set_sun_radius": (C.c_int, [_P, C.c_float, _S, _Z]),
     "pfxh_set_device_limits": (None, [_P]),
     "pfxh_learner_set_loss": (C.c_int, [_P, _U]),
+    "pfxh_learner_set_passes": (C.c_int, [_P, _U]),
     "pfxh_stage_build": (C.c_int, [_P, C.POINTER(_P), _S, _Z]),
     "pfxh_layout_sobol": (C.c_int, [_U, _P, C.POINTER(_U)]),
     "pfxh_sobol_animated_seed": (_U, [_U, _U]),

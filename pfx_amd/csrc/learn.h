@@ -34,12 +34,16 @@ void relu_bwd(hipStream_t s, const float* dy, const float* y, size_t n, float* o
 void scatter(hipStream_t s, const float* dx, uint32_t b, uint32_t h, uint32_t w, uint32_t c0, uint32_t c1, bool up,
              float* d0, float* d1);
 void weight_grads(hipStream_t s, const DwArgs& a, const DwPlan& p, bool bf16, float* dw, float* db);
+// the network input of `mirrors` frames (1, 2 or 4): frame b the image
+// mirrored by b (bit 0 in x, bit 1 in y)
 void learn_in(hipStream_t s, const float4* color, const float4* albedo, const float4* normal, const float* depth,
               uint32_t depth_stride, const float* deviation, uint32_t w, uint32_t h, uint32_t w16, uint32_t h16,
-              float* x);
+              uint32_t mirrors, float* x);
 uint32_t loss(hipStream_t s, const float* y, const float4* target, size_t n, uint32_t mode, float* dy, float* sums);
 void loss_total(hipStream_t s, const float* sums, uint32_t nb, size_t n, float* out);
-void learn_out(hipStream_t s, const float* y, uint32_t w16, const float4* color, uint32_t w, uint32_t h, float4* out);
+// the mean radiance over the `mirrors` frames, each mirrored back
+void learn_out(hipStream_t s, const float* y, uint32_t w16, uint32_t h16, uint32_t mirrors, const float4* color,
+               uint32_t w, uint32_t h, float4* out);
 void gather(hipStream_t s, const float* inputs, const float4* targets, size_t crop_px, uint32_t side,
             const uint32_t* picks, uint32_t batch, float* x, float4* t);
 void tile_residual(hipStream_t s, const float4* noisy, const float4* den, uint32_t w, uint32_t h, uint32_t tile,

@@ -470,15 +470,19 @@ void weight_grads(hipStream_t s, const DwArgs& a, const DwPlan& p, bool bf16, fl
 // log(1 + the standard error of the pixel's mean luminance);
 // the network's frame is the image padded to multiples of 16 with zeros
 __device__ __forceinline__ float finite_or0(float v) { return v == v && fabsf(v) < 1e30f ? v : 0.0f; }
+// Frame b of a batch of `mirrors` holds the image mirrored by b (bit 0 in
+// x, bit 1 in y) within its w x h pixels; the padding stays zero.
 __global__ void k_learn_in(const float4* color, const float4* albedo, const float4* normal, const float* depth,
                            uint32_t depth_stride, const float* deviation, uint32_t w, uint32_t h, uint32_t w16,
-                           uint32_t h16, float* x) {
-    size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (q >= (size_t)w16 * h16) return;
+                           uint32_t h16, uint32_t mirrors, float* x) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x, frame = (size_t)w16 * h16;
+    if (e >= frame * mirrors) return;
+    const uint32_t b = (uint32_t)(e / frame);
+    const size_t q = e % frame;
     uint32_t px = q % w16, py = (uint32_t)(q / w16);
     float v[16] = {};
     if (px < w && py < h) {
-        size_t p = (size_t)py * w + px;
+        size_t p = (size_t)(b & 2u ? h - 1 - py : py) * w + (b & 1u ? w - 1 - px : px);
         float4 c = color[p], a = albedo[p], m = normal[p];
         float d = depth ? finite_or0(depth[p * depth_stride]) : 0.0f;
         v[0] = log1pf(fmaxf(finite_or0(c.x), 0.0f));
@@ -493,7 +497,7 @@ __global__ void k_learn_in(const float4* color, const float4* albedo, const floa
         v[9] = log1pf(fmaxf(d, 0.0f)) * 0.125f;
         v[10] = deviation ? log1pf(fmaxf(finite_or0(deviation[p]), 0.0f)) : 0.0f;
     }
-    float4* o = reinterpret_cast<float4*>(x + q * 16);
+    float4* o = reinterpret_cast<float4*>(x + e * 16);
     for (int k = 0; k < 4; ++k) o[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
 }
 // The loss between the network's log radiance (channels 0..2 of y, 16 per
@@ -536,13 +540,24 @@ __global__ void k_loss(const float* y, const float4* target, size_t n, float sca
     }
     if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
 }
-__global__ void k_learn_out(const float* y, uint32_t w16, const float4* color, uint32_t w, uint32_t h, float4* out) {
+// Radiance of each pixel: the mean, over the `mirrors` frames of k_learn_in,
+// of the network's output at the pixel's mirrored position, decoded
+// (expm1) before averaging; frames added in order, so one frame is exact.
+__global__ void k_learn_out(const float* y, uint32_t w16, uint32_t h16, uint32_t mirrors, const float4* color,
+                            uint32_t w, uint32_t h, float4* out) {
     size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (p >= (size_t)w * h) return;
     uint32_t px = p % w, py = (uint32_t)(p / w);
-    const float* v = y + ((size_t)py * w16 + px) * 16;
-    out[p] = make_float4(expm1f(fmaxf(v[0], 0.0f)), expm1f(fmaxf(v[1], 0.0f)), expm1f(fmaxf(v[2], 0.0f)),
-                         color ? color[p].w : 1.0f);
+    float r = 0.0f, g = 0.0f, bl = 0.0f;
+    for (uint32_t b = 0; b < mirrors; ++b) {
+        const size_t mx = b & 1u ? w - 1 - px : px, my = b & 2u ? h - 1 - py : py;
+        const float* v = y + ((size_t)b * h16 * w16 + my * w16 + mx) * 16;
+        r += expm1f(fmaxf(v[0], 0.0f));
+        g += expm1f(fmaxf(v[1], 0.0f));
+        bl += expm1f(fmaxf(v[2], 0.0f));
+    }
+    const float inv = 1.0f / (float)mirrors;
+    out[p] = make_float4(r * inv, g * inv, bl * inv, color ? color[p].w : 1.0f);
 }
 // batch item b = crop picks[b] >> 3 under the square's symmetry picks[b] & 7
 // (bit 0 mirrors x, bit 1 mirrors y, bit 2 transposes)
@@ -567,17 +582,19 @@ __global__ void k_gather(const float* inputs, const float4* targets, size_t crop
     t[e] = targets[src];
 }
 void learn_in(hipStream_t s, const float4* c, const float4* a, const float4* nm, const float* d, uint32_t ds,
-              const float* dev, uint32_t w, uint32_t h, uint32_t w16, uint32_t h16, float* x) {
-    hipLaunchKernelGGL(k_learn_in, dim3(blocks((size_t)w16 * h16, 256)), dim3(256), 0, s, c, a, nm, d, ds, dev, w,
-                       h, w16, h16, x);
+              const float* dev, uint32_t w, uint32_t h, uint32_t w16, uint32_t h16, uint32_t mirrors, float* x) {
+    hipLaunchKernelGGL(k_learn_in, dim3(blocks((size_t)w16 * h16 * mirrors, 256)), dim3(256), 0, s, c, a, nm, d, ds,
+                       dev, w, h, w16, h16, mirrors, x);
 }
 uint32_t loss(hipStream_t s, const float* y, const float4* t, size_t n, uint32_t mode, float* dy, float* sums) {
     uint32_t nb = blocks(n, 256);
     hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, y, t, n, 1.0f / (3.0f * (float)n), mode, dy, sums);
     return nb;
 }
-void learn_out(hipStream_t s, const float* y, uint32_t w16, const float4* color, uint32_t w, uint32_t h, float4* out) {
-    hipLaunchKernelGGL(k_learn_out, dim3(blocks((size_t)w * h, 256)), dim3(256), 0, s, y, w16, color, w, h, out);
+void learn_out(hipStream_t s, const float* y, uint32_t w16, uint32_t h16, uint32_t mirrors, const float4* color,
+               uint32_t w, uint32_t h, float4* out) {
+    hipLaunchKernelGGL(k_learn_out, dim3(blocks((size_t)w * h, 256)), dim3(256), 0, s, y, w16, h16, mirrors, color, w,
+                       h, out);
 }
 void gather(hipStream_t s, const float* inputs, const float4* targets, size_t crop_px, uint32_t side,
             const uint32_t* picks, uint32_t batch, float* x, float4* t) {

@@ -132,6 +132,7 @@ struct pfxh_learner {
     std::vector<float*> wf, wt;  // packed for k_conv: forward, transposed
     int adam_t = 0;
     uint32_t loss_mode = 0;  // PFXH_LEARN_LOSS_*
+    uint32_t passes = 1;     // inference: mirrored copies of the frame averaged (1, 2 or 4)
     // training set: encoded inputs [crop][side][side][16], targets [crop][side][side]
     uint32_t side = 0, capacity = 0, crops = 0;
     float* inputs = nullptr;
@@ -382,15 +383,24 @@ struct pfxh_learner {
         }
     }
 
+    // Size the inference frame for a width x height image and the current
+    // pass count; pointers into it stay valid until the next call.
+    Frame& prepare_inference(uint32_t width, uint32_t height) {
+        frame(infer_frame, passes, pad16(height), pad16(width), false);
+        return infer_frame;
+    }
     void infer(uint32_t width, uint32_t height, const float4* color, const float4* albedo, const float4* normal,
                const float* depth, uint32_t depth_stride, const float* deviation, float4* out) {
         uint32_t w16 = pad16(width), h16 = pad16(height);
-        frame(infer_frame, 1, h16, w16, false);
+        // the caller sized the frame (prepare_inference) before taking
+        // pointers into it: sizing it here could free them
         Frame& f = infer_frame;
-        pfxl::learn_in(stream, color, albedo, normal, depth, depth_stride, deviation, width, height, w16, h16,
+        if (f.batch != passes || f.h != h16 || f.w != w16 || f.train)
+            throw Failure{PFXH_SETUP, "inference frame not prepared for this image"};
+        pfxl::learn_in(stream, color, albedo, normal, depth, depth_stride, deviation, width, height, w16, h16, passes,
                        f.t[0]);
         forward(f);
-        pfxl::learn_out(stream, f.t[TENSORS - 1], w16, color, width, height, out);
+        pfxl::learn_out(stream, f.t[TENSORS - 1], w16, h16, passes, color, width, height, out);
         hip_check(hipGetLastError(), "learned denoise launch");
     }
 
@@ -416,8 +426,7 @@ pfxh_status pfxh_learned_denoise_device(pfxh_learner* l, uint32_t width, uint32_
         if (width == 0 || height == 0) throw Failure{PFXH_INVALID, "image dimensions must be nonzero"};
         if (after) hip_check(hipStreamSynchronize(after), "learned denoise input");
         uint32_t w16 = pad16(width), h16 = pad16(height);
-        l->frame(l->infer_frame, 1, h16, w16, false);
-        float4* o = l->infer_frame.images + 3 * (size_t)w16 * h16;
+        float4* o = l->prepare_inference(width, height).images + 3 * (size_t)w16 * h16;
         l->infer(width, height, (const float4*)color, (const float4*)albedo, (const float4*)normal, depth,
                  depth_stride, deviation, o);
         if (device_out) *device_out = o;
@@ -501,6 +510,12 @@ pfxh_status pfxh_learner_set_loss(pfxh_learner* l, uint32_t mode) {
     return PFXH_OK;
 }
 
+pfxh_status pfxh_learner_set_passes(pfxh_learner* l, uint32_t passes) {
+    if (!l || !(passes == 1u || passes == 2u || passes == 4u)) return PFXH_INVALID;
+    l->passes = passes;
+    return PFXH_OK;
+}
+
 void pfxh_learner_reset_optimizer(pfxh_learner* l) {
     if (!l) return;
     l->adam_t = 0;
@@ -553,7 +568,7 @@ pfxh_status pfxh_learner_add(pfxh_learner* l, const float* color, const float* a
         if (dev) hip_check(hipMemcpyAsync(dev, deviation, px * 4, hipMemcpyHostToDevice, l->stream), "crop upload");
         hip_check(hipMemcpyAsync(l->targets + px * l->crops, target, bytes, hipMemcpyHostToDevice, l->stream),
                   "crop upload");
-        pfxl::learn_in(l->stream, st, st + px, st + 2 * px, dep, 1, dev, l->side, l->side, l->side, l->side,
+        pfxl::learn_in(l->stream, st, st + px, st + 2 * px, dep, 1, dev, l->side, l->side, l->side, l->side, 1u,
                        l->inputs + px * 16 * l->crops);
         hip_check(hipGetLastError(), "crop encode");
         hip_check(hipStreamSynchronize(l->stream), "crop upload");
@@ -618,8 +633,7 @@ pfxh_status pfxh_learned_denoise(pfxh_learner* l, uint32_t width, uint32_t heigh
     return guard(err, err_len, [&] {
         if (width == 0 || height == 0) throw Failure{PFXH_INVALID, "image dimensions must be nonzero"};
         uint32_t w16 = pad16(width), h16 = pad16(height);
-        l->frame(l->infer_frame, 1, h16, w16, false);
-        pfxh_learner::Frame& f = l->infer_frame;
+        pfxh_learner::Frame& f = l->prepare_inference(width, height);
         size_t px = (size_t)width * height, bytes = px * 16, cap = (size_t)w16 * h16;
         float4* im = f.images;
         hip_check(hipMemcpyAsync(im, color, bytes, hipMemcpyHostToDevice, l->stream), "learned denoise upload");

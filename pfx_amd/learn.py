@@ -130,6 +130,12 @@ class Learner:
         if N.lib().pfxh_learner_set_loss(self._h, code) != N.OK:
             raise ValueError(mode)
 
+    def set_passes(self, passes: int) -> None:
+        """Inference passes over mirror images of the frame, averaged: 1, 2
+        (also mirrored in x) or 4 (also in y, and in both)."""
+        if N.lib().pfxh_learner_set_passes(self._h, int(passes)) != N.OK:
+            raise ValueError(f"passes must be 1, 2 or 4, not {passes}")
+
     def reserve(self, side: int, capacity: int) -> None:
         err = N.err_buffer()
         self._check(N.lib().pfxh_learner_reserve(self._h, side, capacity, err, len(err)), err)

@@ -119,6 +119,39 @@ def test_inference_matches_fp64_oracle(fp32, tol):
     np.testing.assert_array_equal(out[..., 3], color[..., 3])
 
 
+def test_mirror_passes_average_the_mirrored_denoises():
+    """pfxh_learner_set_passes: 2 and 4 passes give the mean, in linear
+    radiance, of one-pass denoises of the frame mirrored in x, y and both,
+    each mirrored back (the batch runs every frame with a one-pass frame's
+    arithmetic; the sum is taken in frame order, then scaled). The frame is
+    not a multiple of 16, so the padding lies outside the mirrored image."""
+    print("learn mirror passes", flush=True)
+    rng = np.random.default_rng(29)
+    h, w = 40, 56
+    color, albedo, normal, depth, _, deviation = (a[0] for a in crops(rng, 1, h, w))
+
+    def mirror(a, b):
+        a = a[:, ::-1] if b & 1 else a
+        return np.ascontiguousarray(a[::-1] if b & 2 else a)
+
+    with Lm.Learner(params=params_for_test(29)) as L:
+        singles = [mirror(L.denoise(*(mirror(x, b) for x in (color, albedo, normal, depth, deviation))), b)
+                   for b in range(4)]
+        for passes in (2, 4):
+            L.set_passes(passes)
+            got = L.denoise(color, albedo, normal, depth, deviation)
+            want = singles[0][..., :3].copy()
+            for k in range(1, passes):
+                want += singles[k][..., :3]
+            want *= np.float32(1.0 / passes)
+            np.testing.assert_allclose(got[..., :3], want, rtol=1e-6, atol=1e-7)
+            np.testing.assert_array_equal(got[..., 3], color[..., 3])
+        L.set_passes(1)
+        np.testing.assert_array_equal(L.denoise(color, albedo, normal, depth, deviation), singles[0])
+        with pytest.raises(ValueError, match="passes must be 1, 2 or 4"):
+            L.set_passes(3)
+
+
 def splitmix_picks(seed, count, crops_n):
     mask = (1 << 64) - 1
     x, out = seed, []
