@@ -10,12 +10,12 @@
 //                1-pixel halo in LDS, and the nine taps. Bias, ReLU and the
 //                2x2 max pool are fused into the epilogue; nearest
 //                upsampling and concatenation are fused into the staging.
-//   k_conv_bf16  the same convolution with BF16 operands for the learned
-//                denoiser (v_mfma_f32_16x16x16_bf16, FP32 sums): inputs
-//                rounded to BF16 once as staged; for layers up to 64
-//                outputs k_conv_bf16x32 (gfx950's v_mfma_f32_16x16x32_bf16,
-//                two rows per wave, the next channel pair prefetched while
-//                weights and patch are read from LDS).
+//   k_conv_bf16x32  the same convolution with BF16 operands for the
+//                learned denoiser on gfx950's v_mfma_f32_16x16x32_bf16
+//                (FP32 sums): inputs rounded to BF16 once as staged, two rows
+//                per wave, the next channel pair prefetched while weights and
+//                patch are read from LDS, wide layers split over groups of
+//                up to four output tiles.
 //   k_bins / k_scale   autoexposure (process.wgsl:104-157)
 //   k_input / k_output PU transfer in and out of the network
 //                      (process.wgsl:33-102)
@@ -26,7 +26,6 @@
 namespace pfxd {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
-typedef short v4s __attribute__((ext_vector_type(4)));
 
 // float -> bfloat16 bits, round to nearest even (finite inputs)
 __device__ __forceinline__ short bf16_bits(float f) {
@@ -34,17 +33,12 @@ __device__ __forceinline__ short bf16_bits(float f) {
     u += 0x7fffu + ((u >> 16) & 1u);
     return (short)(u >> 16);
 }
-__device__ __forceinline__ v4s bf16x4(float4 v) { return v4s{bf16_bits(v.x), bf16_bits(v.y), bf16_bits(v.z), bf16_bits(v.w)}; }
 
 constexpr int TW = 16, TH = 4, PW = TW + 2, PH = TH + 2;
-
 
 __device__ __forceinline__ uint2 bf16x4_bits(float4 v) {
     return make_uint2((uint32_t)(uint16_t)bf16_bits(v.x) | (uint32_t)(uint16_t)bf16_bits(v.y) << 16,
                       (uint32_t)(uint16_t)bf16_bits(v.z) | (uint32_t)(uint16_t)bf16_bits(v.w) << 16);
-}
-__device__ __forceinline__ v4s as_v4s(uint2 u) {
-    return v4s{(short)(u.x & 0xffffu), (short)(u.x >> 16), (short)(u.y & 0xffffu), (short)(u.y >> 16)};
 }
 
 template <int NT>
@@ -146,13 +140,14 @@ __global__ __launch_bounds__(256) void k_conv(ConvDev c) {
 // N tile for rows y0 + ROWS * wave + r.
 template <int NT, int ROWS>
 __device__ __forceinline__ void conv_store(const ConvDev& c, v4f (&acc)[ROWS][NT], float (&pooled)[2][NT][64][2],
-                                           int x0, int y0) {
+                                           int x0, int y0, int tiles = NT) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i = lane & 15, kk = lane >> 4;
     const int h = (int)c.h, w = (int)c.w_;
     if (!c.pool) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
+            if (t >= tiles) break;
             const int n = 16 * t + i;
             const float bias = c.bias ? c.bias[n] : 0.0f;
 #pragma unroll
@@ -175,7 +170,7 @@ __device__ __forceinline__ void conv_store(const ConvDev& c, v4f (&acc)[ROWS][NT
     float m[PAIRS][NT][2];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const float bias = c.bias ? c.bias[16 * t + i] : 0.0f;
+        const float bias = c.bias && t < tiles ? c.bias[16 * t + i] : 0.0f;
 #pragma unroll
         for (int j = 0; j < PAIRS; ++j)
 #pragma unroll
@@ -207,74 +202,11 @@ __device__ __forceinline__ void conv_store(const ConvDev& c, v4f (&acc)[ROWS][NT
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
                 const int ox = x0 / 2 + 2 * kk + p;
-                if (oy + j < ho && ox < wo) c.dst[((size_t)(oy + j) * wo + ox) * c.n_pad + 16 * t + i] = m[j][t][p];
+                if (t < tiles && oy + j < ho && ox < wo)
+                    c.dst[((size_t)(oy + j) * wo + ox) * c.n_pad + 16 * t + i] = m[j][t][p];
             }
 }
 
-// BF16 operands for the learned denoiser: a block computes 4 * ROWS rows x
-// 16 pixels, each wave ROWS rows, so with ROWS = 2 every weight fragment
-// read from memory feeds two matrix-core ops and the 2x2 pool sits inside a
-// wave; wide layers (more than 4 N tiles) keep one row per wave, where the
-// doubled accumulators would cost more occupancy than the reuse gains.
-template <int NT, int ROWS>
-__global__ __launch_bounds__(256) void k_conv_bf16(ConvDev c) {
-    constexpr int TR = 4 * ROWS, PR = TR + 2;
-    __shared__ uint2 patch[PR * PW * 4];  // [row][col][4 channels x 4]
-    __shared__ float pooled[2][NT][64][2];  // ROWS = 1: odd rows' x-pooled values
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int i = lane & 15, kk = lane >> 4;
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TR;
-    const int h = (int)c.h, w = (int)c.w_;
-    const int sw = c.upsample ? w / 2 : w;
-    {
-        const size_t z = blockIdx.z, px = (size_t)h * w;
-        const size_t spx = c.upsample ? (size_t)(h / 2) * sw : px;
-        c.src0 += z * spx * c.c0;
-        if (c.src1) c.src1 += z * px * c.c1;
-        c.dst += z * (c.pool ? px / 4 : px) * c.n_pad;
-    }
-    v4f acc[ROWS][NT];
-    for (int r = 0; r < ROWS; ++r)
-        for (int t = 0; t < NT; ++t) acc[r][t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    const uint32_t chunks = (c.c0 + c.c1) / 16u;
-    for (uint32_t ch = 0; ch < chunks; ++ch) {
-        __syncthreads();
-        for (int e = tid; e < PR * PW * 4; e += 256) {
-            int q = e & 3, p = e >> 2;
-            int py = p / PW, px = p - py * PW;
-            int y = y0 + py - 1, x = x0 + px - 1;
-            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (y >= 0 && y < h && x >= 0 && x < w) {
-                uint32_t cc = ch * 16u + (uint32_t)q * 4u;
-                if (cc < c.c0) {
-                    int yy = c.upsample ? y >> 1 : y, xx = c.upsample ? x >> 1 : x;
-                    v = *reinterpret_cast<const float4*>(c.src0 + ((size_t)yy * sw + xx) * c.c0 + cc);
-                } else {
-                    v = *reinterpret_cast<const float4*>(c.src1 + ((size_t)y * w + x) * c.c1 + (cc - c.c0));
-                }
-            }
-            patch[e] = bf16x4_bits(v);
-        }
-        __syncthreads();
-        const uint16_t* wb = static_cast<const uint16_t*>(c.w) + (size_t)ch * 9u * c.n_pad * 16u;
-#pragma unroll 3
-        for (int tap = 0; tap < 9; ++tap) {
-            const int ky = tap / 3, kx = tap - ky * 3;
-            v4s a[ROWS];
-#pragma unroll
-            for (int r = 0; r < ROWS; ++r) a[r] = as_v4s(patch[((ROWS * wave + r + ky) * PW + i + kx) * 4 + kk]);
-            const uint16_t* wt = wb + (size_t)tap * c.n_pad * 16u;
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const v4s b = as_v4s(*reinterpret_cast<const uint2*>(wt + (size_t)(16 * t + i) * 16u + 4 * kk));
-#pragma unroll
-                for (int r = 0; r < ROWS; ++r)
-                    acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[r], b, acc[r][t], 0, 0, 0);
-            }
-        }
-    }
-    conv_store<NT, ROWS>(c, acc, pooled, x0, y0);
-}
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 __device__ __forceinline__ v8bf as_v8bf(uint4 u) { return __builtin_bit_cast(v8bf, u); }
@@ -310,8 +242,10 @@ __global__ __launch_bounds__(256) void k_conv_bf16x32(ConvDev c) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i = lane & 15, kk = lane >> 4;
     // blockIdx.x: the 16-pixel column tile and, for layers wider than NT
-    // tiles, the group of NT output tiles this block computes
-    const int groups = (int)(c.n_pad / 16u) / NT, grp = (int)blockIdx.x % groups;
+    // tiles, the group of up to NT output tiles this block computes (the
+    // last group may hold fewer: `tiles`)
+    const int nt = (int)(c.n_pad / 16u), groups = (nt + NT - 1) / NT, grp = (int)blockIdx.x % groups;
+    const int tiles = min(NT, nt - grp * NT);
     const int x0 = ((int)blockIdx.x / groups) * TW, y0 = blockIdx.y * TR;
     const int h = (int)c.h, w = (int)c.w_;
     const int sw = c.upsample ? w / 2 : w;
@@ -370,7 +304,7 @@ __global__ __launch_bounds__(256) void k_conv_bf16x32(ConvDev c) {
             const int q = u & 3, n = (u >> 2) % (NT * 16), tap = (u >> 2) / (NT * 16);
             const uint32_t chunk = 2u * g + (uint32_t)(q >> 1);
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (chunk < chunks)
+            if (chunk < chunks && n < tiles * 16)
                 v = *reinterpret_cast<const uint4*>(
                     wg + ((size_t)(chunk * 9u + (uint32_t)tap) * c.n_pad + (uint32_t)(grp * NT * 16 + n)) * 16u +
                     (size_t)(q & 1) * 8u);
@@ -386,6 +320,7 @@ __global__ __launch_bounds__(256) void k_conv_bf16x32(ConvDev c) {
             for (int r = 0; r < ROWS; ++r) a[r] = as_v8bf(patch[((ROWS * wave + r + ky) * PW + i + kx) * 4 + kk]);
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
+                if (t >= tiles) break;  // block-uniform
                 const v8bf b = as_v8bf(wl[(tap * NT * 16 + 16 * t + i) * 4 + kk]);
 #pragma unroll
                 for (int r = 0; r < ROWS; ++r)
@@ -393,7 +328,7 @@ __global__ __launch_bounds__(256) void k_conv_bf16x32(ConvDev c) {
             }
         }
     }
-    conv_store<NT, ROWS>(c, acc, pooled, x0, y0);
+    conv_store<NT, ROWS>(c, acc, pooled, x0, y0, tiles);
 }
 
 static void launch_conv_fp32(hipStream_t s, const ConvDev& c, dim3 grid) {
@@ -409,31 +344,20 @@ static void launch_conv_fp32(hipStream_t s, const ConvDev& c, dim3 grid) {
 }
 
 // Output tiles per block of k_conv_bf16x32 for a layer of nt tiles of 16
-// channels, 0 where the layer takes k_conv_bf16: up to four tiles in one
-// block, six and eight split over two groups of blocks (so a block's
-// weights stay within 36 KiB of LDS); other widths keep the K = 16 kernel.
-constexpr uint32_t x32_tiles(uint32_t nt) { return nt <= 4u ? nt : (nt == 6u ? 3u : (nt == 8u ? 4u : 0u)); }
+// channels: all of them up to four, else as even a split into groups of at
+// most four as the count allows (6 = 3 + 3, 7 = 4 + 3, 14 = 4 + 4 + 4 + 2),
+// so a block's weights stay within 36 KiB of LDS.
+constexpr uint32_t x32_tiles(uint32_t nt) { return (nt + (nt + 3u) / 4u - 1u) / ((nt + 3u) / 4u); }
 
 static void launch_conv_bf16(hipStream_t s, const ConvDev& c, uint32_t z) {
     const uint32_t nt = c.n_pad / 16u, tiles_x = (c.w_ + TW - 1) / TW, bt = x32_tiles(nt);
-    if (bt) {  // two rows per wave, 8 rows per block
-        const dim3 grid(tiles_x * (nt / bt), (c.h + 7) / 8, z);
-        switch (bt) {
-            case 1: hipLaunchKernelGGL((k_conv_bf16x32<1, 2>), grid, dim3(256), 0, s, c); break;
-            case 2: hipLaunchKernelGGL((k_conv_bf16x32<2, 2>), grid, dim3(256), 0, s, c); break;
-            case 3: hipLaunchKernelGGL((k_conv_bf16x32<3, 2>), grid, dim3(256), 0, s, c); break;
-            default: hipLaunchKernelGGL((k_conv_bf16x32<4, 2>), grid, dim3(256), 0, s, c); break;
-        }
-        return;
-    }
-    const dim3 grid(tiles_x, (c.h + 3) / 4, z);  // one row per wave
-    switch (nt) {
-#define PFXD_CONV(n) \
-    case n: hipLaunchKernelGGL((k_conv_bf16<n, 1>), grid, dim3(256), 0, s, c); break;
-        PFXD_CONV(5) PFXD_CONV(6) PFXD_CONV(7) PFXD_CONV(8) PFXD_CONV(9) PFXD_CONV(10) PFXD_CONV(11)
-        PFXD_CONV(12) PFXD_CONV(13) PFXD_CONV(14)
-#undef PFXD_CONV
-        default: break;  // the host refuses other widths
+    // two rows per wave, 8 rows per block
+    const dim3 grid(tiles_x * ((nt + bt - 1) / bt), (c.h + 7) / 8, z);
+    switch (bt) {
+        case 1: hipLaunchKernelGGL((k_conv_bf16x32<1, 2>), grid, dim3(256), 0, s, c); break;
+        case 2: hipLaunchKernelGGL((k_conv_bf16x32<2, 2>), grid, dim3(256), 0, s, c); break;
+        case 3: hipLaunchKernelGGL((k_conv_bf16x32<3, 2>), grid, dim3(256), 0, s, c); break;
+        default: hipLaunchKernelGGL((k_conv_bf16x32<4, 2>), grid, dim3(256), 0, s, c); break;
     }
 }
 
