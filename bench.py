@@ -50,12 +50,13 @@ WIDTH, HEIGHT = 3840, 2160
 ADAPTIVE = Adaptive(threshold=0.01, min_samples=16, max_samples=64, growth=1.5)
 BOUNCES = Bounces.deep(6)
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes of the closest-hit kernel: per node visit the six box
-# planes and four child references of a 4-wide node (112 of its 128 bytes),
+# algorithmic bytes of the closest-hit kernel: per node visit the node
+# record it fetches (Trace.node_bytes: a 64-byte quantized node, or the six
+# box planes and four child references of a 128-byte FP32 node, 112 bytes),
 # per triangle test one 48-byte edge-form triangle, and per ray its queue
 # entry (4), origin and direction (32), hit record (16) and hit instance
 # record (4)
-NODE_BYTES, PRIM_BYTES, RAY_IO_BYTES = 112, 48, 4 + 32 + 16 + 4
+PRIM_BYTES, RAY_IO_BYTES = 48, 4 + 32 + 16 + 4
 # the reference's records (crates/trace/src/bvh.rs:5-13 Node, gpu.rs:17-26 GpuTriangle)
 REF_NODE_BYTES, REF_PRIM_BYTES = 48, 96
 # per-tile fallback of the learned denoiser to the repo denoiser: on the 4K
@@ -89,7 +90,7 @@ COUNTER_PASSES = {
 # device-to-device copy
 CALIBRATION = ("import torch; a=torch.ones(2**28, device='cuda'); b=torch.empty_like(a); "
                "torch.cuda.synchronize(); b.copy_(a); torch.cuda.synchronize()")
-COUNTED_KERNEL = "k_extend<false, false, true>"
+COUNTED_KERNEL = "k_extend<false, false, true, true>"
 
 
 def kernel_hash() -> str:
@@ -141,7 +142,7 @@ def summarise_counters(out_dir: Path) -> dict:
     fb, wb = fetch[0] * 1024 * ff / rays, write[0] * 1024 * wf / rays
     cycles = stall["SQ_WAVE_CYCLES"][0]
     return {
-        "kernel": f"pfxh::{kernel} (closest hit, nearest first)",
+        "kernel": f"pfxh::{kernel} (closest hit, nearest first, quantized nodes)",
         "workload": "terrain still 3840x2160, adaptive<=64 spp, 6 bounces (bench.py defaults, one step)",
         "kernel_sources_sha16": kernel_hash(),
         "dispatches": fetch[1],
@@ -508,7 +509,8 @@ def main() -> int:
     rays = s["closest_rays"] + s["shadow_rays"]
     mrays = rays / total / 1e6
     # roofline of the closest-hit kernel
-    bytes_per_ray = per_ray["nodes_per_ray"] * NODE_BYTES + per_ray["prims_per_ray"] * PRIM_BYTES + RAY_IO_BYTES
+    node_bytes = trace.node_bytes()
+    bytes_per_ray = per_ray["nodes_per_ray"] * node_bytes + per_ray["prims_per_ray"] * PRIM_BYTES + RAY_IO_BYTES
     launches = max(s["launches"], 1)
     rays_per_launch = s["closest_rays"] / launches
     avg_launch_ms = s["trace_ms"] / launches
@@ -587,6 +589,7 @@ def main() -> int:
             "traffic": traffic,
             "valu": valu,
             "bytes_per_ray": round(bytes_per_ray, 1),
+            "node_bytes": node_bytes,
             "reference_layout_bytes_per_ray": round(ref_bytes_per_ray, 1),
             "reference_layout_gbs": round(ref_bytes_per_ray * rays_per_launch / (avg_launch_ms * 1e-3) / 1e9, 1),
             "rays_per_launch": round(rays_per_launch, 1),

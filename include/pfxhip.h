@@ -339,6 +339,13 @@ pfxh_status pfxh_trace_set_environment(pfxh_trace* trace, const pfxh_environment
 uint32_t pfxh_trace_samples(const pfxh_trace* trace);
 /* Trace::gpu_bytes (gpu.rs:2360) */
 uint64_t pfxh_trace_gpu_bytes(const pfxh_trace* trace);
+/* Bytes one node visit of the trace's closest-hit walk fetches: 64 when its
+ * default walks read quantized nodes (one byte per child box plane on a
+ * power-of-two grid, boxes rounded outward), 112 when they read 4-wide FP32
+ * nodes (six box planes and four child references of a 128-byte node: the
+ * left-first walk, or a scene whose boxes do not quantize); 0 for a null
+ * trace. No counterpart in the reference: a measurement aid. */
+uint32_t pfxh_trace_node_bytes(const pfxh_trace* trace);
 /* Closest hits of `count` caller rays (origins and directions each hold
  * 3*count floats: x, y, z per ray, in world space) against the trace's scene, with the trace's triangle test:
  * primitive[k] is the caller's triangle id as in pfxh_probe_hit (UINT32_MAX

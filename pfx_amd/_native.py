@@ -307,6 +307,7 @@ SIGNATURES = {
     "pfxh_trace_restore": (C.c_int, [_P, _S, _P, _Z, _S, _Z]),
     "pfxh_trace_samples": (_U, [_P]),
     "pfxh_trace_gpu_bytes": (C.c_uint64, [_P]),
+    "pfxh_trace_node_bytes": (_U, [_P]),
     "pfxh_trace_probe": (C.c_int, [_P, _P, _S, _Z]),
     "pfxh_trace_stats": (C.c_int, [_P, _P]),
     "pfxh_trace_reset_stats": (None, [_P]),

@@ -34,6 +34,7 @@ struct pfxh_batch {
     uint32_t capacity = 0, max_pixels = 0, total_bounces = 0, cus = 256;
     bool left_first = false;  // closest-hit walks in the reference's left-first order
     bool sobol = false;       // the stills use the Sobol sampler
+    bool quantized = true;    // every still's nodes are quantized (QNode4)
     SceneDev* scenes = nullptr;
     AccumDev* accums = nullptr;
     WavePlan* waves = nullptr;
@@ -143,13 +144,13 @@ void record_wave(pfxh_batch* b, const pfxh_batch_tuning& t) {
         WaveDev Wb = W;
         Wb.q_next = outq;
         launch_roll(s, B, Wb, 0, bounce);
-        launch_extend_batch(s, B, Wb, queue, trace_grid, b->left_first);
+        launch_extend_batch(s, B, Wb, queue, trace_grid, b->left_first, b->quantized);
         launch_roll(s, B, Wb, 1, bounce);
         // only paths that hit, or whose still has a sky, need shading
         launch_hits_batch(s, B, Wb, queue, b->qh, flat_grid);
         launch_shade_batch(s, B, Wb, b->qh, bounce, shade_grid, Wb.counters + C_HITS, b->sobol);
         launch_roll(s, B, Wb, 2, bounce);
-        launch_shadow_batch(s, B, Wb, trace_grid, b->left_first);
+        launch_shadow_batch(s, B, Wb, trace_grid, b->left_first, b->quantized);
         launch_fold_batch(s, Wb, flat_grid);
         queue = outq;
         outq = (outq == b->qa) ? b->qb : b->qa;
@@ -255,6 +256,7 @@ pfxh_status pfxh_batch_new(const pfxh_batch_still* stills, uint32_t count, const
             b->seeds.push_back(stills[i].seed);
             b->max_pixels = std::max(b->max_pixels, stills[i].width * stills[i].height);
             entries = std::max(entries, t->S.entries);
+            b->quantized = b->quantized && t->S.qnodes != nullptr;
             work += (uint64_t)stills[i].width * stills[i].height * stills[i].spp;
         }
         b->total_bounces = detail->bounces.total;

@@ -33,6 +33,22 @@ struct alignas(16) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 is 128 bytes");
 
+// A Node4 with its child boxes quantized outward to one byte per plane (64
+// bytes, half a Node4; same index and child references as its Node4):
+// plane value = fmaf(q, scale[a], origin[a]) in FP32 with origin the
+// children's least lo and scale a power of two, each byte chosen so the
+// decoded box contains the exact one. The default walks (whose box tests
+// are conservative anyway) read these; the left-first walk keeps Node4s.
+// q_lo[a] and q_hi[a] hold byte c for child c.
+struct alignas(16) QNode4 {
+    float origin[3];
+    float scale[3];
+    uint32_t q_lo[3];
+    uint32_t q_hi[3];
+    uint32_t ref[4];
+};
+static_assert(sizeof(QNode4) == 64, "QNode4 is 64 bytes");
+
 // Positions in edge form: a, e1 = b - a, e2 = c - a (the same subtractions
 // the Moller-Trumbore test makes, done once on the host). A scene built for
 // the watertight test keeps the vertices themselves instead (e1 holds b, e2

@@ -16,6 +16,7 @@ constexpr uint32_t GEO_TAME = 1u, GEO_WATERTIGHT = 2u, GEO_LEFT_FIRST = 4u;
 
 struct SceneDev {
     const Node4* nodes;
+    const QNode4* qnodes;    // the nodes quantized, or null (the default walks then read nodes)
     const TriPos* tris;
     const TriAttr* attrs;
     const Group* groups;
@@ -168,13 +169,15 @@ void launch_zero(hipStream_t s, uint32_t* p, uint32_t n);
 void launch_camera_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid);
 void launch_roll(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t phase, uint32_t bounce);
 void launch_extend_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t grid,
-                         bool left_first);
+                         bool left_first,
+                         bool quantized);
 void launch_hits_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t* hitq,
                        uint32_t grid);
 // Shades the wave's queue (length on the device), or queue[0, *count) when count is given.
 void launch_shade_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t bounce,
                         uint32_t grid, const uint32_t* count, bool sobol);
-void launch_shadow_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid, bool left_first);
+void launch_shadow_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid, bool left_first,
+                         bool quantized);
 void launch_fold_batch(hipStream_t s, const WaveDev& W, uint32_t grid);
 void launch_finish_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid);
 void launch_next_wave(hipStream_t s, const BatchDev& B);

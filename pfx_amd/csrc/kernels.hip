@@ -292,6 +292,9 @@ DEV void order2(uint32_t& ka, uint32_t& ra, uint32_t& kb, uint32_t& rb) {
 // (22.6 against 23.2), which is also where any-hit may stop.
 // WIDE: a fast ray tests boxes with slab_wide (the default walks), else with
 // the exact quotients (left-first walks, the reference's arithmetic).
+template <bool NEAR>
+DEV bool choose4(bool h0, bool h1, bool h2, bool h3, float n0, float n1, float n2, float n3, uint4 rf, Stack& st,
+                 uint32_t& cur);
 template <bool NEAR, bool WIDE, class G>
 DEV bool visit4(const G& S, uint32_t index, const Ray& R, float t, Stack& st, uint32_t& cur) {
     const float4* np = reinterpret_cast<const float4*>(S.nodes + index);
@@ -315,6 +318,13 @@ DEV bool visit4(const G& S, uint32_t index, const Ray& R, float t, Stack& st, ui
         h2 = rf.z != REF_NONE && slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, R, t, n2);
         h3 = rf.w != REF_NONE && slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, R, t, n3);
     }
+    return choose4<NEAR>(h0, h1, h2, h3, n0, n1, n2, n3, rf, st, cur);
+}
+// The children a visit continues with and defers, from their box tests (hit
+// flags and entry distances) -- see visit4.
+template <bool NEAR>
+DEV bool choose4(bool h0, bool h1, bool h2, bool h3, float n0, float n1, float n2, float n3, uint4 rf, Stack& st,
+                 uint32_t& cur) {
     if (!NEAR) {
         if (h3 && (h0 || h1 || h2)) st.push(rf.w, n3);
         if (h2 && (h0 || h1)) st.push(rf.z, n2);
@@ -337,6 +347,35 @@ DEV bool visit4(const G& S, uint32_t index, const Ray& R, float t, Stack& st, ui
     if (k1 != MISSED) st.push(r1, ubits(k1));
     cur = r0;
     return k0 != MISSED;
+}
+
+// A plane value of a QNode4: byte c of the plane's word, decoded.
+DEV float dq(uint32_t word, uint32_t c, float scale, float origin) {
+    return __builtin_fmaf((float)((word >> (8u * c)) & 0xffu), scale, origin);
+}
+// The visit of the default walks on a scene with quantized nodes: each
+// child's box is decoded from its bytes just before its test (a box that
+// contains the exact one: the test stays conservative, the walk keeps every
+// triangle the exact walk would test) and tested as visit4 tests a fast
+// ray's boxes, or with the exact quotients for another ray. The node takes
+// four 16-byte loads and 16 registers, against seven and 28 for a Node4.
+template <bool NEAR, class G>
+DEV bool visit4q(const G& S, uint32_t index, const Ray& R, float t, Stack& st, uint32_t& cur) {
+    const uint4* np = reinterpret_cast<const uint4*>(S.qnodes + index);
+    const uint4 a = np[0], b = np[1], c = np[2], rf = np[3];
+    const float ox = __uint_as_float(a.x), oy = __uint_as_float(a.y), oz = __uint_as_float(a.z);
+    const float sx = __uint_as_float(a.w), sy = __uint_as_float(b.x), sz = __uint_as_float(b.y);
+    // q_lo x, y, z = b.z, b.w, c.x; q_hi x, y, z = c.y, c.z, c.w
+    float n[4];
+    bool h[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        const float lx = dq(b.z, k, sx, ox), ly = dq(b.w, k, sy, oy), lz = dq(c.x, k, sz, oz);
+        const float hx = dq(c.y, k, sx, ox), hy = dq(c.z, k, sy, oy), hz = dq(c.w, k, sz, oz);
+        h[k] = R.fast ? slab_wide(lx, ly, lz, hx, hy, hz, R, t, n[k]) : slab(lx, ly, lz, hx, hy, hz, R, t, n[k]);
+    }
+    return choose4<NEAR>(h[0], h[1] && rf.y != REF_NONE, h[2] && rf.z != REF_NONE, h[3] && rf.w != REF_NONE, n[0],
+                         n[1], n[2], n[3], rf, st, cur);
 }
 
 // The traversal of one ray as a resumable state: closest hit
@@ -578,6 +617,8 @@ struct Walk {
             if (COUNT) nodes_ctr++;
             if ((S.geo_flags & GEO_LEFT_FIRST) != 0u) {
                 if (visit4<false, false>(S, cur, R, h.t, st, cur)) return true;
+            } else if (S.qnodes) {
+                if (visit4q<!ANY>(S, cur, R, h.t, st, cur)) return true;
             } else if (ANY) {
                 if (visit4<false, true>(S, cur, R, h.t, st, cur)) return true;
             } else if (visit4<true, true>(S, cur, R, h.t, st, cur)) {
@@ -612,12 +653,13 @@ DEV Closest trace_closest(const G& S, v3 o, v3 d, float tmax, Stack& st) {
 // with the uniform SceneDev itself.
 struct Geo {
     const Node4* nodes;
+    const QNode4* qnodes;
     const TriPos* tris;
     const Group* groups;
     RootEntry root;
     uint32_t geo_flags;
 };
-DEV Geo geo_of(const SceneDev& S) { return Geo{S.nodes, S.tris, S.groups, S.root, S.geo_flags}; }
+DEV Geo geo_of(const SceneDev& S) { return Geo{S.nodes, S.qnodes, S.tris, S.groups, S.root, S.geo_flags}; }
 template <bool BATCH>
 DEV decltype(auto) walk_scene(const SceneDev& S, const Geo& g) {
     if constexpr (BATCH)
@@ -1395,7 +1437,8 @@ struct WaveCount {
     }
 };
 
-template <bool COUNT, bool BATCH, bool NEAR>
+// Q: the scene's nodes are quantized (QNode4) and the walk is nearest first.
+template <bool COUNT, bool BATCH, bool NEAR, bool Q = false>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(SceneDev S, const SceneDev* scenes, WaveDev W, const uint32_t* queue, uint32_t n) {
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
@@ -1436,7 +1479,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))
                     nodes++;
                     boxes += child_boxes(walk_scene<BATCH>(S, g), w.cur);
                 }
-                if (!visit4<NEAR, NEAR>(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) {
+                bool more;
+                if constexpr (Q)
+                    more = visit4q<true>(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur);
+                else
+                    more = visit4<NEAR, NEAR>(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur);
+                if (!more && !w.next(st)) {
                     store_closest(W, ray, w.h, w.record(walk_scene<BATCH>(S, g)));
                     live = false;
                 }
@@ -1773,7 +1821,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // Seven waves per SIMD for a single trace (72 registers, none spilled, and an
 // 11-entry LDS stack so 14 blocks fit a CU): 3% faster than six on the 4K
 // terrain. A batch walk holds its still's scene in registers and keeps six.
-template <bool COUNT, bool BATCH, bool WIDE>
+// Q: the scene's nodes are quantized (QNode4), with the default walk.
+template <bool COUNT, bool BATCH, bool WIDE, bool Q = false>
 __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(BATCH ? 6 : 7))) void k_shadow(SceneDev S, const SceneDev* scenes, WaveDev W, uint32_t n) {
     __shared__ uint2 lds[STACK_SHORT * TRACE_BLOCK];
     uint32_t gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
@@ -1803,7 +1852,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(BAT
         if (inner_round(in, out)) {
             if (in) {
                 if (COUNT) nodes++;
-                if (!visit4<false, WIDE>(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur) && !w.next(st)) live = false;
+                bool more;
+                if constexpr (Q)
+                    more = visit4q<false>(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur);
+                else
+                    more = visit4<false, WIDE>(walk_scene<BATCH>(S, g), w.cur, w.R, w.h.t, st, w.cur);
+                if (!more && !w.next(st)) live = false;
             }
         } else if (out) {
             bool go = true;
@@ -2212,15 +2266,20 @@ void launch_extend(hipStream_t s, const SceneDev& S, const WaveDev& W, const uin
     if (!n) return;
     uint32_t blocks = std::min(blocks_for(n, TRACE_BLOCK), grid);
     const SceneDev* none = nullptr;
-    bool left = (S.geo_flags & GEO_LEFT_FIRST) != 0u;
+    bool left = (S.geo_flags & GEO_LEFT_FIRST) != 0u, q = !left && S.qnodes;
+    const dim3 gd(blocks), block(TRACE_BLOCK);
     if (count && left)
-        hipLaunchKernelGGL((k_extend<true, false, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, queue, n);
+        hipLaunchKernelGGL((k_extend<true, false, false>), gd, block, 0, s, S, none, W, queue, n);
+    else if (count && q)
+        hipLaunchKernelGGL((k_extend<true, false, true, true>), gd, block, 0, s, S, none, W, queue, n);
     else if (count)
-        hipLaunchKernelGGL((k_extend<true, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, queue, n);
+        hipLaunchKernelGGL((k_extend<true, false, true>), gd, block, 0, s, S, none, W, queue, n);
     else if (left)
-        hipLaunchKernelGGL((k_extend<false, false, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, queue, n);
+        hipLaunchKernelGGL((k_extend<false, false, false>), gd, block, 0, s, S, none, W, queue, n);
+    else if (q)
+        hipLaunchKernelGGL((k_extend<false, false, true, true>), gd, block, 0, s, S, none, W, queue, n);
     else
-        hipLaunchKernelGGL((k_extend<false, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, queue, n);
+        hipLaunchKernelGGL((k_extend<false, false, true>), gd, block, 0, s, S, none, W, queue, n);
 }
 void launch_hits(hipStream_t s, const WaveDev& W, const uint32_t* queue, uint32_t n, uint32_t* hitq) {
     if (!n) return;
@@ -2243,15 +2302,20 @@ void launch_shadow(hipStream_t s, const SceneDev& S, const WaveDev& W, uint32_t 
     if (!n) return;
     uint32_t blocks = std::min(blocks_for(n, TRACE_BLOCK), grid);
     const SceneDev* none = nullptr;
-    bool left = (S.geo_flags & GEO_LEFT_FIRST) != 0u;
+    bool left = (S.geo_flags & GEO_LEFT_FIRST) != 0u, q = !left && S.qnodes;
+    const dim3 gd(blocks), block(TRACE_BLOCK);
     if (count && left)
-        hipLaunchKernelGGL((k_shadow<true, false, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, n);
+        hipLaunchKernelGGL((k_shadow<true, false, false>), gd, block, 0, s, S, none, W, n);
+    else if (count && q)
+        hipLaunchKernelGGL((k_shadow<true, false, true, true>), gd, block, 0, s, S, none, W, n);
     else if (count)
-        hipLaunchKernelGGL((k_shadow<true, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, n);
+        hipLaunchKernelGGL((k_shadow<true, false, true>), gd, block, 0, s, S, none, W, n);
     else if (left)
-        hipLaunchKernelGGL((k_shadow<false, false, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, n);
+        hipLaunchKernelGGL((k_shadow<false, false, false>), gd, block, 0, s, S, none, W, n);
+    else if (q)
+        hipLaunchKernelGGL((k_shadow<false, false, true, true>), gd, block, 0, s, S, none, W, n);
     else
-        hipLaunchKernelGGL((k_shadow<false, false, true>), dim3(blocks), dim3(TRACE_BLOCK), 0, s, S, none, W, n);
+        hipLaunchKernelGGL((k_shadow<false, false, true>), gd, block, 0, s, S, none, W, n);
 }
 void launch_fold(hipStream_t s, const WaveDev& W, uint32_t n) {
     if (!n) return;
@@ -2334,10 +2398,13 @@ void launch_roll(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t ph
     hipLaunchKernelGGL(k_roll, dim3(1), dim3(64), 0, s, B, W.counters, phase, bounce);
 }
 void launch_extend_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, const uint32_t* queue, uint32_t grid,
-                         bool left_first) {
+                         bool left_first, bool quantized) {
     SceneDev none{};
     if (left_first)
         hipLaunchKernelGGL((k_extend<false, true, false>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W, queue, 0u);
+    else if (quantized)
+        hipLaunchKernelGGL((k_extend<false, true, true, true>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W,
+                           queue, 0u);
     else
         hipLaunchKernelGGL((k_extend<false, true, true>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W, queue, 0u);
 }
@@ -2355,10 +2422,14 @@ void launch_shade_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, cons
         hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(256), 0, s, none, B.scenes, W, queue, 0u, bounce,
                            count);
 }
-void launch_shadow_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid, bool left_first) {
+void launch_shadow_batch(hipStream_t s, const BatchDev& B, const WaveDev& W, uint32_t grid, bool left_first,
+                         bool quantized) {
     SceneDev none{};
     if (left_first)
         hipLaunchKernelGGL((k_shadow<false, true, false>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W, 0u);
+    else if (quantized)
+        hipLaunchKernelGGL((k_shadow<false, true, true, true>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W,
+                           0u);
     else
         hipLaunchKernelGGL((k_shadow<false, true, true>), dim3(grid), dim3(TRACE_BLOCK), 0, s, none, B.scenes, W, 0u);
 }

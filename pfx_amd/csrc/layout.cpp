@@ -1548,7 +1548,75 @@ std::string build_scene(const pfxh_scene& sc, HostScene& out, bool emitter_sampl
         out.attrs.resize(1);
     }
     if (out.groups.empty()) out.groups.resize(1);
+    quantize_nodes(out);
     return "";
+}
+
+// ---- quantized nodes ---------------------------------------------------------------
+// A decoded plane value the walks can take: finite, not subnormal, and for a
+// scene whose bounds are tame (0 or of magnitude in [2^-72, 2^60], which
+// the conservative box test's error bound assumes) tame itself.
+static bool decodable(float v, bool tame) {
+    if (!std::isfinite(v) || (v != 0.0f && std::fabs(v) < FLT_MIN)) return false;
+    return !tame || v == 0.0f || (std::fabs(v) >= 0x1p-72f && std::fabs(v) <= 0x1p60f);
+}
+
+// One Node4 as a QNode4, per axis: origin = the least lo of the node's
+// children, scale = the least power of two with 255 scale at least their
+// extent plus four units in the last place of the largest bound (room for
+// the rounding of the decode), then per child the largest q_lo whose decoded
+// value fmaf(q_lo, scale, origin) -- the FP32 fused multiply-add the device
+// evaluates, correctly rounded on both -- is at most lo, and the least q_hi
+// whose decoded value is at least hi. False when some value cannot be
+// bounded or decoded so.
+static bool quantize_node(const Node4& n, bool tame, QNode4& q) {
+    memset(&q, 0, sizeof(q));
+    for (int c = 0; c < 4; ++c) q.ref[c] = n.ref[c];
+    const float* lo[3] = {n.lo_x, n.lo_y, n.lo_z};
+    const float* hi[3] = {n.hi_x, n.hi_y, n.hi_z};
+    for (int a = 0; a < 3; ++a) {
+        float least = INFINITY, most = -INFINITY;
+        for (int c = 0; c < 4; ++c) {
+            if (n.ref[c] == REF_NONE) continue;
+            if (!std::isfinite(lo[a][c]) || !std::isfinite(hi[a][c]) || lo[a][c] > hi[a][c]) return false;
+            least = std::fmin(least, lo[a][c]);
+            most = std::fmax(most, hi[a][c]);
+        }
+        if (!(least <= most)) return false;  // no child
+        const float top = std::fmax(std::fabs(least), std::fabs(most));
+        const double ulp = top > 0.0f ? std::ldexp(1.0, std::ilogb(top) - 23) : std::ldexp(1.0, -149);
+        const double need = ((double)most - (double)least + 4.0 * ulp) / 255.0;
+        int e = 0;
+        const double f = std::frexp(need, &e);  // need = f 2^e, f in [0.5, 1)
+        const int k = std::max(f == 0.5 ? e - 1 : e, -126);
+        if (k > 127) return false;
+        const float scale = std::ldexp(1.0f, k), origin = least;
+        q.origin[a] = origin;
+        q.scale[a] = scale;
+        for (int c = 0; c < 4; ++c) {
+            if (n.ref[c] == REF_NONE) continue;
+            double ql = std::floor(((double)lo[a][c] - origin) / scale), qh = std::ceil(((double)hi[a][c] - origin) / scale);
+            ql = std::min(std::max(ql, 0.0), 255.0);
+            qh = std::min(std::max(qh, 0.0), 255.0);
+            while (ql > 0.0 && std::fmaf((float)ql, scale, origin) > lo[a][c]) ql -= 1.0;
+            while (qh < 255.0 && std::fmaf((float)qh, scale, origin) < hi[a][c]) qh += 1.0;
+            const float dl = std::fmaf((float)ql, scale, origin), dh = std::fmaf((float)qh, scale, origin);
+            if (dl > lo[a][c] || dh < hi[a][c] || !decodable(dl, tame) || !decodable(dh, tame)) return false;
+            q.q_lo[a] |= (uint32_t)ql << (8 * c);
+            q.q_hi[a] |= (uint32_t)qh << (8 * c);
+        }
+    }
+    return true;
+}
+
+void quantize_nodes(HostScene& out) {
+    out.qnodes.resize(out.nodes.size());
+    for (size_t i = 0; i < out.nodes.size(); ++i) {
+        if (!quantize_node(out.nodes[i], out.bounds_tame, out.qnodes[i])) {
+            out.qnodes.clear();  // the walks then read the Node4s
+            return;
+        }
+    }
 }
 
 }  // namespace pfxh

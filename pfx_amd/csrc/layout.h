@@ -31,6 +31,7 @@ struct HostScene {
     RefLevels ref;
     // device formats
     std::vector<Node4> nodes;
+    std::vector<QNode4> qnodes;  // the nodes quantized (empty when some node cannot be: see quantize_nodes)
     std::vector<TriPos> tris;
     std::vector<TriAttr> attrs;
     std::vector<uint32_t> slot_prim;   // caller primitive index per triangle slot
@@ -64,6 +65,10 @@ std::string lay_levels(const pfxh_scene& scene, RefLevels& out, std::vector<uint
 // The environment texels, black flag and sky sampling CDF of `out` from
 // `sky` (a scene's sky, or a Trace::set_environment argument).
 std::string lay_sky(const pfxh_environment& sky, HostScene& out);
+// out.qnodes from out.nodes (device_types.h QNode4); left empty when some
+// node's boxes cannot be bounded by decodable values (the walks then read
+// the Node4s).
+void quantize_nodes(HostScene& out);
 
 // tree::lay (crates/trace/src/tree.rs:220-243)
 // Rust's slice::select_nth_unstable_by as the light tree uses it, over

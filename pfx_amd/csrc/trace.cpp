@@ -86,6 +86,7 @@ void trace_setup(pfxh_trace* t, const pfxh_scene& scene, const pfxh_detail& deta
     HIP_TRY(hipHostMalloc((void**)&t->host_counters, C_COUNT * sizeof(uint32_t), hipHostMallocDefault));
     SceneDev& S = t->S;
     S.nodes = t->upload(h.nodes, "bvh nodes");
+    S.qnodes = h.qnodes.empty() ? nullptr : t->upload(h.qnodes, "quantized bvh nodes");
     S.tris = t->upload(h.tris, "triangles");
     S.attrs = t->upload(h.attrs, "triangle attributes");
     S.groups = t->upload(h.groups, "instances");
@@ -856,6 +857,10 @@ pfxh_status pfxh_trace_atrous(pfxh_trace* t, float* out, char* err, size_t err_l
 
 uint32_t pfxh_trace_samples(const pfxh_trace* t) { return t ? t->samples : 0; }
 uint64_t pfxh_trace_gpu_bytes(const pfxh_trace* t) { return t ? t->gpu_bytes : 0; }
+uint32_t pfxh_trace_node_bytes(const pfxh_trace* t) {
+    if (!t) return 0;
+    return t->S.qnodes && (t->S.geo_flags & GEO_LEFT_FIRST) == 0u ? (uint32_t)sizeof(QNode4) : 112u;
+}
 
 pfxh_status pfxh_hbm_probe(uint64_t bytes, uint32_t reps, double* gbs, char* err, size_t err_len) {
     if (!gbs || reps == 0 || bytes < (1u << 20) || bytes % 16 != 0) return PFXH_INVALID;

@@ -259,6 +259,11 @@ class Trace:
     def gpu_bytes(self) -> int:
         return int(N.lib().pfxh_trace_gpu_bytes(self._handle()))
 
+    def node_bytes(self) -> int:
+        """Bytes a closest-hit node visit fetches: 64 (quantized nodes) or
+        112 (4-wide FP32 nodes)."""
+        return int(N.lib().pfxh_trace_node_bytes(self._handle()))
+
     def cast(self, origins, directions, limit: float = 1e20):
         """Closest hits of caller rays: (caller triangle ids, UINT32_MAX on a
         miss; distances, 0 on a miss)."""
