@@ -284,8 +284,8 @@ DEV void order2(uint32_t& ka, uint32_t& ra, uint32_t& kb, uint32_t& rb) {
 // can differ from the reference's only where the reference's own answer
 // depends on its order: a hit within an ulp of a box face, where a box's
 // entry distance and the triangle's distance round apart (measured on rays
-// aimed at the vertices and edges of a closed icosphere: 175 of 46,080
-// Moller-Trumbore and 110 watertight rays keep another triangle, each
+// aimed at the vertices and edges of a closed icosphere: 171 of 46,080
+// Moller-Trumbore and 109 watertight rays keep another triangle, each
 // within 4 ulps of the oracle's distance). Nearest first cuts the closest-hit walk on the 4K
 // terrain from 25.0 to 20.3 node visits per ray; shadow rays start on a
 // surface whose own boxes are nearest, and walk fewer nodes left first

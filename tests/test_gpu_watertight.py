@@ -121,10 +121,10 @@ def test_edge_and_vertex_rays_against_the_oracle(instanced):
     170 (Moller-Trumbore) and 70 (watertight); at exact ties the same
     culling decides which of the tied triangles is kept. The near-first
     device walk may then keep another triangle of the group (world scene:
-    175 and 110 rays; instanced: 199 and 74). Its box test is also conservative (it accepts every
+    171 and 109 rays; instanced: 204 and 75, with quantized nodes). Its box test is also conservative (it accepts every
     box the exact quotients accept, and a few more within rounding), so where
     Moller-Trumbore lets a ray leak between two triangles the device may test
-    a neighbour the oracle's walk culled, and hit it (instanced: 41 rays).
+    a neighbour the oracle's walk culled, and hit it (instanced: 42 rays).
     What holds, and is
     asserted: every ray the oracle hits, the device hits (watertight: no
     misses at all), and a ray only the device hits hits the surface at the
