@@ -462,6 +462,17 @@ pfxh_status pfxh_layout_levels(const pfxh_scene* scene, pfxh_bvh_node* nodes,
                                uint32_t* node_count, uint32_t* order, uint32_t* order_count,
                                char* err, size_t err_len);
 
+/* The traversal nodes of a scene's device layout, for tests: `count`
+ * 4-wide FP32 nodes of 128 bytes (lo x, y, z[4], hi x, y, z[4], ref[4],
+ * pad[4]: the reference's binary tree collapsed two levels at a time) and,
+ * when the scene's boxes quantize (*quantized = 1), as many quantized nodes
+ * of 64 bytes (origin[3], scale[3], q_lo[3], q_hi[3], ref[4]; child c's
+ * plane is byte c of its word, value fmaf(q, scale, origin), a box that
+ * contains the exact one). Call with null arrays to learn the count. No
+ * counterpart in the reference (its nodes are binary, pfxh_layout_levels). */
+pfxh_status pfxh_layout_nodes(const pfxh_scene* scene, void* nodes, void* qnodes, uint32_t* count,
+                              uint32_t* quantized, char* err, size_t err_len);
+
 /* tree::lay (crates/trace/src/tree.rs:220-243): light rows (16 floats
  * each, `GpuLight`) for [non-treed][treed in tree order][tree nodes].
  * rows needs room for 3*count rows (lights plus fewer than 2*count nodes). */

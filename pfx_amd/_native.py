@@ -315,6 +315,7 @@ SIGNATURES = {
     "pfxh_trace_free": (None, [_P]),
     "pfxh_layout_bvh": (C.c_int, [_P, _U, _P, C.POINTER(_U), _P, _S, _Z]),
     "pfxh_layout_levels": (C.c_int, [_P, _P, C.POINTER(_U), _P, C.POINTER(_U), _S, _Z]),
+    "pfxh_layout_nodes": (C.c_int, [_P, _P, _P, C.POINTER(_U), C.POINTER(_U), _S, _Z]),
     "pfxh_layout_lights": (C.c_int, [_P, _U, _P, C.POINTER(_U), C.POINTER(_U), C.POINTER(_U), _S, _Z]),
     "pfxh_layout_select": (C.c_int, [_P, _P, _Z, _Z, C.c_int, C.POINTER(_Z)]),
     "pfxh_hbm_probe": (C.c_int, [C.c_uint64, _U, C.POINTER(C.c_double), _S, _Z]),

@@ -1008,6 +1008,27 @@ pfxh_status pfxh_layout_bvh(const pfxh_triangle* tris, uint32_t count, pfxh_bvh_
     return PFXH_OK;
 }
 
+pfxh_status pfxh_layout_nodes(const pfxh_scene* scene, void* nodes, void* qnodes, uint32_t* count,
+                              uint32_t* quantized, char* err, size_t err_len) {
+    if (!scene || !count || !quantized) return PFXH_INVALID;
+    HostScene h;
+    std::string e;
+    try {
+        e = build_scene(*scene, h);
+    } catch (const std::bad_alloc&) {
+        e = "host memory is exhausted";
+    }
+    if (!e.empty()) {
+        put_err(err, err_len, e);
+        return PFXH_SETUP;
+    }
+    *count = (uint32_t)h.nodes.size();
+    *quantized = h.qnodes.empty() ? 0u : 1u;
+    if (nodes) memcpy(nodes, h.nodes.data(), h.nodes.size() * sizeof(Node4));
+    if (qnodes && !h.qnodes.empty()) memcpy(qnodes, h.qnodes.data(), h.qnodes.size() * sizeof(QNode4));
+    return PFXH_OK;
+}
+
 pfxh_status pfxh_layout_levels(const pfxh_scene* scene, pfxh_bvh_node* nodes, uint32_t* node_count, uint32_t* order,
                                uint32_t* order_count, char* err, size_t err_len) {
     if (!scene || !node_count || !order_count) return PFXH_INVALID;
