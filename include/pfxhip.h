@@ -732,8 +732,10 @@ pfxh_status pfxh_learner_set_loss(pfxh_learner* learner, uint32_t mode);
  * the frames run through the network as one batch and the denoised image is
  * the mean of their outputs, each mirrored back, in linear radiance. The
  * network was trained under these symmetries, so the mean keeps its bias
- * and averages out part of its error; the cost is the passes' convolutions.
- * Applies to pfxh_learned_denoise and the trace entry points; other values
+ * and averages out part of its error; the cost is the passes' convolutions
+ * and their activations (the inference frame holds `passes` frames: about
+ * 7 GB per pass at 3840x2160). Applies to pfxh_learned_denoise and the
+ * trace entry points; other values
  * give PFXH_INVALID. No counterpart in the reference (it has no trained
  * denoiser). */
 pfxh_status pfxh_learner_set_passes(pfxh_learner* learner, uint32_t passes);
