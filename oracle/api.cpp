@@ -87,6 +87,8 @@ uint32_t or_lights(const pfxh_light* lights, uint32_t count, float* rows, uint32
     *tree = l.tree;
     return (uint32_t)l.rows.size();
 }
+// tree_lay's split: 1 = stable sort by centroid, 0 = Rust's selection
+void or_set_tree_split(int sorted) { set_tree_split_by_sort(sorted != 0); }
 // lights.rs pack / falloff / cone
 void or_light_pack(const pfxh_light* l, float* out16) {
     GLight g = light_pack(*l);

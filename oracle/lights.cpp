@@ -382,6 +382,12 @@ static bool total_less(float a, float b) {
     y ^= (int32_t)((uint32_t)(y >> 31) >> 1);
     return x < y;
 }
+// When set, the split sorts the whole slice stably by centroid instead:
+// another valid selection, which tests use to show that the leaf order
+// depends only on which emitters fall on each side, not on the pivots.
+static bool g_split_by_sort = false;
+void set_tree_split_by_sort(bool on) { g_split_by_sort = on; }
+
 // tree.rs:263-293; the split around `middle` is Rust's
 // select_nth_unstable_by on the centroid (select_nth.h), so each side keeps
 // the order that selection leaves, which becomes the leaf order.
@@ -393,10 +399,15 @@ static void grow(Emitter* emitters, size_t count, size_t start, size_t base, siz
     int dim;
     Measure measure = should_split(emitters, count, &split, &middle, &dim);
     if (split) {
-        if (dim >= 0)
-            rsel::select_nth_unstable_by(emitters, count, middle, [dim](const Emitter& l, const Emitter& r) {
+        if (dim >= 0) {
+            auto less = [dim](const Emitter& l, const Emitter& r) {
                 return total_less(l.centroid[dim], r.centroid[dim]);
-            });
+            };
+            if (g_split_by_sort)
+                std::stable_sort(emitters, emitters + count, less);
+            else
+                rsel::select_nth_unstable_by(emitters, count, middle, less);
+        }
         grow(emitters, middle, start, base, root, nodes);
         size_t right_at = root + nodes.size();
         grow(emitters + middle, count - middle, start + middle, base, root, nodes);

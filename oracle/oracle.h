@@ -19,8 +19,12 @@
 //   light tree build .... pinned: tree.rs:433-492 invariants; the split
 //                         restates Rust core's select_nth_unstable_by
 //                         (select_nth.h), pinned by its selection
-//                         properties; its pivot order beyond 16 elements
-//                         has no Rust output to compare: parity unpinned
+//                         properties; with distinct centroids the leaf
+//                         order equals a stable-sort split's (so it does
+//                         not hang on the pivots); pivot order beyond 16
+//                         elements, which places tied centroids and
+//                         orders the node sums, has no Rust output to
+//                         compare: parity unpinned there
 //   light tree pick ..... pinned: enumeration over the uniform (each
 //                         light's share == its pdf; pdfs + failed share
 //                         == 1) and a chi-squared test
@@ -120,6 +124,7 @@ struct LaidLights {
     uint32_t tree = 0;
 };
 LaidLights tree_lay(const pfxh_light* lights, size_t count);
+void set_tree_split_by_sort(bool on);
 
 struct EmitterList {
     std::vector<GEmitter> entries;

@@ -71,6 +71,7 @@ def lib() -> C.CDLL:
             "or_triangle_hit": (_F, [_P, _P, _P, _F]),
             "or_levels": (C.c_int, [_P, _P, C.POINTER(_U), _P, C.POINTER(_U), _P, C.c_char_p, C.c_size_t]),
             "or_lights": (_U, [_P, _U, _P, C.POINTER(_U), C.POINTER(_U)]),
+            "or_set_tree_split": (None, [C.c_int]),
             "or_light_pack": (None, [_P, _P]),
             "or_light_falloff": (_F, [_P, _F]),
             "or_light_cone": (_F, [_P, _P]),

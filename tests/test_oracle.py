@@ -409,3 +409,33 @@ def test_sky_cdf_sums_to_one():
     O.lib().or_sky_cdf(None, 0, 0, 256, 128, rows.ctypes.data, cols.ctypes.data)
     assert np.allclose(rows, (np.arange(128) + 1) / 128.0)
     assert np.allclose(cols[:256], (np.arange(256) + 1) / 256.0)
+
+
+@pytest.mark.parametrize("seed,count", [(1, 40), (2, 200), (3, 1024)])
+def test_leaf_order_does_not_depend_on_the_selection_pivots(seed, count):
+    """tree.rs:263-293 recurses to one emitter per leaf, so with distinct
+    centroids each split's selection property (pinned in test_select.py)
+    already fixes which emitters lie on each side; the pivots Rust's
+    select_nth_unstable_by picks only reorder the emitters inside a side,
+    which the recursion sorts again. Building the tree with a stable sort as
+    the selection must therefore give the same leaf order: the treed rows
+    byte-identical. The node rows sum the same measures in another order,
+    so they agree to rounding."""
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-50.0, 50.0, (count, 3)).astype(np.float32)
+    col = rng.uniform(0.1, 1.0, (count, 3)).astype(np.float32)
+    lights = np.concatenate([point_light(tuple(p), tuple(c), float(rng.uniform(1.0, 40.0)), 30.0)
+                             for p, c in zip(pos, col)])
+    rows, looped, tree = lay(lights)
+    try:
+        O.lib().or_set_tree_split(1)
+        by_sort, looped2, tree2 = lay(lights)
+    finally:
+        O.lib().or_set_tree_split(0)
+    assert (looped2, tree2) == (looped, tree) and tree == count and len(by_sort) == len(rows)
+    leaves = slice(looped, looped + tree)
+    assert rows[leaves].tobytes() == by_sort[leaves].tobytes()
+    nodes = slice(looped + tree, None)
+    np.testing.assert_allclose(by_sort[nodes], rows[nodes], rtol=1e-5, atol=1e-6)
+    # and the default build is the product's, byte for byte
+    assert product_lay(lights)[0].tobytes() == rows.tobytes()
